@@ -1,0 +1,165 @@
+"""Benchmark of the statevector hot path on one MI355X.
+
+Workload: random U3 + CNOT-brickwork circuit, 30 qubits, depth 40, seed 102
+(1780 gates), complex64 state resident in HBM (8 GiB), tiled multi-gate
+passes.  One step = one execution of the whole circuit on the resident state.
+
+Prints ONE JSON line: amp-updates/s (logical gates x 2^n per second), the
+roofline of the dominant kernel (the tile pass, timed with HIP events on its
+own stream inside the timed region), the infidelity and <Z_i Z_i+1> error of
+the complex64 result against the complex128 GPU run of the same circuit, the
+plain per-gate GPU rate, and the FP64 CPU oracle timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(seconds: float, n: int = 26):
+    """FP64 oracle (C + OpenMP, all host threads), gate by gate like the
+    reference's GateOperations path, on the same circuit family at n qubits
+    until `seconds` of work are done."""
+    from oracle import cpu_ref
+    from nqpu_amd.workloads import random_u3_brickwork
+    c = random_u3_brickwork(n, 40, 102)
+    a = cpu_ref.zero_state(n)
+    done = 0
+    t0 = time.perf_counter()
+    for g in c.gates:
+        cpu_ref.apply_gate(a, n, g)
+        done += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done * float(1 << n) / dt, "unit": "amp-updates/s", "cores": cpu_ref.num_threads(),
+            "kind": "port",
+            "sample": f"first {done} gates of the same U3+CNOT brickwork family at {n} qubits, complex128, "
+                      f"one full-state pass per gate ({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--low", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the complex128 accuracy run")
+    ap.add_argument("--no-per-gate", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the statevector path runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    from nqpu_amd import engine as E
+    from nqpu_amd.workloads import random_u3_brickwork
+
+    if E.device_count() == 0:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    n, depth = args.qubits, args.depth
+    circ = random_u3_brickwork(n, depth, 102)
+    nc = circ.native()
+    gates = len(circ.gates)
+    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low)
+    log(f"workload n={n} depth={depth} gates={gates} plan={nc.plan('c64', **run_kw)}")
+
+    sv = E.StateVector(n, "c64")
+    for w in range(args.warmup):
+        sv.run(nc, **run_kw)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    pass_ms, timed_passes, passes = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = sv.run(nc, time_passes=True, **run_kw)
+        pass_ms += st["pass_ms"]
+        timed_passes += st["timed_passes"]
+        passes = st["passes"]
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {elapsed:.3f} s, passes/step={passes}")
+    dim = 1 << n
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = gates * dim * args.steps / elapsed
+    bytes_per_pass = 2.0 * dim * 8
+    avg_pass_ms = pass_ms / max(timed_passes, 1)
+    achieved = bytes_per_pass / (avg_pass_ms * 1e-3) / 1e9
+    hbm_frac_step = passes * bytes_per_pass / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+
+    result = {
+        "metric": "amp-updates/s (30-qubit complex64 random circuit)",
+        "value": value,
+        "unit": "amp-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"random U3 + CNOT brickwork, {n} qubits, depth {depth}, seed 102, "
+                               f"complex64 state, tiled multi-gate passes",
+                   "qubits": n, "depth": depth, "gates": gates, "passes_per_step": passes,
+                   "parallelism": "single GPU"},
+        "hbm_frac_per_step": hbm_frac_step,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tile", "avg_launch_ms": avg_pass_ms,
+                     "bytes_per_launch": bytes_per_pass},
+    }
+    zz = None
+    if not args.no_check:
+        log("accuracy run: complex128 state of the same circuit")
+        ref = E.StateVector(n, "c128")
+        ref.run(nc, **run_kw)
+        sv.reset()
+        sv.run(nc, **run_kw)
+        ov = sv.overlap(ref)
+        result["infidelity_vs_c128"] = 1.0 - abs(ov) ** 2
+        errs = []
+        for i in range(n - 1):
+            s = "I" * i + "ZZ" + "I" * (n - i - 2)
+            errs.append(abs(sv.expectation_pauli(s) - ref.expectation_pauli(s)))
+        zz = max(errs)
+        result["zz_max_abs_err_vs_c128"] = zz
+        result["norm_c64"] = sv.norm()
+        ref.close()
+    if not args.no_per_gate:
+        log("plain per-gate launches on the same GPU")
+        sv.reset()
+        t1 = time.perf_counter()
+        sv.run(nc, fusion="none")
+        dt = time.perf_counter() - t1
+        result["gpu_per_gate"] = {"amp_updates_per_s": gates * dim / dt, "ms_per_step": dt * 1e3,
+                                  "passes": gates}
+        result["speedup_vs_per_gate"] = value / result["gpu_per_gate"]["amp_updates_per_s"]
+    sv.close()
+    if not args.no_cpu:
+        log(f"cpu baseline ({args.cpu_seconds} s)")
+        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

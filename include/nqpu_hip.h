@@ -1,0 +1,279 @@
+/*
+ * nqpu_hip.h — C ABI of the MI355X (gfx950) statevector engine.
+ *
+ * The first block of declarations is the reference's C interface, with the
+ * same names, argument meaning and error behaviour, so that a C caller of
+ * the reference (and the reference's own C tests) can link against
+ * libnqpu_hip.so instead.  The reference's header is generated at
+ * sdk/rust/src/infra/bindings/c_ffi.rs:1480-1620; each function cites the
+ * export it replaces.  Amplitudes live in GPU HBM; host buffers are always
+ * caller-owned.  A handle is bound to one HIP stream and is not thread-safe;
+ * distinct handles may be used from distinct threads.
+ *
+ * The second block holds the extensions the GPU engine adds (precision
+ * selection, lifted qubit cap, packed circuit upload, fused execution, FP64
+ * Pauli reductions, bulk amplitude transfer, batched execution).
+ */
+#ifndef NQPU_HIP_H
+#define NQPU_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ types */
+
+/* Opaque handles (c_ffi.rs:77-92). */
+typedef struct NqpuState NqpuState;
+typedef struct NqpuCircuit NqpuCircuit;
+
+/* Error codes (c_ffi.rs:95-113); 7..9 are GPU-specific additions. */
+typedef enum {
+  NQPU_OK = 0,
+  NQPU_INVALID_QUBIT = 1,
+  NQPU_INVALID_PARAMETER = 2,
+  NQPU_ALLOCATION_FAILED = 3,
+  NQPU_SIMULATION_FAILED = 4,
+  NQPU_NULL_POINTER = 5,
+  NQPU_INVALID_HANDLE = 6,
+  NQPU_DEVICE_ERROR = 7,          /* a HIP call or kernel launch failed */
+  NQPU_OUT_OF_DEVICE_MEMORY = 8,  /* the state does not fit in HBM */
+  NQPU_UNAVAILABLE = 9            /* no usable gfx950 device */
+} NqpuError;
+
+/* c_ffi.rs:118-123 */
+typedef struct {
+  double re;
+  double im;
+} NqpuComplex;
+
+/* c_ffi.rs:161-168 */
+typedef struct {
+  uint32_t outcome;
+  double probability;
+} NqpuMeasurement;
+
+/* c_ffi.rs:171-178 (reserved, as in the reference) */
+typedef struct {
+  uint32_t num_qubits;
+  uint64_t seed;
+  bool use_gpu;
+  uint32_t num_threads;
+} NqpuConfig;
+
+/* c_ffi.rs:181-187 */
+typedef struct {
+  uint32_t major;
+  uint32_t minor;
+  uint32_t patch;
+} NqpuVersion;
+
+/* ---------------------------------------------- reference C ABI (drop-in) */
+
+/* c_ffi.rs:789  — complex128 state, 1 <= n <= 30, NULL on failure */
+NqpuState* nqpu_state_create(uint32_t num_qubits);
+/* c_ffi.rs:805  — NULL-safe */
+void nqpu_state_destroy(NqpuState* state);
+/* c_ffi.rs:817  — device-to-device copy */
+NqpuState* nqpu_state_clone(const NqpuState* state);
+/* c_ffi.rs:836  — 0 on NULL */
+uint32_t nqpu_state_num_qubits(const NqpuState* state);
+/* c_ffi.rs:850  — index >= 2^n -> INVALID_PARAMETER */
+NqpuError nqpu_state_get_amplitude(const NqpuState* state, uint64_t index, NqpuComplex* out);
+/* c_ffi.rs:877  — does not renormalise */
+NqpuError nqpu_state_set_amplitude(NqpuState* state, uint64_t index, NqpuComplex amp);
+/* c_ffi.rs:900  — len < 2^n -> INVALID_PARAMETER */
+NqpuError nqpu_state_probabilities(const NqpuState* state, double* out, uint64_t len);
+/* c_ffi.rs:929  — sum |a|^2 in FP64; -1.0 on NULL */
+double nqpu_state_norm(const NqpuState* state);
+/* c_ffi.rs:941 */
+NqpuError nqpu_state_reset(NqpuState* state);
+
+/* c_ffi.rs:987-1004 (macro-generated gate exports) */
+NqpuError nqpu_gate_h(NqpuState* state, uint32_t qubit);
+NqpuError nqpu_gate_x(NqpuState* state, uint32_t qubit);
+NqpuError nqpu_gate_y(NqpuState* state, uint32_t qubit);
+NqpuError nqpu_gate_z(NqpuState* state, uint32_t qubit);
+NqpuError nqpu_gate_s(NqpuState* state, uint32_t qubit);
+NqpuError nqpu_gate_t(NqpuState* state, uint32_t qubit);
+/* non-finite angle -> INVALID_PARAMETER (c_ffi.rs:987) */
+NqpuError nqpu_gate_rx(NqpuState* state, uint32_t qubit, double angle);
+NqpuError nqpu_gate_ry(NqpuState* state, uint32_t qubit, double angle);
+NqpuError nqpu_gate_rz(NqpuState* state, uint32_t qubit, double angle);
+/* c_ffi.rs:1010, 1032, 1054 — equal qubits -> INVALID_PARAMETER */
+NqpuError nqpu_gate_cx(NqpuState* state, uint32_t control, uint32_t target);
+NqpuError nqpu_gate_cz(NqpuState* state, uint32_t control, uint32_t target);
+NqpuError nqpu_gate_swap(NqpuState* state, uint32_t q1, uint32_t q2);
+/* c_ffi.rs:1079 — row-major [m00, m01, m10, m11] */
+NqpuError nqpu_gate_custom_1q(NqpuState* state, uint32_t qubit, const NqpuComplex* matrix);
+/* c_ffi.rs:1111 — row-major 4x4, basis |q1 q2> with q1 the high bit of the
+ * matrix index (the ordering documented at c_ffi.rs:300-303; every 4-amplitude
+ * group is transformed exactly once) */
+NqpuError nqpu_gate_custom_2q(NqpuState* state, uint32_t q1, uint32_t q2, const NqpuComplex* matrix);
+
+/* c_ffi.rs:1151 — collapses; uses the seeded xorshift64* stream */
+NqpuError nqpu_measure(NqpuState* state, uint32_t qubit, NqpuMeasurement* out);
+/* c_ffi.rs:1177 — measures qubit 0..n-1 in order */
+NqpuError nqpu_measure_all(NqpuState* state, uint32_t* out, uint32_t num_qubits);
+/* c_ffi.rs:1205 */
+NqpuError nqpu_expectation_z(const NqpuState* state, uint32_t qubit, double* out);
+/* c_ffi.rs:1231 — basis indices; outcome = first i with r < cumsum(|a|^2) */
+NqpuError nqpu_sample(const NqpuState* state, uint32_t num_shots, uint64_t* out);
+
+/* c_ffi.rs:1259-1398 */
+NqpuCircuit* nqpu_circuit_create(uint32_t num_qubits);
+void nqpu_circuit_destroy(NqpuCircuit* circuit);
+NqpuError nqpu_circuit_add_h(NqpuCircuit* circuit, uint32_t qubit);
+NqpuError nqpu_circuit_add_cx(NqpuCircuit* circuit, uint32_t control, uint32_t target);
+NqpuError nqpu_circuit_add_rx(NqpuCircuit* circuit, uint32_t qubit, double angle);
+NqpuError nqpu_circuit_execute(const NqpuCircuit* circuit, NqpuState* state);
+uint32_t nqpu_circuit_gate_count(const NqpuCircuit* circuit);
+uint32_t nqpu_circuit_depth(const NqpuCircuit* circuit);
+
+/* c_ffi.rs:1403-1467 */
+NqpuVersion nqpu_version(void);
+const char* nqpu_error_message(NqpuError error);
+void nqpu_set_seed(uint64_t seed);
+/* host-side worker threads (circuit planning); the GPU path ignores it */
+void nqpu_set_num_threads(uint32_t threads);
+
+/* ------------------------------------------------------- GPU extensions */
+
+/* Amplitude precision of a state. */
+typedef enum {
+  NQPU_PRECISION_C64 = 1,   /* complex64: 8 bytes per amplitude */
+  NQPU_PRECISION_C128 = 2   /* complex128: 16 bytes per amplitude */
+} NqpuPrecision;
+
+/* Gate kinds: the order of GateType (sdk/rust/src/core/gates/gates.rs:8-64). */
+typedef enum {
+  NQPU_GATE_H = 0, NQPU_GATE_X, NQPU_GATE_Y, NQPU_GATE_Z, NQPU_GATE_S, NQPU_GATE_T,
+  NQPU_GATE_RX, NQPU_GATE_RY, NQPU_GATE_RZ, NQPU_GATE_U, NQPU_GATE_CNOT, NQPU_GATE_CZ,
+  NQPU_GATE_SWAP, NQPU_GATE_TOFFOLI, NQPU_GATE_CRX, NQPU_GATE_CRY, NQPU_GATE_CRZ,
+  NQPU_GATE_CR, NQPU_GATE_SX, NQPU_GATE_PHASE, NQPU_GATE_ISWAP, NQPU_GATE_CCZ,
+  NQPU_GATE_RXX, NQPU_GATE_RYY, NQPU_GATE_RZZ, NQPU_GATE_CSWAP, NQPU_GATE_CU,
+  NQPU_GATE_CUSTOM1, NQPU_GATE_CUSTOM2, NQPU_GATE_NUM_KINDS
+} NqpuGateKind;
+
+/* One gate, as gates::Gate (gates.rs:632): targets, controls and parameters.
+ * Qubit roles follow the reference constructors (gates.rs:669-760):
+ * CNOT/CZ/CR/CRx/CRy/CRz/CU: controls[0], targets[0]; SWAP/ISWAP/Rxx/Ryy/Rzz:
+ * targets[0..1]; Toffoli/CCZ: controls[0..1], targets[0]; CSWAP: controls[0],
+ * targets[0..1].  params: Rx, Ry, Rz, Phase, CRx, CRy, CRz, CR, Rxx, Ryy, Rzz: params[0]; U: theta, phi,
+ * lambda; CU: theta, phi, lambda, gamma.  matrix: CUSTOM1 2x2 / CUSTOM2 4x4,
+ * row-major, with targets[0] as the high bit of a CUSTOM2 index. */
+typedef struct {
+  uint32_t kind;
+  uint32_t num_targets;
+  uint32_t num_controls;
+  uint32_t targets[3];
+  uint32_t controls[3];
+  uint32_t reserved;
+  double params[4];
+  NqpuComplex matrix[16];
+} NqpuGateOp;
+
+/* Gate semantics of a run. */
+typedef enum {
+  /* the matrices of gates.rs for every kind */
+  NQPU_SEMANTICS_STANDARD = 0,
+  /* what the reference's dispatcher executes (ascii_viz.rs:124-179): CRx acts
+   * as lib.rs:1292, CRz as lib.rs:1439-1473, Rxx/Ryy/Rzz/CSWAP/CU/Custom are
+   * no-ops */
+  NQPU_SEMANTICS_REFERENCE = 1
+} NqpuSemantics;
+
+/* How a circuit is turned into device passes. */
+typedef enum {
+  /* one pass per gate, as GateOperations (lib.rs:666-1700) */
+  NQPU_FUSION_NONE = 0,
+  /* the reference's fuse_gates + fuse_diagonal_pass (gate_fusion.rs:537-851),
+   * one pass per fused item, as execute_fused_circuit (gate_fusion.rs:1158) */
+  NQPU_FUSION_REFERENCE = 1,
+  /* LDS-tiled multi-gate passes: every gate whose qubits fit one tile of up
+   * to `tile_qubits` qubits is applied in a single HBM sweep */
+  NQPU_FUSION_TILED = 2
+} NqpuFusion;
+
+typedef struct {
+  uint32_t fusion;       /* NqpuFusion; default NQPU_FUSION_TILED */
+  uint32_t semantics;    /* NqpuSemantics; default NQPU_SEMANTICS_STANDARD */
+  uint32_t tile_qubits;  /* 0 = autotuned default */
+  uint32_t low_qubits;   /* contiguous low qubits always in a tile; 0 = default */
+  uint32_t flags;        /* NQPU_RUN_TIME_PASSES: time every pass with HIP events */
+} NqpuRunOptions;
+
+#define NQPU_RUN_TIME_PASSES 1u
+
+/* Statistics of the last run on a state. */
+typedef struct {
+  uint64_t gates;           /* logical gates in the circuit */
+  uint64_t device_ops;      /* gate operations after lowering and 1q merging */
+  uint64_t passes;          /* full-state HBM sweeps (kernel launches) */
+  double bytes_moved;       /* algorithmic HBM bytes of those sweeps */
+  double pass_ms;           /* with NQPU_RUN_TIME_PASSES: summed kernel time of the passes */
+  uint64_t timed_passes;    /* passes that pass_ms covers */
+} NqpuRunStats;
+
+/* 1 <= n <= 34 (c64) or 33 (c128), bounded by free device memory. */
+NqpuState* nqpu_state_create_ex(uint32_t num_qubits, uint32_t precision);
+uint32_t nqpu_state_precision(const NqpuState* state);
+
+/* Append any gate to a circuit; validates qubits and angles. */
+NqpuError nqpu_circuit_add_gate(NqpuCircuit* circuit, const NqpuGateOp* op);
+NqpuError nqpu_circuit_add_gates(NqpuCircuit* circuit, const NqpuGateOp* ops, uint64_t count);
+/* nqpu_circuit_create without the 30-qubit cap (n <= 34). */
+NqpuCircuit* nqpu_circuit_create_ex(uint32_t num_qubits);
+
+/* Execute with explicit fusion/semantics options (NULL = defaults). */
+NqpuError nqpu_run(const NqpuCircuit* circuit, NqpuState* state, const NqpuRunOptions* opts);
+NqpuError nqpu_last_run_stats(const NqpuState* state, NqpuRunStats* out);
+
+/* FP64 reductions on the device. */
+NqpuError nqpu_expectation_z_all(const NqpuState* state, double* out, uint32_t len);
+/* <P> for the Pauli string with X on xmask&~zmask, Y on xmask&zmask, Z on
+ * zmask&~xmask (qubit q = bit q). */
+NqpuError nqpu_expectation_pauli(const NqpuState* state, uint64_t xmask, uint64_t zmask, double* out);
+/* sum_k coeffs[k] * <P_k> */
+NqpuError nqpu_expectation_pauli_sum(const NqpuState* state, const uint64_t* xmasks,
+                                     const uint64_t* zmasks, const double* coeffs,
+                                     uint64_t count, double* out);
+/* Scale the state to unit norm (FP64 norm). */
+NqpuError nqpu_state_normalize(NqpuState* state);
+/* <a|b> = sum conj(a_i) b_i in FP64 over two states of equal qubit count
+ * (any precisions); |<a|b>|^2 is QuantumState::fidelity (lib.rs:388-403). */
+NqpuError nqpu_state_overlap(const NqpuState* a, const NqpuState* b, NqpuComplex* out);
+
+/* Bulk transfer of all 2^n amplitudes as complex128 (len >= 2^n). */
+NqpuError nqpu_state_get_amplitudes(const NqpuState* state, NqpuComplex* out, uint64_t len);
+NqpuError nqpu_state_set_amplitudes(NqpuState* state, const NqpuComplex* in, uint64_t len);
+
+/* The passes nqpu_run would launch for `circuit` on a state of the given
+ * precision, computed on the host (no device needed). */
+NqpuError nqpu_circuit_plan(const NqpuCircuit* circuit, const NqpuRunOptions* opts,
+                            uint32_t precision, NqpuRunStats* out);
+
+/* Counters of the reference fusion pass (FusionResult, gate_fusion.rs:96-101). */
+typedef struct {
+  uint64_t original_count;
+  uint64_t fused_count;
+  uint64_t gates_eliminated;
+  uint64_t items;
+} NqpuFusionCounts;
+NqpuError nqpu_circuit_fusion_counts(const NqpuCircuit* circuit, NqpuFusionCounts* out);
+
+/* Detail string of the last error on the calling thread. */
+const char* nqpu_last_error_detail(void);
+
+/* Number of usable devices (0 when none or when NQPU_HIP_FORCE_NO_DEVICE=1). */
+uint32_t nqpu_device_count(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* NQPU_HIP_H */

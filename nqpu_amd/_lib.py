@@ -1,0 +1,164 @@
+"""ctypes binding of libnqpu_hip.so (include/nqpu_hip.h).
+
+The library must exist (``python -m nqpu_amd.build`` or
+``__graft_entry__.build()``); there is no CPU fallback.  Loading fails loudly
+if it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnqpu_hip.so")
+
+NQPU_OK = 0
+ERROR_NAMES = {0: "OK", 1: "INVALID_QUBIT", 2: "INVALID_PARAMETER", 3: "ALLOCATION_FAILED",
+               4: "SIMULATION_FAILED", 5: "NULL_POINTER", 6: "INVALID_HANDLE", 7: "DEVICE_ERROR",
+               8: "OUT_OF_DEVICE_MEMORY", 9: "UNAVAILABLE"}
+
+PRECISION_C64 = 1
+PRECISION_C128 = 2
+FUSION_NONE, FUSION_REFERENCE, FUSION_TILED = 0, 1, 2
+SEMANTICS_STANDARD, SEMANTICS_REFERENCE = 0, 1
+
+GATE_KINDS = ["h", "x", "y", "z", "s", "t", "rx", "ry", "rz", "u", "cnot", "cz", "swap",
+              "toffoli", "crx", "cry", "crz", "cr", "sx", "phase", "iswap", "ccz", "rxx",
+              "ryy", "rzz", "cswap", "cu", "custom1", "custom2"]
+GATE_ID = {k: i for i, k in enumerate(GATE_KINDS)}
+
+
+class NqpuComplex(ct.Structure):
+    _fields_ = [("re", ct.c_double), ("im", ct.c_double)]
+
+
+class NqpuMeasurement(ct.Structure):
+    _fields_ = [("outcome", ct.c_uint32), ("probability", ct.c_double)]
+
+
+class NqpuVersion(ct.Structure):
+    _fields_ = [("major", ct.c_uint32), ("minor", ct.c_uint32), ("patch", ct.c_uint32)]
+
+
+class NqpuGateOp(ct.Structure):
+    _fields_ = [("kind", ct.c_uint32), ("num_targets", ct.c_uint32), ("num_controls", ct.c_uint32),
+                ("targets", ct.c_uint32 * 3), ("controls", ct.c_uint32 * 3), ("reserved", ct.c_uint32),
+                ("params", ct.c_double * 4), ("matrix", NqpuComplex * 16)]
+
+
+class NqpuRunOptions(ct.Structure):
+    _fields_ = [("fusion", ct.c_uint32), ("semantics", ct.c_uint32), ("tile_qubits", ct.c_uint32),
+                ("low_qubits", ct.c_uint32), ("flags", ct.c_uint32)]
+
+
+RUN_TIME_PASSES = 1
+
+
+class NqpuRunStats(ct.Structure):
+    _fields_ = [("gates", ct.c_uint64), ("device_ops", ct.c_uint64), ("passes", ct.c_uint64),
+                ("bytes_moved", ct.c_double), ("pass_ms", ct.c_double), ("timed_passes", ct.c_uint64)]
+
+
+class NqpuFusionCounts(ct.Structure):
+    _fields_ = [("original_count", ct.c_uint64), ("fused_count", ct.c_uint64),
+                ("gates_eliminated", ct.c_uint64), ("items", ct.c_uint64)]
+
+
+# name -> (restype, argtypes)
+_P = ct.c_void_p
+_E = ct.c_int
+SIGNATURES = {
+    "nqpu_state_create": (_P, [ct.c_uint32]),
+    "nqpu_state_destroy": (None, [_P]),
+    "nqpu_state_clone": (_P, [_P]),
+    "nqpu_state_num_qubits": (ct.c_uint32, [_P]),
+    "nqpu_state_get_amplitude": (_E, [_P, ct.c_uint64, ct.POINTER(NqpuComplex)]),
+    "nqpu_state_set_amplitude": (_E, [_P, ct.c_uint64, NqpuComplex]),
+    "nqpu_state_probabilities": (_E, [_P, _P, ct.c_uint64]),
+    "nqpu_state_norm": (ct.c_double, [_P]),
+    "nqpu_state_reset": (_E, [_P]),
+    "nqpu_gate_h": (_E, [_P, ct.c_uint32]),
+    "nqpu_gate_x": (_E, [_P, ct.c_uint32]),
+    "nqpu_gate_y": (_E, [_P, ct.c_uint32]),
+    "nqpu_gate_z": (_E, [_P, ct.c_uint32]),
+    "nqpu_gate_s": (_E, [_P, ct.c_uint32]),
+    "nqpu_gate_t": (_E, [_P, ct.c_uint32]),
+    "nqpu_gate_rx": (_E, [_P, ct.c_uint32, ct.c_double]),
+    "nqpu_gate_ry": (_E, [_P, ct.c_uint32, ct.c_double]),
+    "nqpu_gate_rz": (_E, [_P, ct.c_uint32, ct.c_double]),
+    "nqpu_gate_cx": (_E, [_P, ct.c_uint32, ct.c_uint32]),
+    "nqpu_gate_cz": (_E, [_P, ct.c_uint32, ct.c_uint32]),
+    "nqpu_gate_swap": (_E, [_P, ct.c_uint32, ct.c_uint32]),
+    "nqpu_gate_custom_1q": (_E, [_P, ct.c_uint32, _P]),
+    "nqpu_gate_custom_2q": (_E, [_P, ct.c_uint32, ct.c_uint32, _P]),
+    "nqpu_measure": (_E, [_P, ct.c_uint32, ct.POINTER(NqpuMeasurement)]),
+    "nqpu_measure_all": (_E, [_P, _P, ct.c_uint32]),
+    "nqpu_expectation_z": (_E, [_P, ct.c_uint32, ct.POINTER(ct.c_double)]),
+    "nqpu_sample": (_E, [_P, ct.c_uint32, _P]),
+    "nqpu_circuit_create": (_P, [ct.c_uint32]),
+    "nqpu_circuit_destroy": (None, [_P]),
+    "nqpu_circuit_add_h": (_E, [_P, ct.c_uint32]),
+    "nqpu_circuit_add_cx": (_E, [_P, ct.c_uint32, ct.c_uint32]),
+    "nqpu_circuit_add_rx": (_E, [_P, ct.c_uint32, ct.c_double]),
+    "nqpu_circuit_execute": (_E, [_P, _P]),
+    "nqpu_circuit_gate_count": (ct.c_uint32, [_P]),
+    "nqpu_circuit_depth": (ct.c_uint32, [_P]),
+    "nqpu_version": (NqpuVersion, []),
+    "nqpu_error_message": (ct.c_char_p, [_E]),
+    "nqpu_set_seed": (None, [ct.c_uint64]),
+    "nqpu_set_num_threads": (None, [ct.c_uint32]),
+    "nqpu_state_create_ex": (_P, [ct.c_uint32, ct.c_uint32]),
+    "nqpu_state_precision": (ct.c_uint32, [_P]),
+    "nqpu_circuit_add_gate": (_E, [_P, ct.POINTER(NqpuGateOp)]),
+    "nqpu_circuit_add_gates": (_E, [_P, _P, ct.c_uint64]),
+    "nqpu_circuit_create_ex": (_P, [ct.c_uint32]),
+    "nqpu_run": (_E, [_P, _P, ct.POINTER(NqpuRunOptions)]),
+    "nqpu_last_run_stats": (_E, [_P, ct.POINTER(NqpuRunStats)]),
+    "nqpu_expectation_z_all": (_E, [_P, _P, ct.c_uint32]),
+    "nqpu_expectation_pauli": (_E, [_P, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_double)]),
+    "nqpu_expectation_pauli_sum": (_E, [_P, _P, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),
+    "nqpu_state_normalize": (_E, [_P]),
+    "nqpu_state_overlap": (_E, [_P, _P, ct.POINTER(NqpuComplex)]),
+    "nqpu_state_get_amplitudes": (_E, [_P, _P, ct.c_uint64]),
+    "nqpu_state_set_amplitudes": (_E, [_P, _P, ct.c_uint64]),
+    "nqpu_last_error_detail": (ct.c_char_p, []),
+    "nqpu_device_count": (ct.c_uint32, []),
+    "nqpu_circuit_plan": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, ct.POINTER(NqpuRunStats)]),
+    "nqpu_circuit_fusion_counts": (_E, [_P, ct.POINTER(NqpuFusionCounts)]),
+}
+
+_lib = None
+
+
+class NqpuLibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Load libnqpu_hip.so and declare every prototype; raises if missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NqpuLibraryMissing(
+            f"{LIB_PATH} is not built; run `python -m nqpu_amd.build` (hipcc, gfx950)")
+    lib = ct.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class NqpuError(RuntimeError):
+    def __init__(self, code: int, where: str = ""):
+        lib = load()
+        detail = (lib.nqpu_last_error_detail() or b"").decode(errors="replace")
+        self.code = code
+        super().__init__(f"{where}: {ERROR_NAMES.get(code, code)} ({detail})")
+
+
+def check(code: int, where: str = "") -> None:
+    if code != NQPU_OK:
+        raise NqpuError(code, where)

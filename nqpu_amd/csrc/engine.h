@@ -1,0 +1,99 @@
+// engine.h — internal interface between the host runtime (engine.hip,
+// capi.hip) and the gfx950 kernels (kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace nqpu {
+
+using cd = std::complex<double>;
+
+// Lowered operation kinds.  A gate of any kind is lowered to one or more of
+// these (engine.hip: lower_gate); all carry a control mask whose bits must be
+// 1 for the operation to act.
+enum OpKind : int32_t {
+  OP_U1 = 0,    // dense 2x2 on qubit a
+  OP_D1 = 1,    // diag(m[0], m[1]) on qubit a
+  OP_U2 = 2,    // dense 4x4 on (a, b); sub-index (bit_b << 1) | bit_a
+  OP_SWAP = 3,  // exchange |..a=1..b=0..> with |..a=0..b=1..>
+  OP_D2 = 4,    // diag(m[0..3]) on (a, b); index (bit_b << 1) | bit_a
+  OP_FLIP = 5,  // X on qubit a (a permutation; CNOT/Toffoli with controls)
+  OP_DIAGK = 6  // phase LUT over up to 16 qubits (fused diagonal item)
+};
+
+// Operation in global qubit numbering, FP64 coefficients.
+struct HostOp {
+  int32_t kind = OP_U1;
+  int32_t a = 0, b = 0;
+  uint64_t cmask = 0;
+  cd m[16];
+  std::vector<int32_t> dq;   // OP_DIAGK qubits
+  std::vector<cd> lut;       // OP_DIAGK phases, index = bits of dq
+  uint64_t qubit_mask() const;
+};
+
+// Operation as the tile kernel reads it: qubit positions are tile-local bits.
+template <typename R>
+struct DevOp {
+  int32_t kind, a, b, pad;
+  uint64_t cmask;
+  uint64_t pad2;
+  R m[32];  // interleaved re/im, up to 4x4
+};
+
+constexpr int kMaxTile = 14;  // maximum tile qubits (LDS: 2^13 * 16 B = 128 KiB)
+
+// One tile pass: tile qubits (global positions, ascending), the ops it
+// applies, and where they live in the uploaded op buffer.
+struct PassArgs {
+  int32_t K;
+  int32_t n;
+  int32_t nops;
+  int32_t op0;
+  int32_t q[kMaxTile];
+};
+
+// ----------------------------------------------------------- kernel launches
+// All launches are asynchronous on `s`; R = float (c64) or double (c128).
+template <typename R>
+struct Launch {
+  static void init_zero(void* st, int n, hipStream_t s);
+  static void fill_basis(void* st, int n, uint64_t idx, hipStream_t s);
+  static void u1(void* st, int n, int a, uint64_t cmask, const cd* m, hipStream_t s);
+  static void d1(void* st, int n, int a, uint64_t cmask, cd p0, cd p1, hipStream_t s);
+  static void flip(void* st, int n, int a, uint64_t cmask, hipStream_t s);
+  static void u2(void* st, int n, int a, int b, uint64_t cmask, const cd* m, hipStream_t s);
+  static void d2(void* st, int n, int a, int b, uint64_t cmask, const cd* p, hipStream_t s);
+  static void swap(void* st, int n, int a, int b, uint64_t cmask, hipStream_t s);
+  static void diagk(void* st, int n, int k, const int32_t* q, const void* dlut, hipStream_t s);
+  static void tile_pass(void* st, const PassArgs& pa, const void* dops, hipStream_t s);
+  static void scale(void* st, int n, double f, hipStream_t s);
+  // FP64 reductions: per-chunk partials, then deterministic final sums.
+  static void sumsq_chunks(const void* st, int n, int chunk_bits, double* out, hipStream_t s);
+  static void zsum_chunks(const void* st, int n, int chunk_bits, double* out, hipStream_t s);
+  static void pauli_chunks(const void* st, int n, uint64_t xmask, uint64_t zmask, int chunk_bits,
+                           double* out, hipStream_t s);
+  static void probs(const void* st, int n, double* out, hipStream_t s);
+  static void to_c128(const void* st, uint64_t count, uint64_t off, void* out, hipStream_t s);
+  static void from_c128(void* st, uint64_t count, uint64_t off, const void* in, hipStream_t s);
+  static void collapse(void* st, int n, int q, uint32_t outcome, double renorm, hipStream_t s);
+  static void sample_search(const void* st, int n, int chunk_bits, int sub_bits,
+                            const double* chunk_prefix, const double* sub_sums,
+                            const double* r, uint64_t shots, uint64_t* out, hipStream_t s);
+  static void sub_sums(const void* st, int n, int sub_bits, double* out, hipStream_t s);
+};
+
+// Reduce per-chunk double partials [nchunks][width] -> [width] in a fixed
+// order (deterministic for a given n).
+void launch_reduce_rows(const double* in, uint64_t rows, int width, double* out, hipStream_t s);
+// Per-chunk [Re, Im] of sum conj(a_i) b_i; a64/b64 select complex64 storage.
+void launch_overlap(const void* a, bool a64, const void* b, bool b64, int n, int cb, double* out,
+                    hipStream_t s);
+void launch_exclusive_scan(const double* in, uint64_t count, double* out, hipStream_t s);
+
+}  // namespace nqpu

@@ -1,0 +1,1180 @@
+// engine.hip — host runtime: state handles on the device, gate lowering,
+// the reference fusion pass (restated), the LDS tile planner, execution,
+// FP64 reductions and sampling.
+#include "runtime.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+
+namespace nqpu {
+
+// ------------------------------------------------------------------ errors
+
+static thread_local std::string g_err;
+void set_error(const std::string& s) { g_err = s; }
+const char* last_error() { return g_err.c_str(); }
+
+NqpuError hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return NQPU_OK;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? NQPU_OUT_OF_DEVICE_MEMORY : NQPU_DEVICE_ERROR;
+}
+#define HIPTRY(expr, what)                              \
+  do {                                                  \
+    NqpuError e_ = hip_check((expr), what);             \
+    if (e_ != NQPU_OK) return e_;                       \
+  } while (0)
+
+// ------------------------------------------------------------------- masks
+
+static inline uint64_t bit(int q) { return 1ull << q; }
+
+uint64_t HostOp::qubit_mask() const {
+  uint64_t m = cmask | bit(a);
+  if (kind == OP_U2 || kind == OP_SWAP || kind == OP_D2) m |= bit(b);
+  if (kind == OP_DIAGK)
+    for (int q : dq) m |= bit(q);
+  return m;
+}
+
+uint64_t Gate::qubit_mask() const {
+  uint64_t m = 0;
+  for (int i = 0; i < nt; ++i) m |= bit(t[i]);
+  for (int i = 0; i < nc; ++i) m |= bit(c[i]);
+  return m;
+}
+
+// ---------------------------------------------------------------- matrices
+// Gate matrices follow GateType::matrix() (sdk/rust/src/core/gates/gates.rs:70-598).
+
+static const cd I1(0, 1);
+static cd ph(double a) { return cd(std::cos(a), std::sin(a)); }
+
+static void u3m(double th, double phi, double la, cd* m) {  // gates.rs:136-150
+  double c = std::cos(th / 2), s = std::sin(th / 2);
+  m[0] = cd(c, 0);
+  m[1] = cd(-s * std::cos(la), -s * std::sin(la));
+  m[2] = cd(s * std::cos(phi), s * std::sin(phi));
+  m[3] = cd(c * std::cos(phi + la), c * std::sin(phi + la));
+}
+
+// 2x2 matrix of a single-qubit kind (gate_to_matrix, gate_fusion.rs:127-214).
+static bool matrix_1q(const Gate& g, cd* m) {
+  const double r = 1.0 / std::sqrt(2.0), th = g.p[0];
+  switch (g.kind) {
+    case NQPU_GATE_H: m[0] = r; m[1] = r; m[2] = r; m[3] = -r; return true;
+    case NQPU_GATE_X: m[0] = 0; m[1] = 1; m[2] = 1; m[3] = 0; return true;
+    case NQPU_GATE_Y: m[0] = 0; m[1] = -I1; m[2] = I1; m[3] = 0; return true;
+    case NQPU_GATE_Z: m[0] = 1; m[1] = 0; m[2] = 0; m[3] = -1; return true;
+    case NQPU_GATE_S: m[0] = 1; m[1] = 0; m[2] = 0; m[3] = I1; return true;
+    case NQPU_GATE_T: m[0] = 1; m[1] = 0; m[2] = 0; m[3] = cd(std::sqrt(2.0) / 2, std::sqrt(2.0) / 2); return true;
+    case NQPU_GATE_RX: m[0] = std::cos(th / 2); m[1] = cd(0, -std::sin(th / 2)); m[2] = m[1]; m[3] = m[0]; return true;
+    case NQPU_GATE_RY: m[0] = std::cos(th / 2); m[1] = -std::sin(th / 2); m[2] = std::sin(th / 2); m[3] = m[0]; return true;
+    case NQPU_GATE_RZ: m[0] = cd(std::cos(th / 2), -std::sin(th / 2)); m[1] = 0; m[2] = 0; m[3] = cd(std::cos(th / 2), std::sin(th / 2)); return true;
+    case NQPU_GATE_U: u3m(g.p[0], g.p[1], g.p[2], m); return true;
+    case NQPU_GATE_SX: m[0] = cd(.5, .5); m[1] = cd(.5, -.5); m[2] = cd(.5, -.5); m[3] = cd(.5, .5); return true;
+    case NQPU_GATE_PHASE: m[0] = 1; m[1] = 0; m[2] = 0; m[3] = ph(th); return true;
+    default: return false;
+  }
+}
+
+static HostOp op_u1(int a, uint64_t cm, const cd* m) {
+  HostOp o;
+  o.kind = OP_U1; o.a = a; o.cmask = cm;
+  for (int i = 0; i < 4; ++i) o.m[i] = m[i];
+  return o;
+}
+static HostOp op_d1(int a, uint64_t cm, cd p0, cd p1) {
+  HostOp o;
+  o.kind = OP_D1; o.a = a; o.cmask = cm; o.m[0] = p0; o.m[1] = p1;
+  return o;
+}
+static HostOp op_flip(int a, uint64_t cm) {
+  HostOp o;
+  o.kind = OP_FLIP; o.a = a; o.cmask = cm;
+  return o;
+}
+
+// Lower one gate.  Reference semantics reproduce what apply_gate_to_state
+// (sdk/rust/src/circuits/visualization/ascii_viz.rs:124-179) executes.
+std::vector<HostOp> lower_gate(const Gate& g, bool ref) {
+  std::vector<HostOp> v;
+  const int t0 = g.t[0], t1 = g.t[1];
+  const uint64_t c0 = g.nc > 0 ? bit(g.c[0]) : 0, c01 = c0 | (g.nc > 1 ? bit(g.c[1]) : 0);
+  const double th = g.p[0];
+  cd m[16];
+  switch (g.kind) {
+    case NQPU_GATE_X: v.push_back(op_flip(t0, 0)); break;
+    case NQPU_GATE_Z: v.push_back(op_d1(t0, 0, 1, -1)); break;
+    case NQPU_GATE_S: v.push_back(op_d1(t0, 0, 1, I1)); break;
+    case NQPU_GATE_T: v.push_back(op_d1(t0, 0, 1, ph(M_PI / 4))); break;
+    case NQPU_GATE_RZ: v.push_back(op_d1(t0, 0, ph(-th / 2), ph(th / 2))); break;
+    case NQPU_GATE_PHASE: v.push_back(op_d1(t0, 0, 1, ph(th))); break;
+    case NQPU_GATE_H: case NQPU_GATE_Y: case NQPU_GATE_RX: case NQPU_GATE_RY: case NQPU_GATE_U: case NQPU_GATE_SX:
+      matrix_1q(g, m);
+      v.push_back(op_u1(t0, 0, m));
+      break;
+    case NQPU_GATE_CNOT: v.push_back(op_flip(t0, c0)); break;
+    case NQPU_GATE_CZ: v.push_back(op_d1(t0, c0, 1, -1)); break;
+    case NQPU_GATE_CR: v.push_back(op_d1(t0, c0, 1, ph(th))); break;
+    case NQPU_GATE_SWAP:
+      if (t0 != t1) {
+        HostOp o; o.kind = OP_SWAP; o.a = t0; o.b = t1; v.push_back(o);
+      }
+      break;
+    case NQPU_GATE_TOFFOLI: v.push_back(op_flip(t0, c01)); break;
+    case NQPU_GATE_CCZ: v.push_back(op_d1(t0, c01, 1, -1)); break;
+    case NQPU_GATE_CRX:
+      if (ref) {  // lib.rs:1292-1358 applies a controlled-Rz
+        v.push_back(op_d1(t0, c0, ph(-th / 2), ph(th / 2)));
+      } else {
+        Gate r = g; r.kind = NQPU_GATE_RX; matrix_1q(r, m); v.push_back(op_u1(t0, c0, m));
+      }
+      break;
+    case NQPU_GATE_CRY: {
+      Gate r = g; r.kind = NQPU_GATE_RY; matrix_1q(r, m); v.push_back(op_u1(t0, c0, m));
+      break;
+    }
+    case NQPU_GATE_CRZ:
+      if (ref)  // lib.rs:1439-1473: only |c=1,t=0> *= e^{+i th/2}
+        v.push_back(op_d1(t0, c0, ph(th / 2), 1));
+      else
+        v.push_back(op_d1(t0, c0, ph(-th / 2), ph(th / 2)));
+      break;
+    case NQPU_GATE_ISWAP: {  // lib.rs:1652-1686
+      HostOp o; o.kind = OP_U2; o.a = t0; o.b = t1;
+      for (int i = 0; i < 16; ++i) o.m[i] = 0;
+      o.m[0] = 1; o.m[15] = 1; o.m[1 * 4 + 2] = I1; o.m[2 * 4 + 1] = I1;
+      v.push_back(o);
+      break;
+    }
+    case NQPU_GATE_RXX: case NQPU_GATE_RYY: {
+      if (ref) break;  // ascii_viz.rs:166-168: no-op
+      double c = std::cos(th / 2), s = std::sin(th / 2);
+      HostOp o; o.kind = OP_U2; o.a = t0; o.b = t1;
+      for (int i = 0; i < 16; ++i) o.m[i] = 0;
+      for (int i = 0; i < 4; ++i) o.m[i * 5] = c;
+      const double y = (g.kind == NQPU_GATE_RYY) ? 1.0 : -1.0;  // gates.rs:434-484
+      o.m[0 * 4 + 3] = cd(0, y * s); o.m[3 * 4 + 0] = cd(0, y * s);
+      o.m[1 * 4 + 2] = cd(0, -s); o.m[2 * 4 + 1] = cd(0, -s);
+      v.push_back(o);
+      break;
+    }
+    case NQPU_GATE_RZZ: {  // gates.rs:486-511
+      if (ref) break;
+      HostOp o; o.kind = OP_D2; o.a = t0; o.b = t1;
+      o.m[0] = ph(-th / 2); o.m[1] = ph(th / 2); o.m[2] = ph(th / 2); o.m[3] = ph(-th / 2);
+      v.push_back(o);
+      break;
+    }
+    case NQPU_GATE_CSWAP: {
+      if (ref) break;
+      HostOp o; o.kind = OP_SWAP; o.a = t0; o.b = t1; o.cmask = c0;
+      v.push_back(o);
+      break;
+    }
+    case NQPU_GATE_CU: {  // gates.rs:548-598, rows 2-3
+      if (ref) break;
+      double thh = g.p[0], phi = g.p[1], la = g.p[2], ga = g.p[3];
+      double c = std::cos(thh / 2), s = std::sin(thh / 2);
+      m[0] = ph(ga) * c; m[1] = -ph(ga + la) * s; m[2] = ph(ga + phi) * s; m[3] = ph(ga + phi + la) * c;
+      v.push_back(op_u1(t0, c0, m));
+      break;
+    }
+    case NQPU_GATE_CUSTOM1:
+      if (ref) break;
+      v.push_back(op_u1(t0, 0, g.m));
+      break;
+    case NQPU_GATE_CUSTOM2: {
+      if (ref) break;
+      // targets[0] is the high bit of the 4x4 index
+      HostOp o; o.kind = OP_U2; o.a = t1; o.b = t0;
+      for (int i = 0; i < 16; ++i) o.m[i] = g.m[i];
+      v.push_back(o);
+      break;
+    }
+    default: break;
+  }
+  return v;
+}
+
+std::vector<HostOp> lower_circuit(const Circuit& c, bool ref) {
+  std::vector<HostOp> v;
+  v.reserve(c.gates.size() * 2);
+  for (const Gate& g : c.gates) {
+    auto w = lower_gate(g, ref);
+    v.insert(v.end(), w.begin(), w.end());
+  }
+  return v;
+}
+
+// 2x2 of an uncontrolled single-qubit op.
+static bool op_matrix_1q(const HostOp& o, cd* m) {
+  if (o.cmask) return false;
+  if (o.kind == OP_U1) { for (int i = 0; i < 4; ++i) m[i] = o.m[i]; return true; }
+  if (o.kind == OP_D1) { m[0] = o.m[0]; m[1] = 0; m[2] = 0; m[3] = o.m[1]; return true; }
+  if (o.kind == OP_FLIP) { m[0] = 0; m[1] = 1; m[2] = 1; m[3] = 0; return true; }
+  return false;
+}
+
+static void mul2(const cd* b, const cd* a, cd* out) {  // out = b * a
+  cd r[4];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) r[i * 2 + j] = b[i * 2] * a[j] + b[i * 2 + 1] * a[2 + j];
+  for (int i = 0; i < 4; ++i) out[i] = r[i];
+}
+
+static bool is_identity2(const cd* m, double eps) {  // is_identity_matrix, gate_fusion.rs:217
+  return std::abs(m[0].real() - 1) < eps && std::abs(m[0].imag()) < eps && std::abs(m[1].real()) < eps &&
+         std::abs(m[1].imag()) < eps && std::abs(m[2].real()) < eps && std::abs(m[2].imag()) < eps &&
+         std::abs(m[3].real() - 1) < eps && std::abs(m[3].imag()) < eps;
+}
+
+// Per-qubit accumulation of consecutive uncontrolled single-qubit ops into
+// one 2x2 (the single-qubit part of fuse_gates, gate_fusion.rs:537-666).
+std::vector<HostOp> merge_single_qubit(const std::vector<HostOp>& ops) {
+  struct Pend { bool live = false; cd m[4]; int count = 0; HostOp first; };
+  std::vector<Pend> pend(64);
+  std::vector<HostOp> out;
+  out.reserve(ops.size());
+  auto flush = [&](int q) {
+    Pend& p = pend[q];
+    if (!p.live) return;
+    p.live = false;
+    if (p.count == 1) { out.push_back(p.first); return; }
+    if (is_identity2(p.m, 1e-12)) return;
+    if (p.m[1] == cd(0) && p.m[2] == cd(0)) out.push_back(op_d1(q, 0, p.m[0], p.m[3]));
+    else out.push_back(op_u1(q, 0, p.m));
+  };
+  for (const HostOp& o : ops) {
+    cd m[4];
+    if (op_matrix_1q(o, m)) {
+      Pend& p = pend[o.a];
+      if (!p.live) {
+        p.live = true; p.count = 1; p.first = o;
+        for (int i = 0; i < 4; ++i) p.m[i] = m[i];
+      } else {
+        mul2(m, p.m, p.m);
+        p.count++;
+      }
+      continue;
+    }
+    uint64_t qm = o.qubit_mask();
+    for (int q = 0; q < 64; ++q)
+      if ((qm >> q) & 1) flush(q);
+    out.push_back(o);
+  }
+  for (int q = 0; q < 64; ++q) flush(q);
+  return out;
+}
+
+// Greedy tile planner.  A pass owns a set S of at most K qubits, always
+// including the `low` lowest qubits so that tile loads are contiguous runs
+// of 2^low amplitudes.  Walking the remaining ops in order, an op joins the
+// pass when none of its qubits is blocked by an earlier op left behind and
+// its qubits fit in S; otherwise its qubits become blocked.  Ops that join
+// commute with every op they overtake (disjoint qubits), so the order of
+// application is preserved.
+std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po) {
+  std::vector<Pass> passes;
+  const int K = std::min(po.tile_qubits, n);
+  const int L = std::min(po.low_qubits, K);
+  const uint64_t all = (n >= 64) ? ~0ull : (bit(n) - 1);
+  std::vector<size_t> rem(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i) rem[i] = i;
+  while (!rem.empty()) {
+    uint64_t S = (L > 0) ? (bit(L) - 1) : 0, blocked = 0;
+    std::vector<size_t> chosen, next;
+    next.reserve(rem.size());
+    size_t i = 0;
+    for (; i < rem.size(); ++i) {
+      const HostOp& o = ops[rem[i]];
+      uint64_t qm = o.qubit_mask();
+      if (qm & blocked) {
+        blocked |= qm;
+        next.push_back(rem[i]);
+      } else if (__builtin_popcountll(S | qm) <= K) {
+        S |= qm;
+        chosen.push_back(rem[i]);
+      } else {
+        blocked |= qm;
+        next.push_back(rem[i]);
+      }
+      if ((blocked & all) == all) {  // nothing further can join this pass
+        ++i;
+        break;
+      }
+    }
+    for (; i < rem.size(); ++i) next.push_back(rem[i]);
+    if (chosen.empty()) {  // op wider than a tile cannot happen (ops touch <= 4 qubits)
+      chosen.push_back(next.front());
+      next.erase(next.begin());
+    }
+    for (int q = 0; q < n && __builtin_popcountll(S) < K; ++q) S |= bit(q);
+    Pass p;
+    for (int q = 0; q < n; ++q)
+      if ((S >> q) & 1) p.qubits.push_back(q);
+    for (size_t k : chosen) p.ops.push_back(ops[k]);
+    passes.push_back(std::move(p));
+    rem.swap(next);
+  }
+  return passes;
+}
+
+// ------------------------------------------------------ reference fusion
+// Restatement of fuse_gates / fuse_diagonal_pass
+// (sdk/rust/src/circuits/synthesis/gate_fusion.rs:537-851).
+
+static bool is_fusable_1q(const Gate& g) {  // gate_fusion.rs:440-462
+  if (g.nc != 0 || g.nt != 1) return false;
+  switch (g.kind) {
+    case NQPU_GATE_H: case NQPU_GATE_X: case NQPU_GATE_Y: case NQPU_GATE_Z: case NQPU_GATE_S: case NQPU_GATE_T:
+    case NQPU_GATE_RX: case NQPU_GATE_RY: case NQPU_GATE_RZ: case NQPU_GATE_U: case NQPU_GATE_SX: case NQPU_GATE_PHASE:
+      return true;
+    default: return false;
+  }
+}
+
+// two_qubit_gate_to_matrix (gate_fusion.rs:323-426): lower qubit -> bit 0.
+static bool matrix_2q_ref(const Gate& g, cd* m) {
+  std::vector<int> qs;
+  for (int i = 0; i < g.nt; ++i) qs.push_back(g.t[i]);
+  for (int i = 0; i < g.nc; ++i) qs.push_back(g.c[i]);
+  std::sort(qs.begin(), qs.end());
+  qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+  if (qs.size() != 2) return false;
+  const int qa = qs[0];
+  auto remap = [&](int q) { return q == qa ? 0 : 1; };
+  int control = g.nc > 0 ? remap(g.c[0]) : 0;
+  int target = g.nt > 0 ? remap(g.t[0]) : 0;
+  for (int i = 0; i < 16; ++i) m[i] = 0;
+  switch (g.kind) {
+    case NQPU_GATE_CNOT:
+      if (control == 1 && target == 0) { m[0] = 1; m[5] = 1; m[2 * 4 + 3] = 1; m[3 * 4 + 2] = 1; }
+      else { m[0] = 1; m[1 * 4 + 3] = 1; m[2 * 4 + 2] = 1; m[3 * 4 + 1] = 1; }
+      return true;
+    case NQPU_GATE_CZ: m[0] = 1; m[5] = 1; m[10] = 1; m[15] = -1; return true;
+    case NQPU_GATE_SWAP: m[0] = 1; m[1 * 4 + 2] = 1; m[2 * 4 + 1] = 1; m[15] = 1; return true;
+    case NQPU_GATE_CR: m[0] = 1; m[5] = 1; m[10] = 1; m[15] = ph(g.p[0]); return true;
+    case NQPU_GATE_ISWAP: m[0] = 1; m[1 * 4 + 2] = I1; m[2 * 4 + 1] = I1; m[15] = 1; return true;
+    default: return false;  // CRx/CRy/CRz and the rest go through the dispatcher
+  }
+}
+
+static void mul4(const cd* b, const cd* a, cd* out) {  // out = b * a
+  cd r[16];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      cd s = 0;
+      for (int k = 0; k < 4; ++k) s += b[i * 4 + k] * a[k * 4 + j];
+      r[i * 4 + j] = s;
+    }
+  for (int i = 0; i < 16; ++i) out[i] = r[i];
+}
+
+static bool is_diag_1q_kind(int k) {
+  return k == NQPU_GATE_Z || k == NQPU_GATE_S || k == NQPU_GATE_T || k == NQPU_GATE_RZ || k == NQPU_GATE_PHASE;
+}
+
+static std::pair<cd, cd> diag_1q_phases(const Gate& g) {  // gate_fusion.rs:690-707
+  switch (g.kind) {
+    case NQPU_GATE_Z: return {1, -1};
+    case NQPU_GATE_S: return {1, I1};
+    case NQPU_GATE_T: return {1, cd(std::sqrt(2.0) / 2, std::sqrt(2.0) / 2)};
+    case NQPU_GATE_RZ: return {cd(std::cos(g.p[0] / 2), -std::sin(g.p[0] / 2)), cd(std::cos(g.p[0] / 2), std::sin(g.p[0] / 2))};
+    case NQPU_GATE_PHASE: return {1, ph(g.p[0])};
+    default: return {1, 1};
+  }
+}
+
+FusionResult fuse_gates_reference(const std::vector<Gate>& gates) {
+  FusionResult res;
+  res.original_count = gates.size();
+  struct Acc { bool live = false; cd m[4]; size_t count = 0; bool has_orig = false; Gate orig; };
+  std::map<int, Acc> acc;  // ordered: the flush order of commuting 1q items is irrelevant
+  std::vector<FusedItem> out;
+  size_t elim = 0;
+  auto flush = [&](int q) {
+    auto it = acc.find(q);
+    if (it == acc.end()) return;
+    Acc a = it->second;
+    acc.erase(it);
+    if (a.count == 0) return;
+    if (is_identity2(a.m, 1e-12)) { elim += a.count; return; }
+    if (a.count == 1 && a.has_orig) {
+      FusedItem f; f.kind = FusedItem::ORIGINAL; f.gate = a.orig; out.push_back(f); return;
+    }
+    FusedItem f; f.kind = FusedItem::FUSED; f.q = q; f.fused_count = (int)a.count;
+    for (int i = 0; i < 4; ++i) f.m[i] = a.m[i];
+    out.push_back(f);
+    elim += a.count - 1;
+  };
+  for (const Gate& g : gates) {
+    if (is_fusable_1q(g)) {
+      cd m[4];
+      matrix_1q(g, m);
+      Acc& a = acc[g.t[0]];
+      if (!a.live) {
+        a.live = true; a.m[0] = 1; a.m[1] = 0; a.m[2] = 0; a.m[3] = 1; a.count = 0;
+      }
+      mul2(m, a.m, a.m);
+      a.count++;
+      if (a.count == 1) { a.has_orig = true; a.orig = g; } else { a.has_orig = false; }
+      continue;
+    }
+    std::vector<int> qs;
+    for (int i = 0; i < g.nt; ++i) qs.push_back(g.t[i]);
+    for (int i = 0; i < g.nc; ++i) qs.push_back(g.c[i]);
+    std::sort(qs.begin(), qs.end());
+    qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+    cd g4[16];
+    if (qs.size() == 2 && matrix_2q_ref(g, g4)) {
+      const int lo = qs[0], hi = qs[1];
+      cd mlo[4] = {1, 0, 0, 1}, mhi[4] = {1, 0, 0, 1};
+      size_t clo = 0, chi = 0;
+      auto il = acc.find(lo);
+      if (il != acc.end()) { for (int i = 0; i < 4; ++i) mlo[i] = il->second.m[i]; clo = il->second.count; acc.erase(il); }
+      auto ih = acc.find(hi);
+      if (ih != acc.end()) { for (int i = 0; i < 4; ++i) mhi[i] = ih->second.m[i]; chi = ih->second.count; acc.erase(ih); }
+      if (clo == 0 && chi == 0) {
+        FusedItem f; f.kind = FusedItem::ORIGINAL; f.gate = g; out.push_back(f);
+      } else {
+        cd pre[16];  // kronecker_2x2(hi, lo), gate_fusion.rs:255
+        for (int r = 0; r < 4; ++r)
+          for (int c = 0; c < 4; ++c) pre[r * 4 + c] = mhi[(r >> 1) * 2 + (c >> 1)] * mlo[(r & 1) * 2 + (c & 1)];
+        FusedItem f; f.kind = FusedItem::FUSED_TWO; f.qlo = lo; f.qhi = hi;
+        mul4(g4, pre, f.m);
+        f.fused_count = (int)(1 + clo + chi);
+        elim += clo + chi;
+        out.push_back(f);
+      }
+    } else {
+      for (int q : qs) flush(q);
+      FusedItem f; f.kind = FusedItem::ORIGINAL; f.gate = g; out.push_back(f);
+    }
+  }
+  std::vector<int> keys;
+  for (auto& kv : acc) keys.push_back(kv.first);
+  for (int q : keys) flush(q);
+
+  // fuse_diagonal_pass (gate_fusion.rs:757-851)
+  std::vector<FusedItem> dout;
+  std::map<int, std::pair<cd, cd>> d1;
+  std::map<std::pair<int, int>, cd> d2;
+  size_t dcount = 0;
+  auto is_diag_item = [](const FusedItem& it) {
+    if (it.kind != FusedItem::ORIGINAL) return false;
+    const Gate& g = it.gate;
+    if (g.nt == 1 && g.nc == 0) return is_diag_1q_kind(g.kind);
+    return g.kind == NQPU_GATE_CZ || g.kind == NQPU_GATE_CR;
+  };
+  auto flush_diag = [&]() {
+    if (dcount >= 2) {
+      FusedItem f; f.kind = FusedItem::DIAGONAL; f.fused_count = (int)dcount;
+      for (auto& kv : d1) f.single.push_back(kv);
+      for (auto& kv : d2) f.pairs.push_back(kv);
+      dout.push_back(f);
+    } else {  // drain_diag_singles (gate_fusion.rs:847-902)
+      for (auto& kv : d1) {
+        cd m[4] = {kv.second.first, 0, 0, kv.second.second};
+        if (is_identity2(m, 1e-12)) continue;
+        FusedItem f; f.kind = FusedItem::FUSED; f.q = kv.first;
+        for (int i = 0; i < 4; ++i) f.m[i] = m[i];
+        dout.push_back(f);
+      }
+      for (auto& kv : d2) {
+        FusedItem f; f.kind = FusedItem::FUSED_TWO; f.qlo = kv.first.first; f.qhi = kv.first.second;
+        for (int i = 0; i < 16; ++i) f.m[i] = 0;
+        f.m[0] = 1; f.m[5] = 1; f.m[10] = 1; f.m[15] = kv.second;
+        bool ident = std::abs(kv.second.real() - 1) <= 1e-12 && std::abs(kv.second.imag()) <= 1e-12;
+        if (!ident) dout.push_back(f);
+      }
+    }
+    d1.clear();
+    d2.clear();
+    dcount = 0;
+  };
+  for (const FusedItem& it : out) {
+    if (is_diag_item(it)) {
+      const Gate& g = it.gate;
+      if (g.nt == 1 && g.nc == 0) {
+        auto p = diag_1q_phases(g);
+        auto e = d1.find(g.t[0]);
+        if (e == d1.end()) d1[g.t[0]] = p;
+        else { e->second.first *= p.first; e->second.second *= p.second; }
+      } else {
+        int a = g.t[0], b = g.c[0];
+        int lo = std::min(a, b), hi = std::max(a, b);
+        cd p11 = g.kind == NQPU_GATE_CZ ? cd(-1) : ph(g.p[0]);
+        auto e = d2.find({lo, hi});
+        if (e == d2.end()) d2[{lo, hi}] = p11; else e->second *= p11;
+      }
+      dcount++;
+    } else {
+      flush_diag();
+      dout.push_back(it);
+    }
+  }
+  flush_diag();
+  res.items = std::move(dout);
+  res.gates_eliminated = elim;
+  for (auto& it : res.items)
+    if (it.kind != FusedItem::ORIGINAL) res.fused_count++;
+  return res;
+}
+
+// ------------------------------------------------------------------ device
+
+int device_count() {
+  const char* f = std::getenv("NQPU_HIP_FORCE_NO_DEVICE");
+  if (f && f[0] == '1') return 0;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+DeviceScratch::~DeviceScratch() {
+  if (p) (void)hipFree(p);
+}
+bool DeviceScratch::ensure(size_t b) {
+  if (b <= bytes) return true;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+  size_t nb = std::max(b, (size_t)1 << 20);
+  if (hipMalloc(&p, nb) != hipSuccess) {
+    p = nullptr;
+    set_error("hipMalloc of scratch failed");
+    return false;
+  }
+  bytes = nb;
+  return true;
+}
+
+State::~State() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (amps) (void)hipFree(amps);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename F>
+static void dispatch(const State* s, F&& f) {
+  if (s->prec == NQPU_PRECISION_C64) f((float*)nullptr);
+  else f((double*)nullptr);
+}
+
+NqpuError create_state(int n, int prec, State** out) {
+  *out = nullptr;
+  if (device_count() == 0) {
+    set_error("no usable HIP device (or NQPU_HIP_FORCE_NO_DEVICE=1)");
+    return NQPU_UNAVAILABLE;
+  }
+  std::unique_ptr<State> s(new State());
+  s->n = n;
+  s->prec = prec;
+  HIPTRY(hipSetDevice(0), "hipSetDevice");
+  s->device = 0;
+  size_t bytes = s->amp_size() << n;
+  size_t fr = 0, tot = 0;
+  HIPTRY(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+  if (bytes + (size_t(1) << 29) > fr) {
+    set_error("state of " + std::to_string(bytes) + " bytes does not fit in free device memory");
+    return NQPU_OUT_OF_DEVICE_MEMORY;
+  }
+  HIPTRY(hipStreamCreate(&s->stream), "hipStreamCreate");
+  HIPTRY(hipMalloc(&s->amps, bytes), "hipMalloc(state)");
+  NqpuError e = reset_state(s.get());
+  if (e != NQPU_OK) return e;
+  *out = s.release();
+  return NQPU_OK;
+}
+
+NqpuError reset_state(State* s) {
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    Launch<R>::init_zero(s->amps, s->n, s->stream);
+  });
+  HIPTRY(hipGetLastError(), "init_zero launch");
+  HIPTRY(hipStreamSynchronize(s->stream), "init_zero");
+  return NQPU_OK;
+}
+
+NqpuError clone_state(const State* s, State** out) {
+  *out = nullptr;
+  std::unique_ptr<State> c(new State());
+  c->n = s->n;
+  c->prec = s->prec;
+  c->device = s->device;
+  size_t bytes = s->amp_size() << s->n;
+  HIPTRY(hipStreamCreate(&c->stream), "hipStreamCreate");
+  HIPTRY(hipMalloc(&c->amps, bytes), "hipMalloc(clone)");
+  HIPTRY(hipStreamSynchronize(s->stream), "sync");
+  HIPTRY(hipMemcpyAsync(c->amps, s->amps, bytes, hipMemcpyDeviceToDevice, c->stream), "clone copy");
+  HIPTRY(hipStreamSynchronize(c->stream), "clone sync");
+  *out = c.release();
+  return NQPU_OK;
+}
+
+static uint64_t op_bytes_direct(const HostOp& o, int n, size_t asz) {
+  // touched amplitudes x (read + write)
+  uint64_t dim = 1ull << n;
+  int fixed = __builtin_popcountll(o.qubit_mask());
+  switch (o.kind) {
+    case OP_DIAGK: return 2 * dim * asz;
+    case OP_D1: {
+      bool only1 = (o.m[0] == cd(1, 0));
+      uint64_t groups = dim >> fixed;
+      return 2 * groups * (only1 ? 1 : 2) * asz;
+    }
+    default: {
+      uint64_t groups = dim >> fixed;
+      int per = (o.kind == OP_U2 || o.kind == OP_D2) ? 4 : 2;
+      return 2 * groups * per * asz;
+    }
+  }
+}
+
+template <typename R>
+static NqpuError exec_direct(State* s, const HostOp& o) {
+  void* st = s->amps;
+  const int n = s->n;
+  hipStream_t q = s->stream;
+  if (o.kind == OP_DIAGK) {
+    // the previous LUT may still be in use: synchronise before overwriting it
+    size_t lb = o.lut.size() * sizeof(double) * 2;
+    HIPTRY(hipStreamSynchronize(q), "sync before LUT upload");
+    if (!s->scratch2.ensure(lb)) return NQPU_OUT_OF_DEVICE_MEMORY;
+    HIPTRY(hipMemcpy(s->scratch2.p, o.lut.data(), lb, hipMemcpyHostToDevice), "LUT upload");
+  }
+  s->timer.mark(q);
+  switch (o.kind) {
+    case OP_U1: Launch<R>::u1(st, n, o.a, o.cmask, o.m, q); break;
+    case OP_FLIP: Launch<R>::flip(st, n, o.a, o.cmask, q); break;
+    case OP_D1: Launch<R>::d1(st, n, o.a, o.cmask, o.m[0], o.m[1], q); break;
+    case OP_U2: Launch<R>::u2(st, n, o.a, o.b, o.cmask, o.m, q); break;
+    case OP_D2: Launch<R>::d2(st, n, o.a, o.b, o.cmask, o.m, q); break;
+    case OP_SWAP: Launch<R>::swap(st, n, o.a, o.b, o.cmask, q); break;
+    case OP_DIAGK: Launch<R>::diagk(st, n, (int)o.dq.size(), o.dq.data(), s->scratch2.p, q); break;
+    default: break;
+  }
+  s->timer.mark(q);
+  HIPTRY(hipGetLastError(), "direct kernel launch");
+  s->stats.passes++;
+  s->stats.bytes_moved += (double)op_bytes_direct(o, n, s->amp_size());
+  return NQPU_OK;
+}
+
+// LUT items for a fused diagonal, at most 16 qubits each (execute_diagonal,
+// gate_fusion.rs:905-1033: phase per bit pattern).
+static std::vector<HostOp> diagonal_ops(const FusedItem& it) {
+  std::vector<HostOp> v;
+  std::vector<std::pair<int, std::pair<cd, cd>>> singles = it.single;
+  std::vector<std::pair<std::pair<int, int>, cd>> pairs = it.pairs;
+  while (!singles.empty() || !pairs.empty()) {
+    HostOp o;
+    o.kind = OP_DIAGK;
+    std::vector<int> qs;
+    auto has = [&](int q) { return std::find(qs.begin(), qs.end(), q) != qs.end(); };
+    std::vector<std::pair<int, std::pair<cd, cd>>> s_take, s_rest;
+    std::vector<std::pair<std::pair<int, int>, cd>> p_take, p_rest;
+    for (auto& p : pairs) {
+      int need = (has(p.first.first) ? 0 : 1) + (has(p.first.second) ? 0 : 1);
+      if ((int)qs.size() + need <= 16) {
+        if (!has(p.first.first)) qs.push_back(p.first.first);
+        if (!has(p.first.second)) qs.push_back(p.first.second);
+        p_take.push_back(p);
+      } else {
+        p_rest.push_back(p);
+      }
+    }
+    for (auto& sp : singles) {
+      if (has(sp.first) || qs.size() < 16) {
+        if (!has(sp.first)) qs.push_back(sp.first);
+        s_take.push_back(sp);
+      } else {
+        s_rest.push_back(sp);
+      }
+    }
+    std::sort(qs.begin(), qs.end());
+    auto pos = [&](int q) { return (int)(std::find(qs.begin(), qs.end(), q) - qs.begin()); };
+    o.dq.assign(qs.begin(), qs.end());
+    o.a = qs.empty() ? 0 : qs[0];
+    size_t L = size_t(1) << qs.size();
+    o.lut.assign(L, cd(1, 0));
+    for (size_t pat = 0; pat < L; ++pat) {
+      cd phs = 1;
+      for (auto& sp : s_take) phs *= ((pat >> pos(sp.first)) & 1) ? sp.second.second : sp.second.first;
+      for (auto& pp : p_take)
+        if (((pat >> pos(pp.first.first)) & 1) && ((pat >> pos(pp.first.second)) & 1)) phs *= pp.second;
+      o.lut[pat] = phs;
+    }
+    v.push_back(o);
+    singles.swap(s_rest);
+    pairs.swap(p_rest);
+  }
+  return v;
+}
+
+template <typename R>
+static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanOptions& po) {
+  std::vector<Pass> passes = plan_tiles(ops, s->n, po);
+  std::vector<DevOp<R>> dops;
+  std::vector<PassArgs> args;
+  for (const Pass& p : passes) {
+    PassArgs pa{};
+    pa.K = (int)p.qubits.size();
+    pa.n = s->n;
+    pa.op0 = (int)dops.size();
+    pa.nops = (int)p.ops.size();
+    int loc[64];
+    for (int q = 0; q < 64; ++q) loc[q] = -1;
+    for (int j = 0; j < pa.K; ++j) {
+      pa.q[j] = p.qubits[j];
+      loc[p.qubits[j]] = j;
+    }
+    for (const HostOp& o : p.ops) {
+      DevOp<R> d{};
+      d.kind = o.kind;
+      d.a = loc[o.a];
+      d.b = (o.kind == OP_U2 || o.kind == OP_SWAP || o.kind == OP_D2) ? loc[o.b] : 0;
+      uint64_t cm = 0;
+      for (int q = 0; q < 64; ++q)
+        if ((o.cmask >> q) & 1) cm |= bit(loc[q]);
+      d.cmask = cm;
+      int nm = (o.kind == OP_U2) ? 16 : (o.kind == OP_U1 ? 4 : (o.kind == OP_D2 ? 4 : 2));
+      for (int i = 0; i < nm; ++i) {
+        d.m[2 * i] = (R)o.m[i].real();
+        d.m[2 * i + 1] = (R)o.m[i].imag();
+      }
+      dops.push_back(d);
+    }
+    args.push_back(pa);
+  }
+  size_t ob = std::max<size_t>(dops.size(), 1) * sizeof(DevOp<R>);
+  HIPTRY(hipStreamSynchronize(s->stream), "sync before op upload");
+  if (!s->opbuf.ensure(ob)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  if (!dops.empty())
+    HIPTRY(hipMemcpy(s->opbuf.p, dops.data(), dops.size() * sizeof(DevOp<R>), hipMemcpyHostToDevice), "op upload");
+  for (const PassArgs& pa : args) {
+    s->timer.mark(s->stream);
+    Launch<R>::tile_pass(s->amps, pa, s->opbuf.p, s->stream);
+    s->timer.mark(s->stream);
+    HIPTRY(hipGetLastError(), "tile pass launch");
+  }
+  s->stats.passes += passes.size();
+  s->stats.device_ops += dops.size();
+  s->stats.bytes_moved += (double)passes.size() * 2.0 * (double)(s->amp_size() << s->n);
+  return NQPU_OK;
+}
+
+PlanOptions plan_options(const NqpuRunOptions& o, int prec) {
+  PlanOptions po;
+  const int kmax = (prec == NQPU_PRECISION_C64) ? 13 : 12;
+  po.tile_qubits = o.tile_qubits ? std::min<int>((int)o.tile_qubits, kmax) : (prec == NQPU_PRECISION_C64 ? 12 : 11);
+  if (po.tile_qubits < 5) po.tile_qubits = 5;
+  po.low_qubits = o.low_qubits ? (int)o.low_qubits : (prec == NQPU_PRECISION_C64 ? 5 : 4);
+  po.low_qubits = std::min(po.low_qubits, po.tile_qubits);
+  return po;
+}
+
+NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
+  NqpuRunStats st{};
+  st.gates = c.gates.size();
+  const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
+  const size_t asz = prec == NQPU_PRECISION_C64 ? 8 : 16;
+  auto add_direct = [&](const HostOp& op) {
+    st.device_ops++;
+    st.passes++;
+    st.bytes_moved += (double)op_bytes_direct(op, c.n, asz);
+  };
+  if (o.fusion == NQPU_FUSION_NONE) {
+    for (const Gate& g : c.gates)
+      for (const HostOp& op : lower_gate(g, ref)) add_direct(op);
+  } else if (o.fusion == NQPU_FUSION_REFERENCE) {
+    FusionResult fr = fuse_gates_reference(c.gates);
+    for (const FusedItem& it : fr.items) {
+      if (it.kind == FusedItem::FUSED || it.kind == FusedItem::FUSED_TWO) {
+        HostOp h; h.kind = it.kind == FusedItem::FUSED ? OP_U1 : OP_U2; h.a = it.kind == FusedItem::FUSED ? it.q : it.qlo; h.b = it.qhi;
+        add_direct(h);
+      } else if (it.kind == FusedItem::DIAGONAL) {
+        for (const HostOp& op : diagonal_ops(it)) add_direct(op);
+      } else {
+        for (const HostOp& op : lower_gate(it.gate, ref)) add_direct(op);
+      }
+    }
+  } else {
+    std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
+    std::vector<Pass> passes = plan_tiles(ops, c.n, plan_options(o, prec));
+    st.passes = passes.size();
+    st.device_ops = ops.size();
+    st.bytes_moved = (double)passes.size() * 2.0 * (double)(asz << c.n);
+  }
+  return st;
+}
+
+NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o) {
+  s->stats = NqpuRunStats{};
+  s->stats.gates = c.gates.size();
+  const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
+  NqpuError e = NQPU_OK;
+  s->timer.begin((o.flags & NQPU_RUN_TIME_PASSES) != 0);
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    if (o.fusion == NQPU_FUSION_NONE) {
+      for (const Gate& g : c.gates) {
+        for (const HostOp& op : lower_gate(g, ref)) {
+          s->stats.device_ops++;
+          if ((e = exec_direct<R>(s, op)) != NQPU_OK) return;
+        }
+      }
+    } else if (o.fusion == NQPU_FUSION_REFERENCE) {
+      FusionResult fr = fuse_gates_reference(c.gates);
+      for (const FusedItem& it : fr.items) {
+        std::vector<HostOp> ops;
+        if (it.kind == FusedItem::FUSED) ops.push_back(op_u1(it.q, 0, it.m));
+        else if (it.kind == FusedItem::FUSED_TWO) {
+          HostOp h; h.kind = OP_U2; h.a = it.qlo; h.b = it.qhi;
+          for (int i = 0; i < 16; ++i) h.m[i] = it.m[i];
+          ops.push_back(h);
+        } else if (it.kind == FusedItem::DIAGONAL) ops = diagonal_ops(it);
+        else ops = lower_gate(it.gate, ref);
+        for (const HostOp& op : ops) {
+          s->stats.device_ops++;
+          if ((e = exec_direct<R>(s, op)) != NQPU_OK) return;
+        }
+      }
+    } else {
+      PlanOptions po = plan_options(o, s->prec);
+      std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
+      e = run_tiled<R>(s, ops, po);
+    }
+  });
+  if (e != NQPU_OK) {
+    s->timer.begin(false);
+    return e;
+  }
+  HIPTRY(hipStreamSynchronize(s->stream), "run");
+  if (s->timer.on) {
+    s->stats.pass_ms = s->timer.total_ms();
+    s->stats.timed_passes = s->timer.used / 2;
+  }
+  s->timer.begin(false);
+  return NQPU_OK;
+}
+
+void PassTimer::mark(hipStream_t s) {
+  if (!on) return;
+  if (used == pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      on = false;
+      return;
+    }
+    pool.push_back(e);
+  }
+  (void)hipEventRecord(pool[used++], s);
+}
+
+double PassTimer::total_ms() {
+  double t = 0;
+  for (size_t i = 0; i + 1 < used; i += 2) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, pool[i], pool[i + 1]) == hipSuccess) t += ms;
+  }
+  return t;
+}
+
+PassTimer::~PassTimer() {
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+NqpuError overlap(const State* a0, const State* b0, NqpuComplex* out) {
+  State* a = const_cast<State*>(a0);
+  State* b = const_cast<State*>(b0);
+  HIPTRY(hipStreamSynchronize(b->stream), "sync");
+  const int cb = std::min(a->n, 16);
+  const uint64_t nch = a->dim() >> cb;
+  if (!a->scratch.ensure(nch * 16)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  launch_overlap(a->amps, a->prec == NQPU_PRECISION_C64, b->amps, b->prec == NQPU_PRECISION_C64, a->n, cb,
+                 (double*)a->scratch.p, a->stream);
+  HIPTRY(hipGetLastError(), "overlap");
+  HIPTRY(hipStreamSynchronize(a->stream), "overlap");
+  std::vector<double> rows(nch * 2);
+  HIPTRY(hipMemcpy(rows.data(), a->scratch.p, nch * 16, hipMemcpyDeviceToHost), "overlap copy");
+  double re = 0, im = 0;
+  for (uint64_t c = 0; c < nch; ++c) {
+    re += rows[2 * c];
+    im += rows[2 * c + 1];
+  }
+  out->re = re;
+  out->im = im;
+  return NQPU_OK;
+}
+
+NqpuError apply_gate_now(State* s, const Gate& g) {
+  NqpuError e = NQPU_OK;
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    for (const HostOp& op : lower_gate(g, false))
+      if ((e = exec_direct<R>(s, op)) != NQPU_OK) return;
+  });
+  if (e != NQPU_OK) return e;
+  HIPTRY(hipStreamSynchronize(s->stream), "gate");
+  return NQPU_OK;
+}
+
+// ---------------------------------------------------------- amplitudes I/O
+
+static const uint64_t kXferChunk = 1ull << 24;  // amplitudes per staging copy
+
+NqpuError get_amplitudes(const State* s0, uint64_t off, uint64_t count, NqpuComplex* out) {
+  State* s = const_cast<State*>(s0);
+  HIPTRY(hipStreamSynchronize(s->stream), "sync");
+  if (s->prec == NQPU_PRECISION_C128) {
+    HIPTRY(hipMemcpy(out, (char*)s->amps + off * 16, count * 16, hipMemcpyDeviceToHost), "amplitude copy");
+    return NQPU_OK;
+  }
+  for (uint64_t b = 0; b < count; b += kXferChunk) {
+    uint64_t c = std::min(kXferChunk, count - b);
+    if (!s->scratch.ensure(c * 16)) return NQPU_OUT_OF_DEVICE_MEMORY;
+    Launch<float>::to_c128(s->amps, c, off + b, s->scratch.p, s->stream);
+    HIPTRY(hipGetLastError(), "to_c128");
+    HIPTRY(hipStreamSynchronize(s->stream), "to_c128");
+    HIPTRY(hipMemcpy(out + b, s->scratch.p, c * 16, hipMemcpyDeviceToHost), "amplitude copy");
+  }
+  return NQPU_OK;
+}
+
+NqpuError set_amplitudes(State* s, uint64_t off, uint64_t count, const NqpuComplex* in) {
+  HIPTRY(hipStreamSynchronize(s->stream), "sync");
+  if (s->prec == NQPU_PRECISION_C128) {
+    HIPTRY(hipMemcpy((char*)s->amps + off * 16, in, count * 16, hipMemcpyHostToDevice), "amplitude copy");
+    return NQPU_OK;
+  }
+  for (uint64_t b = 0; b < count; b += kXferChunk) {
+    uint64_t c = std::min(kXferChunk, count - b);
+    if (!s->scratch.ensure(c * 16)) return NQPU_OUT_OF_DEVICE_MEMORY;
+    HIPTRY(hipMemcpy(s->scratch.p, in + b, c * 16, hipMemcpyHostToDevice), "amplitude copy");
+    Launch<float>::from_c128(s->amps, c, off + b, s->scratch.p, s->stream);
+    HIPTRY(hipGetLastError(), "from_c128");
+    HIPTRY(hipStreamSynchronize(s->stream), "from_c128");
+  }
+  return NQPU_OK;
+}
+
+// -------------------------------------------------------------- reductions
+
+static int chunk_bits_for(int n) { return std::min(n, 16); }
+
+NqpuError probabilities(const State* s0, double* out) {
+  State* s = const_cast<State*>(s0);
+  const uint64_t dim = s->dim();
+  // the probs kernel covers the whole state; stage in slices of kXferChunk
+  if (dim <= kXferChunk) {
+    if (!s->scratch.ensure(dim * 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+    dispatch(s, [&](auto* tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      Launch<R>::probs(s->amps, s->n, (double*)s->scratch.p, s->stream);
+    });
+    HIPTRY(hipGetLastError(), "probs");
+    HIPTRY(hipStreamSynchronize(s->stream), "probs");
+    HIPTRY(hipMemcpy(out, s->scratch.p, dim * 8, hipMemcpyDeviceToHost), "probs copy");
+    return NQPU_OK;
+  }
+  // large states: convert slices to c128 on the host side of the copy
+  std::vector<NqpuComplex> buf(kXferChunk);
+  for (uint64_t b = 0; b < dim; b += kXferChunk) {
+    uint64_t c = std::min(kXferChunk, dim - b);
+    NqpuError e = get_amplitudes(s, b, c, buf.data());
+    if (e != NQPU_OK) return e;
+    for (uint64_t i = 0; i < c; ++i) out[b + i] = buf[i].re * buf[i].re + buf[i].im * buf[i].im;
+  }
+  return NQPU_OK;
+}
+
+static NqpuError chunk_sums(const State* s0, std::vector<double>* sums, int* cb_out) {
+  State* s = const_cast<State*>(s0);
+  const int cb = chunk_bits_for(s->n);
+  const uint64_t nch = s->dim() >> cb;
+  if (!s->scratch.ensure(nch * 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    Launch<R>::sumsq_chunks(s->amps, s->n, cb, (double*)s->scratch.p, s->stream);
+  });
+  HIPTRY(hipGetLastError(), "sumsq");
+  HIPTRY(hipStreamSynchronize(s->stream), "sumsq");
+  sums->resize(nch);
+  HIPTRY(hipMemcpy(sums->data(), s->scratch.p, nch * 8, hipMemcpyDeviceToHost), "sumsq copy");
+  *cb_out = cb;
+  return NQPU_OK;
+}
+
+NqpuError norm(const State* s, double* out) {
+  std::vector<double> sums;
+  int cb;
+  NqpuError e = chunk_sums(s, &sums, &cb);
+  if (e != NQPU_OK) return e;
+  double t = 0;
+  for (double v : sums) t += v;
+  *out = t;
+  return NQPU_OK;
+}
+
+// tot = sum |a|^2; s1[q] = sum |a_i|^2 * bit_q(i)
+NqpuError zsum_all(const State* s0, double* tot, std::vector<double>* s1) {
+  State* s = const_cast<State*>(s0);
+  const int n = s->n;
+  s1->assign(n, 0.0);
+  if (n < 8) {
+    std::vector<double> p(s->dim());
+    NqpuError e = probabilities(s, p.data());
+    if (e != NQPU_OK) return e;
+    double t = 0;
+    for (uint64_t i = 0; i < s->dim(); ++i) {
+      t += p[i];
+      for (int q = 0; q < n; ++q)
+        if ((i >> q) & 1) (*s1)[q] += p[i];
+    }
+    *tot = t;
+    return NQPU_OK;
+  }
+  const int cb = chunk_bits_for(n);
+  const uint64_t nch = s->dim() >> cb;
+  const int w = cb + 1;
+  if (!s->scratch.ensure(nch * w * 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    Launch<R>::zsum_chunks(s->amps, n, cb, (double*)s->scratch.p, s->stream);
+  });
+  HIPTRY(hipGetLastError(), "zsum");
+  HIPTRY(hipStreamSynchronize(s->stream), "zsum");
+  std::vector<double> rows(nch * w);
+  HIPTRY(hipMemcpy(rows.data(), s->scratch.p, nch * w * 8, hipMemcpyDeviceToHost), "zsum copy");
+  double t = 0;
+  for (uint64_t c = 0; c < nch; ++c) {
+    const double* r = &rows[c * w];
+    t += r[0];
+    for (int q = 0; q < cb; ++q) (*s1)[q] += r[1 + q];
+    for (int q = cb; q < n; ++q)
+      if ((c >> (q - cb)) & 1) (*s1)[q] += r[0];
+  }
+  *tot = t;
+  return NQPU_OK;
+}
+
+NqpuError pauli(const State* s0, uint64_t x, uint64_t z, double* out) {
+  State* s = const_cast<State*>(s0);
+  const int n = s->n;
+  const uint64_t work = x ? (s->dim() >> 1) : s->dim();
+  const int cb = std::min(16, std::max(0, (int)(x ? n - 1 : n)));
+  const uint64_t nch = (work + (1ull << cb) - 1) >> cb;
+  if (!s->scratch.ensure(nch * 16)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    Launch<R>::pauli_chunks(s->amps, n, x, z, cb, (double*)s->scratch.p, s->stream);
+  });
+  HIPTRY(hipGetLastError(), "pauli");
+  HIPTRY(hipStreamSynchronize(s->stream), "pauli");
+  std::vector<double> rows(nch * 2);
+  HIPTRY(hipMemcpy(rows.data(), s->scratch.p, nch * 16, hipMemcpyDeviceToHost), "pauli copy");
+  double sr = 0, si = 0;
+  for (uint64_t c = 0; c < nch; ++c) {
+    sr += rows[2 * c];
+    si += rows[2 * c + 1];
+  }
+  // <P> = Re(i^{#Y} * S)
+  int ny = __builtin_popcountll(x & z) & 3;
+  double r = 0;
+  switch (ny) {
+    case 0: r = sr; break;
+    case 1: r = -si; break;
+    case 2: r = -sr; break;
+    default: r = si; break;
+  }
+  *out = r;
+  return NQPU_OK;
+}
+
+NqpuError scale(State* s, double f) {
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    Launch<R>::scale(s->amps, s->n, f, s->stream);
+  });
+  HIPTRY(hipGetLastError(), "scale");
+  HIPTRY(hipStreamSynchronize(s->stream), "scale");
+  return NQPU_OK;
+}
+
+NqpuError sample(const State* s0, const std::vector<double>& r, uint64_t* out) {
+  State* s = const_cast<State*>(s0);
+  if (r.empty()) return NQPU_OK;
+  std::vector<double> sums;
+  int cb;
+  NqpuError e = chunk_sums(s, &sums, &cb);
+  if (e != NQPU_OK) return e;
+  const uint64_t nch = sums.size();
+  std::vector<double> pre(nch + 1);
+  pre[0] = 0;
+  for (uint64_t c = 0; c < nch; ++c) pre[c + 1] = pre[c] + sums[c];
+  const int sb = std::min(cb, 10);
+  const uint64_t nsub = s->dim() >> sb;
+  const uint64_t shots = r.size();
+  // scratch layout: [pre (nch+1)] [subs (nsub)] [r (shots)] [out (shots)]
+  size_t bytes = (nch + 1 + nsub + shots) * 8 + shots * 8;
+  if (!s->scratch.ensure(bytes)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  double* dpre = (double*)s->scratch.p;
+  double* dsub = dpre + (nch + 1);
+  double* dr = dsub + nsub;
+  uint64_t* dout = (uint64_t*)(dr + shots);
+  HIPTRY(hipMemcpy(dpre, pre.data(), (nch + 1) * 8, hipMemcpyHostToDevice), "prefix upload");
+  HIPTRY(hipMemcpy(dr, r.data(), shots * 8, hipMemcpyHostToDevice), "uniforms upload");
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    Launch<R>::sub_sums(s->amps, s->n, sb, dsub, s->stream);
+    Launch<R>::sample_search(s->amps, s->n, cb, sb, dpre, dsub, dr, shots, dout, s->stream);
+  });
+  HIPTRY(hipGetLastError(), "sample");
+  HIPTRY(hipStreamSynchronize(s->stream), "sample");
+  HIPTRY(hipMemcpy(out, dout, shots * 8, hipMemcpyDeviceToHost), "sample copy");
+  return NQPU_OK;
+}
+
+// c_ffi.rs:473-500: p0 over bit q = 0, p1 = 1 - p0, collapse and renormalise.
+NqpuError measure_qubit(State* s, int q, double r, uint32_t* outcome, double* prob) {
+  double tot;
+  std::vector<double> s1;
+  NqpuError e = zsum_all(s, &tot, &s1);
+  if (e != NQPU_OK) return e;
+  double p0 = tot - s1[q];
+  double p1 = 1.0 - p0;
+  uint32_t o = r < p0 ? 0 : 1;
+  double po = o ? p1 : p0;
+  double rn = 1.0 / std::sqrt(po);
+  dispatch(s, [&](auto* tag) {
+    using R = std::remove_pointer_t<decltype(tag)>;
+    Launch<R>::collapse(s->amps, s->n, q, o, rn, s->stream);
+  });
+  HIPTRY(hipGetLastError(), "collapse");
+  HIPTRY(hipStreamSynchronize(s->stream), "collapse");
+  *outcome = o;
+  *prob = po;
+  return NQPU_OK;
+}
+
+// ------------------------------------------------------------------- RNG
+
+static thread_local uint64_t g_rng = 0x853c49e6748fea9bULL;
+void rng_seed(uint64_t seed) { g_rng = seed == 0 ? 0x853c49e6748fea9bULL : seed; }
+double rng_next() {
+  uint64_t x = g_rng;
+  x ^= x >> 12;
+  x ^= x << 25;
+  x ^= x >> 27;
+  g_rng = x;
+  return (double)((x * 0x2545F4914F6CDD1DULL) >> 11) / (double)(1ull << 53);
+}
+
+}  // namespace nqpu

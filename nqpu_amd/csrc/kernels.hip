@@ -1,0 +1,843 @@
+// kernels.hip — gfx950 kernels of the statevector engine.
+//
+// State layout in HBM: 2^n interleaved complex amplitudes (float2 for c64,
+// double2 for c128), qubit q = bit q of the index (LSB order, as the
+// reference's QuantumState, sdk/rust/src/lib.rs:326-346).
+//
+// Two kernel families apply gates:
+//  * direct kernels: one full-state (or touched-subset) sweep per operation,
+//    one thread per amplitude pair / group enumerated with the reference's
+//    insert-zero-bit scheme (lib.rs:575-632).  These are the per-gate path.
+//  * k_tile: the multi-gate pass.  A workgroup stages a tile of 2^K
+//    amplitudes (K chosen qubits, the low ones contiguous so loads coalesce)
+//    in LDS, applies every operation of the pass there, and writes the tile
+//    back: one HBM read + write of the state for any number of gates.
+// Reductions accumulate in FP64 whatever the amplitude precision.
+#include "engine.h"
+
+#include <algorithm>
+
+namespace nqpu {
+
+template <typename R> struct V2;
+template <> struct V2<float> { using T = float2; };
+template <> struct V2<double> { using T = double2; };
+
+template <typename T> __device__ __forceinline__ T cmul_(T a, T b) {
+  T r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+// m0*x + m1*y for complex m0, m1 given as (re, im) pairs
+template <typename R, typename T>
+__device__ __forceinline__ T mac2(R m0r, R m0i, R m1r, R m1i, T x, T y) {
+  T r;
+  r.x = m0r * x.x - m0i * x.y + m1r * y.x - m1i * y.y;
+  r.y = m0r * x.y + m0i * x.x + m1r * y.y + m1i * y.x;
+  return r;
+}
+
+__device__ __forceinline__ uint64_t ins0(uint64_t v, int q) {
+  uint64_t lo = v & ((1ull << q) - 1);
+  return ((v >> q) << (q + 1)) | lo;
+}
+__device__ __forceinline__ uint32_t ins0u(uint32_t v, int q) {
+  uint32_t lo = v & ((1u << q) - 1);
+  return ((v >> q) << (q + 1)) | lo;
+}
+
+// Sorted bit positions with fixed values (targets and controls of an op).
+struct Fixed {
+  int32_t nf;
+  int32_t p[8];
+};
+__device__ __forceinline__ uint64_t deposit(uint64_t k, const Fixed& f) {
+  for (int j = 0; j < f.nf; ++j) k = ins0(k, f.p[j]);
+  return k;
+}
+
+struct M2 { double v[8]; };
+struct M4 { double v[32]; };
+
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(uint64_t work) {
+  uint64_t g = (work + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+static Fixed make_fixed(std::initializer_list<int> qs, uint64_t cmask) {
+  std::vector<int> v(qs);
+  for (int q = 0; q < 64; ++q)
+    if ((cmask >> q) & 1) v.push_back(q);
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  Fixed f{};
+  f.nf = (int)v.size();
+  for (size_t i = 0; i < v.size() && i < 8; ++i) f.p[i] = v[i];
+  return f;
+}
+
+// ------------------------------------------------------------------ direct
+
+template <typename R, typename T>
+__global__ void k_init_zero(T* st, uint64_t dim) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    T z;
+    z.x = (i == 0) ? R(1) : R(0);
+    z.y = R(0);
+    st[i] = z;
+  }
+}
+
+template <typename R, typename T>
+__global__ void k_fill_basis(T* st, uint64_t dim, uint64_t idx) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    T z;
+    z.x = (i == idx) ? R(1) : R(0);
+    z.y = R(0);
+    st[i] = z;
+  }
+}
+
+// Dense 2x2 on qubit a over the amplitude pairs whose control bits are set.
+template <typename R, typename T>
+__global__ void k_u1(T* st, uint64_t work, Fixed f, uint64_t cmask, int a, M2 mm) {
+  const R m0r = (R)mm.v[0], m0i = (R)mm.v[1], m1r = (R)mm.v[2], m1i = (R)mm.v[3];
+  const R m2r = (R)mm.v[4], m2i = (R)mm.v[5], m3r = (R)mm.v[6], m3i = (R)mm.v[7];
+  const uint64_t am = 1ull << a;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < work;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t i0 = deposit(k, f) | cmask, i1 = i0 | am;
+    T x = st[i0], y = st[i1];
+    st[i0] = mac2<R>(m0r, m0i, m1r, m1i, x, y);
+    st[i1] = mac2<R>(m2r, m2i, m3r, m3i, x, y);
+  }
+}
+
+template <typename R, typename T>
+__global__ void k_flip(T* st, uint64_t work, Fixed f, uint64_t cmask, int a) {
+  const uint64_t am = 1ull << a;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < work;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t i0 = deposit(k, f) | cmask, i1 = i0 | am;
+    T x = st[i0], y = st[i1];
+    st[i0] = y;
+    st[i1] = x;
+  }
+}
+
+// Diagonal on qubit a: `only1` touches only the |1> amplitudes (p0 == 1).
+template <typename R, typename T>
+__global__ void k_d1(T* st, uint64_t work, Fixed f, uint64_t cmask, int a, M2 mm, int only1) {
+  T p0, p1;
+  p0.x = (R)mm.v[0]; p0.y = (R)mm.v[1];
+  p1.x = (R)mm.v[2]; p1.y = (R)mm.v[3];
+  const uint64_t am = 1ull << a;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < work;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t i0 = deposit(k, f) | cmask, i1 = i0 | am;
+    if (!only1) st[i0] = cmul_(p0, st[i0]);
+    st[i1] = cmul_(p1, st[i1]);
+  }
+}
+
+// Dense 4x4 on (a, b), sub-index (bit_b << 1) | bit_a.
+template <typename R, typename T>
+__global__ void k_u2(T* st, uint64_t work, Fixed f, uint64_t cmask, int a, int b, M4 mm) {
+  R m[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) m[i] = (R)mm.v[i];
+  const uint64_t am = 1ull << a, bm = 1ull << b;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < work;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t base = deposit(k, f) | cmask;
+    uint64_t idx[4] = {base, base | am, base | bm, base | am | bm};
+    T v[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) v[s] = st[idx[s]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      T acc;
+      acc.x = 0;
+      acc.y = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        R mr = m[2 * (4 * r + c)], mi = m[2 * (4 * r + c) + 1];
+        acc.x += mr * v[c].x - mi * v[c].y;
+        acc.y += mr * v[c].y + mi * v[c].x;
+      }
+      st[idx[r]] = acc;
+    }
+  }
+}
+
+template <typename R, typename T>
+__global__ void k_d2(T* st, uint64_t work, Fixed f, uint64_t cmask, int a, int b, M4 mm) {
+  T p[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    p[s].x = (R)mm.v[2 * s];
+    p[s].y = (R)mm.v[2 * s + 1];
+  }
+  const uint64_t am = 1ull << a, bm = 1ull << b;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < work;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t base = deposit(k, f) | cmask;
+    uint64_t idx[4] = {base, base | am, base | bm, base | am | bm};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) st[idx[s]] = cmul_(p[s], st[idx[s]]);
+  }
+}
+
+template <typename R, typename T>
+__global__ void k_swap(T* st, uint64_t work, Fixed f, uint64_t cmask, int a, int b) {
+  const uint64_t am = 1ull << a, bm = 1ull << b;
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < work;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    uint64_t base = deposit(k, f) | cmask;
+    T x = st[base | am], y = st[base | bm];
+    st[base | am] = y;
+    st[base | bm] = x;
+  }
+}
+
+struct QList {
+  int32_t k;
+  int32_t q[16];
+};
+
+// Fused diagonal: phase LUT indexed by the bits of up to 16 qubits
+// (execute_diagonal, gate_fusion.rs:905-1033).
+template <typename R, typename T>
+__global__ void k_diagk(T* st, uint64_t dim, QList ql, const double2* lut) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t li = 0;
+    for (int j = 0; j < ql.k; ++j) li |= (uint32_t)((i >> ql.q[j]) & 1) << j;
+    double2 ph = lut[li];
+    T p;
+    p.x = (R)ph.x;
+    p.y = (R)ph.y;
+    st[i] = cmul_(p, st[i]);
+  }
+}
+
+template <typename R, typename T>
+__global__ void k_scale(T* st, uint64_t dim, double f) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    T v = st[i];
+    v.x = (R)(v.x * f);
+    v.y = (R)(v.y * f);
+    st[i] = v;
+  }
+}
+
+// -------------------------------------------------------------- tile pass
+//
+// Tile element t (0 <= t < 2^K) lives at global index base | dep(t), where
+// dep scatters bit j of t to qubit q[j] and base scatters the tile number to
+// the remaining qubits.  Thread `tid` owns t = tid + r*NT for the load and
+// the store; operations re-partition the tile among threads as pairs /
+// groups, so every operation ends with a barrier.
+
+template <typename R, typename T, int NT>
+__device__ __forceinline__ void tile_apply_ops(T* sm, const int K, const DevOp<R>* ops, int nops) {
+  const uint32_t TT = 1u << K;
+  const int tid = threadIdx.x;
+  for (int o = 0; o < nops; ++o) {
+    const DevOp<R>& op = ops[o];
+    const int kind = op.kind, a = op.a, b = op.b;
+    const uint32_t cm = (uint32_t)op.cmask;
+    if (kind == OP_U1) {
+      const R m0r = op.m[0], m0i = op.m[1], m1r = op.m[2], m1i = op.m[3];
+      const R m2r = op.m[4], m2i = op.m[5], m3r = op.m[6], m3i = op.m[7];
+      for (uint32_t p = tid; p < (TT >> 1); p += NT) {
+        uint32_t i0 = ins0u(p, a), i1 = i0 | (1u << a);
+        if ((i0 & cm) != cm) continue;
+        T x = sm[i0], y = sm[i1];
+        sm[i0] = mac2<R>(m0r, m0i, m1r, m1i, x, y);
+        sm[i1] = mac2<R>(m2r, m2i, m3r, m3i, x, y);
+      }
+    } else if (kind == OP_FLIP) {
+      for (uint32_t p = tid; p < (TT >> 1); p += NT) {
+        uint32_t i0 = ins0u(p, a), i1 = i0 | (1u << a);
+        if ((i0 & cm) != cm) continue;
+        T x = sm[i0], y = sm[i1];
+        sm[i0] = y;
+        sm[i1] = x;
+      }
+    } else if (kind == OP_D1) {
+      T p0, p1;
+      p0.x = op.m[0]; p0.y = op.m[1];
+      p1.x = op.m[2]; p1.y = op.m[3];
+      for (uint32_t t = tid; t < TT; t += NT) {
+        if ((t & cm) != cm) continue;
+        sm[t] = cmul_(((t >> a) & 1) ? p1 : p0, sm[t]);
+      }
+    } else if (kind == OP_D2) {
+      T p[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        p[s].x = op.m[2 * s];
+        p[s].y = op.m[2 * s + 1];
+      }
+      for (uint32_t t = tid; t < TT; t += NT) {
+        if ((t & cm) != cm) continue;
+        uint32_t s = ((t >> a) & 1) | (((t >> b) & 1) << 1);
+        sm[t] = cmul_(p[s], sm[t]);
+      }
+    } else if (kind == OP_U2) {
+      const int lo = a < b ? a : b, hi = a < b ? b : a;
+      for (uint32_t g = tid; g < (TT >> 2); g += NT) {
+        uint32_t base = ins0u(ins0u(g, lo), hi);
+        if ((base & cm) != cm) continue;
+        uint32_t idx[4] = {base, base | (1u << a), base | (1u << b), base | (1u << a) | (1u << b)};
+        T v[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) v[s] = sm[idx[s]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          T acc;
+          acc.x = 0;
+          acc.y = 0;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const R mr = op.m[2 * (4 * r + c)], mi = op.m[2 * (4 * r + c) + 1];
+            acc.x += mr * v[c].x - mi * v[c].y;
+            acc.y += mr * v[c].y + mi * v[c].x;
+          }
+          sm[idx[r]] = acc;
+        }
+      }
+    } else if (kind == OP_SWAP) {
+      const int lo = a < b ? a : b, hi = a < b ? b : a;
+      for (uint32_t g = tid; g < (TT >> 2); g += NT) {
+        uint32_t base = ins0u(ins0u(g, lo), hi);
+        if ((base & cm) != cm) continue;
+        uint32_t ia = base | (1u << a), ib = base | (1u << b);
+        T x = sm[ia], y = sm[ib];
+        sm[ia] = y;
+        sm[ib] = x;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R, typename T, int NT, int EPT>
+__global__ __launch_bounds__(NT) void k_tile(T* __restrict__ st, PassArgs pa, uint64_t ntiles,
+                                             const DevOp<R>* __restrict__ ops) {
+  extern __shared__ unsigned char smraw[];
+  T* sm = reinterpret_cast<T*>(smraw);
+  const int K = pa.K;
+  const int tid = threadIdx.x;
+  // dep() of this thread's low index bits (t & (NT-1)); EPT = 2^K / NT.
+  constexpr int LB = (NT == 256) ? 8 : (NT == 128 ? 7 : 6);
+  uint64_t dlo = 0;
+#pragma unroll
+  for (int j = 0; j < LB; ++j) dlo |= (uint64_t)((tid >> j) & 1) << pa.q[j];
+  uint64_t dhi[EPT];
+#pragma unroll
+  for (int r = 0; r < EPT; ++r) {
+    uint64_t d = 0;
+    for (int j = LB; j < K; ++j) d |= (uint64_t)((r >> (j - LB)) & 1) << pa.q[j];
+    dhi[r] = d;
+  }
+  const DevOp<R>* myops = ops + pa.op0;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint64_t base = tile;
+    for (int j = 0; j < K; ++j) base = ins0(base, pa.q[j]);
+    T v[EPT];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) v[r] = st[base | dlo | dhi[r]];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) sm[tid + r * NT] = v[r];
+    __syncthreads();
+    tile_apply_ops<R, T, NT>(sm, K, myops, pa.nops);
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) v[r] = sm[tid + r * NT];
+#pragma unroll
+    for (int r = 0; r < EPT; ++r) st[base | dlo | dhi[r]] = v[r];
+    __syncthreads();
+  }
+}
+
+// Tiles smaller than the workgroup (2^K < NT): element-per-thread loops,
+// any number of tiles.
+template <typename R, typename T, int NT>
+__global__ __launch_bounds__(NT) void k_tile_small(T* __restrict__ st, PassArgs pa, uint64_t ntiles,
+                                                   const DevOp<R>* __restrict__ ops) {
+  extern __shared__ unsigned char smraw[];
+  T* sm = reinterpret_cast<T*>(smraw);
+  const int K = pa.K;
+  const uint32_t TT = 1u << K;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint64_t base = tile;
+    for (int j = 0; j < K; ++j) base = ins0(base, pa.q[j]);
+    for (uint32_t t = threadIdx.x; t < TT; t += NT) {
+      uint64_t d = base;
+      for (int j = 0; j < K; ++j) d |= (uint64_t)((t >> j) & 1) << pa.q[j];
+      sm[t] = st[d];
+    }
+    __syncthreads();
+    tile_apply_ops<R, T, NT>(sm, K, ops + pa.op0, pa.nops);
+    for (uint32_t t = threadIdx.x; t < TT; t += NT) {
+      uint64_t d = base;
+      for (int j = 0; j < K; ++j) d |= (uint64_t)((t >> j) & 1) << pa.q[j];
+      st[d] = sm[t];
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------- reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Block sum (result valid on thread 0).  `red` holds NT/64 doubles.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; ++i) s += red[i];
+  __syncthreads();
+  return s;
+}
+
+template <typename R, typename T>
+__device__ __forceinline__ double nsq(T a) {
+  return (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+}
+
+// out[c] = sum |a_i|^2 over chunk c of 2^cb amplitudes.
+template <typename R, typename T>
+__global__ __launch_bounds__(kBlock) void k_sumsq(const T* st, int cb, uint64_t dim, double* out) {
+  __shared__ double red[kBlock / 64];
+  const uint64_t ch = 1ull << cb, c = blockIdx.x, beg = c * ch, end = min(beg + ch, dim);
+  double s = 0;
+  for (uint64_t i = beg + threadIdx.x; i < end; i += kBlock) s += nsq<R>(st[i]);
+  s = block_sum<kBlock>(s, red);
+  if (threadIdx.x == 0) out[c] = s;
+}
+
+// Row c: [total, S_0, ..., S_{cb-1}] with S_q = sum |a_i|^2 * bit_q(i) over
+// chunk c (2^cb amplitudes; cb >= 8).
+template <typename R, typename T>
+__global__ __launch_bounds__(kBlock) void k_zsum(const T* st, int cb, double* out) {
+  __shared__ double red[kBlock / 64];
+  const uint64_t beg = (uint64_t)blockIdx.x << cb;
+  const int nr = 1 << (cb - 8);
+  double tot = 0, hi[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) hi[j] = 0;
+  for (int r = 0; r < nr; ++r) {
+    double p = nsq<R>(st[beg + threadIdx.x + ((uint64_t)r << 8)]);
+    tot += p;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (j < cb - 8 && ((r >> j) & 1)) hi[j] += p;
+  }
+  double* row = out + (uint64_t)blockIdx.x * (cb + 1);
+  double s = block_sum<kBlock>(tot, red);
+  if (threadIdx.x == 0) row[0] = s;
+  for (int q = 0; q < 8; ++q) {
+    s = block_sum<kBlock>(((threadIdx.x >> q) & 1) ? tot : 0.0, red);
+    if (threadIdx.x == 0) row[1 + q] = s;
+  }
+  for (int j = 0; j < cb - 8; ++j) {
+    double h = 0;
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj)
+      if (jj == j) h = hi[jj];
+    s = block_sum<kBlock>(h, red);
+    if (threadIdx.x == 0) row[9 + j] = s;
+  }
+}
+
+// Pauli string: partial sums of S = sum_i (-1)^popc(i&z) conj(a_{i^x}) a_i
+// over amplitude pairs (i, i^x) with bit h of i clear (h = top bit of x), or
+// over all i when x == 0.  Row c = [Re S_c, Im S_c].
+template <typename R, typename T>
+__global__ __launch_bounds__(kBlock) void k_pauli(const T* st, uint64_t work, int cb, uint64_t x,
+                                                   uint64_t z, int h, double* out) {
+  __shared__ double red[kBlock / 64];
+  const uint64_t ch = 1ull << cb, beg = (uint64_t)blockIdx.x * ch, end = min(beg + ch, work);
+  double sr = 0, si = 0;
+  for (uint64_t k = beg + threadIdx.x; k < end; k += kBlock) {
+    if (x == 0) {
+      double p = nsq<R>(st[k]);
+      sr += (__popcll(k & z) & 1) ? -p : p;
+    } else {
+      uint64_t i = ins0(k, h), j = i ^ x;
+      T ai = st[i], aj = st[j];
+      double si_ = (__popcll(i & z) & 1) ? -1.0 : 1.0;
+      double sj_ = (__popcll(j & z) & 1) ? -1.0 : 1.0;
+      // s_i conj(a_j) a_i + s_j conj(a_i) a_j
+      double cr = (double)aj.x * ai.x + (double)aj.y * ai.y;   // Re(conj(aj) ai)
+      double ci = (double)aj.x * ai.y - (double)aj.y * ai.x;   // Im(conj(aj) ai)
+      sr += si_ * cr + sj_ * cr;
+      si += si_ * ci - sj_ * ci;
+    }
+  }
+  double a = block_sum<kBlock>(sr, red);
+  double b = block_sum<kBlock>(si, red);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = a;
+    out[2 * blockIdx.x + 1] = b;
+  }
+}
+
+template <typename R, typename T>
+__global__ void k_probs(const T* st, uint64_t dim, double* out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = nsq<R>(st[i]);
+}
+
+template <typename R, typename T>
+__global__ void k_to_c128(const T* st, uint64_t count, uint64_t off, double2* out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    T v = st[off + i];
+    out[i] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
+template <typename R, typename T>
+__global__ void k_from_c128(T* st, uint64_t count, uint64_t off, const double2* in) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    double2 v = in[i];
+    T w;
+    w.x = (R)v.x;
+    w.y = (R)v.y;
+    st[off + i] = w;
+  }
+}
+
+template <typename R, typename T>
+__global__ void k_collapse(T* st, uint64_t dim, int q, uint32_t outcome, double renorm) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    T v = st[i];
+    if ((uint32_t)((i >> q) & 1) == outcome) {
+      v.x = (R)(v.x * renorm);
+      v.y = (R)(v.y * renorm);
+    } else {
+      v.x = 0;
+      v.y = 0;
+    }
+    st[i] = v;
+  }
+}
+
+// Sum of |a|^2 over sub-blocks of 2^sb amplitudes.
+template <typename R, typename T>
+__global__ void k_subsums(const T* st, uint64_t nsub, int sb, double* out) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nsub;
+       k += (uint64_t)gridDim.x * blockDim.x) {
+    double s = 0;
+    const uint64_t b = k << sb;
+    for (uint64_t i = 0; i < (1ull << sb); ++i) s += nsq<R>(st[b + i]);
+    out[k] = s;
+  }
+}
+
+// One thread per shot: outcome = first i with r < cumsum_i (the C ABI's
+// linear scan, c_ffi.rs:506-525), found by chunk prefix -> sub-block sums ->
+// amplitudes.  r >= total mass returns dim-1, as the reference does.
+template <typename R, typename T>
+__global__ void k_sample(const T* st, int n, int cb, int sb, const double* cpre,
+                         const double* subs, const double* rr, uint64_t shots, uint64_t* out) {
+  const uint64_t dim = 1ull << n, nch = dim >> cb;
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < shots;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    const double r = rr[s];
+    // first chunk c with r < cpre[c+1]  (cpre has nch+1 entries)
+    uint64_t lo = 0, hi = nch;
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (r < cpre[mid + 1]) hi = mid; else lo = mid + 1;
+    }
+    if (lo >= nch) {
+      out[s] = dim - 1;
+      continue;
+    }
+    double cum = cpre[lo];
+    const uint64_t sub0 = (lo << cb) >> sb, nsub = 1ull << (cb - sb);
+    uint64_t res = ((lo + 1) << cb) - 1;
+    bool found = false;
+    for (uint64_t k = 0; k < nsub && !found; ++k) {
+      const double ss = subs[sub0 + k];
+      if (r < cum + ss) {
+        const uint64_t b = (sub0 + k) << sb;
+        for (uint64_t i = 0; i < (1ull << sb); ++i) {
+          cum += nsq<R>(st[b + i]);
+          if (r < cum) {
+            res = b + i;
+            found = true;
+            break;
+          }
+        }
+        if (!found) {  // rounding between the sub-block sum and the scan
+          res = b + (1ull << sb) - 1;
+          found = true;
+        }
+      } else {
+        cum += ss;
+      }
+    }
+    out[s] = res;
+  }
+}
+
+// Row c = [Re, Im] of sum conj(a_i) b_i over chunk c (FP64 accumulation).
+template <typename TA, typename TB>
+__global__ __launch_bounds__(kBlock) void k_overlap(const TA* a, const TB* b, int cb, uint64_t dim,
+                                                     double* out) {
+  __shared__ double red[kBlock / 64];
+  const uint64_t ch = 1ull << cb, beg = (uint64_t)blockIdx.x * ch, end = min(beg + ch, dim);
+  double sr = 0, si = 0;
+  for (uint64_t i = beg + threadIdx.x; i < end; i += kBlock) {
+    const double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y;
+    sr += ar * br + ai * bi;
+    si += ar * bi - ai * br;
+  }
+  double x = block_sum<kBlock>(sr, red);
+  double y = block_sum<kBlock>(si, red);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = x;
+    out[2 * blockIdx.x + 1] = y;
+  }
+}
+
+void launch_overlap(const void* a, bool a64, const void* b, bool b64, int n, int cb, double* out,
+                    hipStream_t s) {
+  const uint64_t dim = 1ull << n, nch = (dim + (1ull << cb) - 1) >> cb;
+  dim3 g((unsigned)nch), bl(kBlock);
+  if (a64 && b64)
+    hipLaunchKernelGGL((k_overlap<float2, float2>), g, bl, 0, s, (const float2*)a, (const float2*)b, cb, dim, out);
+  else if (a64)
+    hipLaunchKernelGGL((k_overlap<float2, double2>), g, bl, 0, s, (const float2*)a, (const double2*)b, cb, dim, out);
+  else if (b64)
+    hipLaunchKernelGGL((k_overlap<double2, float2>), g, bl, 0, s, (const double2*)a, (const float2*)b, cb, dim, out);
+  else
+    hipLaunchKernelGGL((k_overlap<double2, double2>), g, bl, 0, s, (const double2*)a, (const double2*)b, cb, dim,
+                       out);
+}
+
+__global__ void k_reduce_rows(const double* in, uint64_t rows, int width, double* out) {
+  for (int w = threadIdx.x; w < width; w += blockDim.x) {
+    double s = 0;
+    for (uint64_t r = 0; r < rows; ++r) s += in[r * width + w];
+    out[w] = s;
+  }
+}
+
+// ------------------------------------------------------------ launchers
+
+#define T_ typename V2<R>::T
+
+template <typename R>
+void Launch<R>::init_zero(void* st, int n, hipStream_t s) {
+  uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL((k_init_zero<R, T_>), dim3(grid_for(dim)), dim3(kBlock), 0, s, (T_*)st, dim);
+}
+template <typename R>
+void Launch<R>::fill_basis(void* st, int n, uint64_t idx, hipStream_t s) {
+  uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL((k_fill_basis<R, T_>), dim3(grid_for(dim)), dim3(kBlock), 0, s, (T_*)st, dim, idx);
+}
+
+static M2 to_m2(const cd* m) {
+  M2 r;
+  for (int i = 0; i < 4; ++i) {
+    r.v[2 * i] = m[i].real();
+    r.v[2 * i + 1] = m[i].imag();
+  }
+  return r;
+}
+static M4 to_m4(const cd* m) {
+  M4 r;
+  for (int i = 0; i < 16; ++i) {
+    r.v[2 * i] = m[i].real();
+    r.v[2 * i + 1] = m[i].imag();
+  }
+  return r;
+}
+
+template <typename R>
+void Launch<R>::u1(void* st, int n, int a, uint64_t cmask, const cd* m, hipStream_t s) {
+  Fixed f = make_fixed({a}, cmask);
+  uint64_t work = (1ull << n) >> f.nf;
+  hipLaunchKernelGGL((k_u1<R, T_>), dim3(grid_for(work)), dim3(kBlock), 0, s, (T_*)st, work, f, cmask, a,
+                     to_m2(m));
+}
+template <typename R>
+void Launch<R>::flip(void* st, int n, int a, uint64_t cmask, hipStream_t s) {
+  Fixed f = make_fixed({a}, cmask);
+  uint64_t work = (1ull << n) >> f.nf;
+  hipLaunchKernelGGL((k_flip<R, T_>), dim3(grid_for(work)), dim3(kBlock), 0, s, (T_*)st, work, f, cmask, a);
+}
+template <typename R>
+void Launch<R>::d1(void* st, int n, int a, uint64_t cmask, cd p0, cd p1, hipStream_t s) {
+  Fixed f = make_fixed({a}, cmask);
+  uint64_t work = (1ull << n) >> f.nf;
+  cd m[4] = {p0, p1, 0, 0};
+  int only1 = (p0 == cd(1, 0)) ? 1 : 0;
+  hipLaunchKernelGGL((k_d1<R, T_>), dim3(grid_for(work)), dim3(kBlock), 0, s, (T_*)st, work, f, cmask, a,
+                     to_m2(m), only1);
+}
+template <typename R>
+void Launch<R>::u2(void* st, int n, int a, int b, uint64_t cmask, const cd* m, hipStream_t s) {
+  Fixed f = make_fixed({a, b}, cmask);
+  uint64_t work = (1ull << n) >> f.nf;
+  hipLaunchKernelGGL((k_u2<R, T_>), dim3(grid_for(work)), dim3(kBlock), 0, s, (T_*)st, work, f, cmask, a, b,
+                     to_m4(m));
+}
+template <typename R>
+void Launch<R>::d2(void* st, int n, int a, int b, uint64_t cmask, const cd* p, hipStream_t s) {
+  Fixed f = make_fixed({a, b}, cmask);
+  uint64_t work = (1ull << n) >> f.nf;
+  cd m[16];
+  for (int i = 0; i < 4; ++i) m[i] = p[i];
+  hipLaunchKernelGGL((k_d2<R, T_>), dim3(grid_for(work)), dim3(kBlock), 0, s, (T_*)st, work, f, cmask, a, b,
+                     to_m4(m));
+}
+template <typename R>
+void Launch<R>::swap(void* st, int n, int a, int b, uint64_t cmask, hipStream_t s) {
+  Fixed f = make_fixed({a, b}, cmask);
+  uint64_t work = (1ull << n) >> f.nf;
+  hipLaunchKernelGGL((k_swap<R, T_>), dim3(grid_for(work)), dim3(kBlock), 0, s, (T_*)st, work, f, cmask, a, b);
+}
+template <typename R>
+void Launch<R>::diagk(void* st, int n, int k, const int32_t* q, const void* dlut, hipStream_t s) {
+  QList ql{};
+  ql.k = k;
+  for (int i = 0; i < k && i < 16; ++i) ql.q[i] = q[i];
+  uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL((k_diagk<R, T_>), dim3(grid_for(dim)), dim3(kBlock), 0, s, (T_*)st, dim, ql,
+                     (const double2*)dlut);
+}
+template <typename R>
+void Launch<R>::scale(void* st, int n, double f, hipStream_t s) {
+  uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL((k_scale<R, T_>), dim3(grid_for(dim)), dim3(kBlock), 0, s, (T_*)st, dim, f);
+}
+
+template <typename R, int EPT>
+static void launch_tile(void* st, const PassArgs& pa, const void* dops, hipStream_t s) {
+  const uint64_t ntiles = 1ull << (pa.n - pa.K);
+  const size_t lds = sizeof(T_) << pa.K;
+  uint64_t g = ntiles < (1ull << 20) ? ntiles : (1ull << 20);
+  auto fn = k_tile<R, T_, kBlock, EPT>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(fn, dim3((unsigned)g), dim3(kBlock), lds, s, (T_*)st, pa, ntiles,
+                     (const DevOp<R>*)dops);
+}
+
+template <typename R>
+void Launch<R>::tile_pass(void* st, const PassArgs& pa, const void* dops, hipStream_t s) {
+  switch (pa.K) {
+    case 8: launch_tile<R, 1>(st, pa, dops, s); return;
+    case 9: launch_tile<R, 2>(st, pa, dops, s); return;
+    case 10: launch_tile<R, 4>(st, pa, dops, s); return;
+    case 11: launch_tile<R, 8>(st, pa, dops, s); return;
+    case 12: launch_tile<R, 16>(st, pa, dops, s); return;
+    case 13: launch_tile<R, 32>(st, pa, dops, s); return;
+    default: break;
+  }
+  // K < 8: tiles smaller than a workgroup
+  const uint64_t ntiles = 1ull << (pa.n - pa.K);
+  const uint64_t g = ntiles < (1ull << 20) ? ntiles : (1ull << 20);
+  const size_t lds = sizeof(T_) << pa.K;
+  hipLaunchKernelGGL((k_tile_small<R, T_, kBlock>), dim3((unsigned)g), dim3(kBlock), lds, s, (T_*)st, pa, ntiles,
+                     (const DevOp<R>*)dops);
+}
+
+template <typename R>
+void Launch<R>::sumsq_chunks(const void* st, int n, int cb, double* out, hipStream_t s) {
+  uint64_t dim = 1ull << n;
+  uint64_t nch = (dim + (1ull << cb) - 1) >> cb;
+  hipLaunchKernelGGL((k_sumsq<R, T_>), dim3((unsigned)nch), dim3(kBlock), 0, s, (const T_*)st, cb, dim, out);
+}
+template <typename R>
+void Launch<R>::zsum_chunks(const void* st, int n, int cb, double* out, hipStream_t s) {
+  uint64_t nch = (1ull << n) >> cb;
+  hipLaunchKernelGGL((k_zsum<R, T_>), dim3((unsigned)nch), dim3(kBlock), 0, s, (const T_*)st, cb, out);
+}
+template <typename R>
+void Launch<R>::pauli_chunks(const void* st, int n, uint64_t x, uint64_t z, int cb, double* out,
+                             hipStream_t s) {
+  uint64_t work = x ? (1ull << (n - 1)) : (1ull << n);
+  int h = 0;
+  for (int q = 63; q >= 0; --q)
+    if ((x >> q) & 1) {
+      h = q;
+      break;
+    }
+  uint64_t nch = (work + (1ull << cb) - 1) >> cb;
+  hipLaunchKernelGGL((k_pauli<R, T_>), dim3((unsigned)nch), dim3(kBlock), 0, s, (const T_*)st, work, cb, x, z,
+                     h, out);
+}
+template <typename R>
+void Launch<R>::probs(const void* st, int n, double* out, hipStream_t s) {
+  uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL((k_probs<R, T_>), dim3(grid_for(dim)), dim3(kBlock), 0, s, (const T_*)st, dim, out);
+}
+template <typename R>
+void Launch<R>::to_c128(const void* st, uint64_t count, uint64_t off, void* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_to_c128<R, T_>), dim3(grid_for(count)), dim3(kBlock), 0, s, (const T_*)st, count, off,
+                     (double2*)out);
+}
+template <typename R>
+void Launch<R>::from_c128(void* st, uint64_t count, uint64_t off, const void* in, hipStream_t s) {
+  hipLaunchKernelGGL((k_from_c128<R, T_>), dim3(grid_for(count)), dim3(kBlock), 0, s, (T_*)st, count, off,
+                     (const double2*)in);
+}
+template <typename R>
+void Launch<R>::collapse(void* st, int n, int q, uint32_t outcome, double renorm, hipStream_t s) {
+  uint64_t dim = 1ull << n;
+  hipLaunchKernelGGL((k_collapse<R, T_>), dim3(grid_for(dim)), dim3(kBlock), 0, s, (T_*)st, dim, q, outcome,
+                     renorm);
+}
+template <typename R>
+void Launch<R>::sub_sums(const void* st, int n, int sb, double* out, hipStream_t s) {
+  uint64_t nsub = (1ull << n) >> sb;
+  hipLaunchKernelGGL((k_subsums<R, T_>), dim3(grid_for(nsub)), dim3(kBlock), 0, s, (const T_*)st, nsub, sb, out);
+}
+template <typename R>
+void Launch<R>::sample_search(const void* st, int n, int cb, int sb, const double* cpre,
+                              const double* subs, const double* r, uint64_t shots, uint64_t* out,
+                              hipStream_t s) {
+  hipLaunchKernelGGL((k_sample<R, T_>), dim3(grid_for(shots)), dim3(kBlock), 0, s, (const T_*)st, n, cb, sb, cpre,
+                     subs, r, shots, out);
+}
+
+void launch_reduce_rows(const double* in, uint64_t rows, int width, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(64), 0, s, in, rows, width, out);
+}
+
+template struct Launch<float>;
+template struct Launch<double>;
+
+}  // namespace nqpu

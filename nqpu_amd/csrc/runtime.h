@@ -1,0 +1,125 @@
+// runtime.h — host-side runtime of the engine: device state handles,
+// circuits, gate lowering, the fusion passes and the pass planner.
+#pragma once
+
+#include "../../include/nqpu_hip.h"
+#include "engine.h"
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace nqpu {
+
+// A gate in the circuit IR (gates::Gate, sdk/rust/src/core/gates/gates.rs:632).
+struct Gate {
+  int32_t kind = 0;
+  int32_t nt = 0, nc = 0;
+  int32_t t[3] = {0, 0, 0};
+  int32_t c[3] = {0, 0, 0};
+  double p[4] = {0, 0, 0, 0};
+  cd m[16];
+  uint64_t qubit_mask() const;
+};
+
+struct Circuit {
+  int n = 0;
+  std::vector<Gate> gates;
+};
+
+// Item of the reference fusion pass (FusedOrOriginal, gate_fusion.rs:86-93).
+struct FusedItem {
+  enum Kind { FUSED, FUSED_TWO, DIAGONAL, ORIGINAL } kind = ORIGINAL;
+  int q = 0, qlo = 0, qhi = 0;
+  cd m[16];
+  std::vector<std::pair<int, std::pair<cd, cd>>> single;  // (q, (p0, p1))
+  std::vector<std::pair<std::pair<int, int>, cd>> pairs;  // ((lo, hi), p11)
+  Gate gate;
+  int fused_count = 1;
+};
+
+struct FusionResult {
+  std::vector<FusedItem> items;
+  size_t original_count = 0, fused_count = 0, gates_eliminated = 0;
+};
+
+// One planned tile pass.
+struct Pass {
+  std::vector<int> qubits;  // ascending global positions, size K
+  std::vector<HostOp> ops;  // global numbering
+};
+
+struct PlanOptions {
+  int tile_qubits = 12;
+  int low_qubits = 6;
+};
+
+// lowering and planning (pure host code; unit-testable without a device)
+std::vector<HostOp> lower_gate(const Gate& g, bool reference_semantics);
+std::vector<HostOp> lower_circuit(const Circuit& c, bool reference_semantics);
+std::vector<HostOp> merge_single_qubit(const std::vector<HostOp>& ops);
+std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po);
+FusionResult fuse_gates_reference(const std::vector<Gate>& gates);
+PlanOptions plan_options(const NqpuRunOptions& o, int prec);
+// What run_circuit would launch, computed on the host.
+NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec);
+
+struct DeviceScratch {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DeviceScratch();
+  bool ensure(size_t b);
+};
+
+// HIP events bracketing each launch of a run (when the run asks for timing).
+struct PassTimer {
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  bool on = false;
+  void begin(bool enable) { on = enable; used = 0; }
+  void mark(hipStream_t s);
+  double total_ms();  // sum over (start, stop) pairs; call after the stream is synchronised
+  ~PassTimer();
+};
+
+struct State {
+  int n = 0;
+  int prec = NQPU_PRECISION_C128;
+  void* amps = nullptr;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  NqpuRunStats stats{};
+  DeviceScratch scratch, scratch2, opbuf;
+  PassTimer timer;
+  ~State();
+  size_t amp_size() const { return prec == NQPU_PRECISION_C64 ? 8 : 16; }
+  uint64_t dim() const { return 1ull << n; }
+};
+
+// Error plumbing: the last error detail is thread-local.
+void set_error(const std::string& s);
+const char* last_error();
+NqpuError hip_check(hipError_t e, const char* what);
+
+int device_count();
+NqpuError create_state(int n, int prec, State** out);
+NqpuError clone_state(const State* s, State** out);
+NqpuError reset_state(State* s);
+NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o);
+NqpuError apply_gate_now(State* s, const Gate& g);
+NqpuError get_amplitudes(const State* s, uint64_t off, uint64_t count, NqpuComplex* out);
+NqpuError set_amplitudes(State* s, uint64_t off, uint64_t count, const NqpuComplex* in);
+NqpuError probabilities(const State* s, double* out);
+NqpuError norm(const State* s, double* out);
+NqpuError zsum_all(const State* s, double* tot, std::vector<double>* s1);
+NqpuError pauli(const State* s, uint64_t x, uint64_t z, double* out);
+NqpuError scale(State* s, double f);
+NqpuError overlap(const State* a, const State* b, NqpuComplex* out);
+NqpuError sample(const State* s, const std::vector<double>& r, uint64_t* out);
+NqpuError measure_qubit(State* s, int q, double r, uint32_t* outcome, double* prob);
+
+// The C ABI's thread-local xorshift64* stream (c_ffi.rs:48-67).
+void rng_seed(uint64_t seed);
+double rng_next();
+
+}  // namespace nqpu

@@ -1,0 +1,333 @@
+"""Python host API over the C ABI: gates, circuits and device-resident states.
+
+Mirrors the reference's statevector surface — ``QuantumState`` /
+``QuantumSimulator`` (sdk/rust/src/lib.rs:326-564, 1738-2070) and the C ABI
+(sdk/rust/src/infra/bindings/c_ffi.rs) — with amplitudes resident in HBM.
+Every call goes through libnqpu_hip.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass, field
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+@dataclass(frozen=True)
+class Gate:
+    """A gate as ``gates::Gate`` (sdk/rust/src/core/gates/gates.rs:632)."""
+
+    kind: str
+    targets: tuple = ()
+    controls: tuple = ()
+    params: tuple = ()
+    matrix: object = None
+
+    def qubits(self) -> tuple:
+        return tuple(self.targets) + tuple(self.controls)
+
+
+def _op(g: Gate) -> L.NqpuGateOp:
+    op = L.NqpuGateOp()
+    op.kind = L.GATE_ID[g.kind]
+    op.num_targets = len(g.targets)
+    op.num_controls = len(g.controls)
+    for i, q in enumerate(g.targets):
+        op.targets[i] = int(q)
+    for i, q in enumerate(g.controls):
+        op.controls[i] = int(q)
+    for i, v in enumerate(g.params):
+        op.params[i] = float(v)
+    if g.matrix is not None:
+        flat = np.asarray(g.matrix, dtype=np.complex128).reshape(-1)
+        for i, v in enumerate(flat):
+            op.matrix[i].re = float(v.real)
+            op.matrix[i].im = float(v.imag)
+    return op
+
+
+@dataclass
+class Circuit:
+    """Ordered gate list on ``num_qubits`` qubits (LSB order: qubit q = bit q)."""
+
+    num_qubits: int
+    gates: list = field(default_factory=list)
+
+    def add(self, kind, targets=(), controls=(), params=(), matrix=None) -> "Circuit":
+        self.gates.append(Gate(kind, tuple(targets), tuple(controls), tuple(params), matrix))
+        return self
+
+    # constructors in the reference's argument order (gates.rs:669-760)
+    def h(self, q): return self.add("h", (q,))
+    def x(self, q): return self.add("x", (q,))
+    def y(self, q): return self.add("y", (q,))
+    def z(self, q): return self.add("z", (q,))
+    def s(self, q): return self.add("s", (q,))
+    def t(self, q): return self.add("t", (q,))
+    def sx(self, q): return self.add("sx", (q,))
+    def rx(self, q, a): return self.add("rx", (q,), (), (a,))
+    def ry(self, q, a): return self.add("ry", (q,), (), (a,))
+    def rz(self, q, a): return self.add("rz", (q,), (), (a,))
+    def phase(self, q, a): return self.add("phase", (q,), (), (a,))
+    def u(self, q, theta, phi, lam): return self.add("u", (q,), (), (theta, phi, lam))
+    def cnot(self, c, t): return self.add("cnot", (t,), (c,))
+    cx = cnot
+    def cz(self, c, t): return self.add("cz", (t,), (c,))
+    def cr(self, c, t, a): return self.add("cr", (t,), (c,), (a,))
+    cp = cr
+    def crx(self, c, t, a): return self.add("crx", (t,), (c,), (a,))
+    def cry(self, c, t, a): return self.add("cry", (t,), (c,), (a,))
+    def crz(self, c, t, a): return self.add("crz", (t,), (c,), (a,))
+    def swap(self, a, b): return self.add("swap", (a, b))
+    def iswap(self, a, b): return self.add("iswap", (a, b))
+    def toffoli(self, c1, c2, t): return self.add("toffoli", (t,), (c1, c2))
+    def ccz(self, c1, c2, t): return self.add("ccz", (t,), (c1, c2))
+    def rxx(self, a, b, th): return self.add("rxx", (a, b), (), (th,))
+    def ryy(self, a, b, th): return self.add("ryy", (a, b), (), (th,))
+    def rzz(self, a, b, th): return self.add("rzz", (a, b), (), (th,))
+    def cswap(self, c, a, b): return self.add("cswap", (a, b), (c,))
+    def cu(self, c, t, theta, phi, lam, gamma): return self.add("cu", (t,), (c,), (theta, phi, lam, gamma))
+    def custom1(self, q, m): return self.add("custom1", (q,), (), (), np.asarray(m, np.complex128))
+    def custom2(self, q_hi, q_lo, m): return self.add("custom2", (q_hi, q_lo), (), (), np.asarray(m, np.complex128))
+
+    def extend(self, gates: Iterable[Gate]) -> "Circuit":
+        self.gates.extend(gates)
+        return self
+
+    def native(self) -> "NativeCircuit":
+        return NativeCircuit(self)
+
+
+class NativeCircuit:
+    """A circuit uploaded into the library's circuit IR."""
+
+    def __init__(self, circuit: Circuit):
+        lib = L.load()
+        self.num_qubits = circuit.num_qubits
+        self.ptr = lib.nqpu_circuit_create_ex(circuit.num_qubits)
+        if not self.ptr:
+            raise ValueError(f"invalid qubit count {circuit.num_qubits}")
+        ops = (L.NqpuGateOp * max(1, len(circuit.gates)))()
+        for i, g in enumerate(circuit.gates):
+            ops[i] = _op(g)
+        L.check(lib.nqpu_circuit_add_gates(self.ptr, ops, len(circuit.gates)), "nqpu_circuit_add_gates")
+
+    def plan(self, precision="c64", fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0):
+        lib = L.load()
+        st = L.NqpuRunStats()
+        o = run_options(fusion, semantics, tile_qubits, low_qubits)
+        L.check(lib.nqpu_circuit_plan(self.ptr, ct.byref(o), _prec(precision), ct.byref(st)), "plan")
+        return _stats(st)
+
+    def fusion_counts(self):
+        lib = L.load()
+        fc = L.NqpuFusionCounts()
+        L.check(lib.nqpu_circuit_fusion_counts(self.ptr, ct.byref(fc)), "fusion_counts")
+        return {"original_count": fc.original_count, "fused_count": fc.fused_count,
+                "gates_eliminated": fc.gates_eliminated, "items": fc.items}
+
+    def close(self):
+        if self.ptr:
+            L.load().nqpu_circuit_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_FUSION = {"none": L.FUSION_NONE, "reference": L.FUSION_REFERENCE, "tiled": L.FUSION_TILED}
+_SEMANTICS = {"standard": L.SEMANTICS_STANDARD, "reference": L.SEMANTICS_REFERENCE}
+
+
+def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
+                time_passes=False) -> L.NqpuRunOptions:
+    o = L.NqpuRunOptions()
+    o.fusion = _FUSION[fusion]
+    o.semantics = _SEMANTICS[semantics]
+    o.tile_qubits = tile_qubits
+    o.low_qubits = low_qubits
+    o.flags = L.RUN_TIME_PASSES if time_passes else 0
+    return o
+
+
+def _stats(st: L.NqpuRunStats) -> dict:
+    return {"gates": st.gates, "device_ops": st.device_ops, "passes": st.passes, "bytes": st.bytes_moved,
+            "pass_ms": st.pass_ms, "timed_passes": st.timed_passes}
+
+
+def _prec(p) -> int:
+    if p in ("c64", "complex64", L.PRECISION_C64):
+        return L.PRECISION_C64
+    if p in ("c128", "complex128", L.PRECISION_C128):
+        return L.PRECISION_C128
+    raise ValueError(f"unknown precision {p!r}")
+
+
+def pauli_masks(pauli: str | Sequence[str]) -> tuple[int, int]:
+    """Pauli string (character q acts on qubit q, as lib.rs:465) -> (xmask, zmask)."""
+    x = z = 0
+    for q, c in enumerate(pauli):
+        c = c.upper()
+        if c == "X":
+            x |= 1 << q
+        elif c == "Y":
+            x |= 1 << q
+            z |= 1 << q
+        elif c == "Z":
+            z |= 1 << q
+        elif c != "I":
+            raise ValueError(f"unknown Pauli operator {c!r}")
+    return x, z
+
+
+def device_count() -> int:
+    return int(L.load().nqpu_device_count())
+
+
+def set_seed(seed: int) -> None:
+    L.load().nqpu_set_seed(seed)
+
+
+class StateVector:
+    """A 2^n-amplitude state resident in GPU memory."""
+
+    def __init__(self, num_qubits: int, precision="c128"):
+        lib = L.load()
+        self.num_qubits = num_qubits
+        self.precision = _prec(precision)
+        self.ptr = lib.nqpu_state_create_ex(num_qubits, self.precision)
+        if not self.ptr:
+            detail = (lib.nqpu_last_error_detail() or b"").decode()
+            code = 9 if "no usable" in detail else (8 if "does not fit" in detail else 7)
+            raise L.NqpuError(code, f"nqpu_state_create_ex({num_qubits}, {precision})")
+
+    @property
+    def dim(self) -> int:
+        return 1 << self.num_qubits
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            L.load().nqpu_state_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- execution
+    def run(self, circuit: Circuit | NativeCircuit, fusion="tiled", semantics="standard",
+            tile_qubits=0, low_qubits=0, time_passes=False) -> dict:
+        lib = L.load()
+        nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
+        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes)
+        L.check(lib.nqpu_run(nc.ptr, self.ptr, ct.byref(o)), "nqpu_run")
+        return self.last_stats()
+
+    def last_stats(self) -> dict:
+        st = L.NqpuRunStats()
+        L.check(L.load().nqpu_last_run_stats(self.ptr, ct.byref(st)), "stats")
+        return _stats(st)
+
+    def reset(self):
+        L.check(L.load().nqpu_state_reset(self.ptr), "reset")
+
+    # --- data
+    def amplitudes(self) -> np.ndarray:
+        out = np.empty(self.dim, dtype=np.complex128)
+        L.check(L.load().nqpu_state_get_amplitudes(self.ptr, out.ctypes.data, self.dim), "get_amplitudes")
+        return out
+
+    def set_amplitudes(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        L.check(L.load().nqpu_state_set_amplitudes(self.ptr, a.ctypes.data, len(a)), "set_amplitudes")
+
+    def probabilities(self) -> np.ndarray:
+        out = np.empty(self.dim, dtype=np.float64)
+        L.check(L.load().nqpu_state_probabilities(self.ptr, out.ctypes.data, self.dim), "probabilities")
+        return out
+
+    def norm(self) -> float:
+        return float(L.load().nqpu_state_norm(self.ptr))
+
+    def normalize(self):
+        L.check(L.load().nqpu_state_normalize(self.ptr), "normalize")
+
+    def overlap(self, other: "StateVector") -> complex:
+        """<self|other> in FP64 on the device."""
+        v = L.NqpuComplex()
+        L.check(L.load().nqpu_state_overlap(self.ptr, other.ptr, ct.byref(v)), "overlap")
+        return complex(v.re, v.im)
+
+    def fidelity(self, other: "StateVector") -> float:
+        """|<self|other>|^2 (QuantumState::fidelity, lib.rs:388-403)."""
+        return abs(self.overlap(other)) ** 2
+
+    def expectation_z(self, q: int) -> float:
+        v = ct.c_double()
+        L.check(L.load().nqpu_expectation_z(self.ptr, q, ct.byref(v)), "expectation_z")
+        return v.value
+
+    def expectation_z_all(self) -> np.ndarray:
+        out = np.empty(self.num_qubits, dtype=np.float64)
+        L.check(L.load().nqpu_expectation_z_all(self.ptr, out.ctypes.data, self.num_qubits), "z_all")
+        return out
+
+    def expectation_pauli(self, pauli=None, xmask=None, zmask=None) -> float:
+        if pauli is not None:
+            xmask, zmask = pauli_masks(pauli)
+        v = ct.c_double()
+        L.check(L.load().nqpu_expectation_pauli(self.ptr, int(xmask), int(zmask), ct.byref(v)), "pauli")
+        return v.value
+
+    def expectation_pauli_sum(self, terms) -> float:
+        """terms: [(coeff, pauli string)] as expectation_hamiltonian (lib.rs:518)."""
+        xs, zs, cs = [], [], []
+        for c, p in terms:
+            x, z = pauli_masks(p)
+            xs.append(x); zs.append(z); cs.append(float(c))
+        xa = np.array(xs or [0], dtype=np.uint64)
+        za = np.array(zs or [0], dtype=np.uint64)
+        ca = np.array(cs or [0.0], dtype=np.float64)
+        v = ct.c_double()
+        L.check(L.load().nqpu_expectation_pauli_sum(self.ptr, xa.ctypes.data, za.ctypes.data, ca.ctypes.data,
+                                                    len(xs), ct.byref(v)), "pauli_sum")
+        return v.value
+
+    def sample(self, shots: int, seed: int | None = None) -> np.ndarray:
+        lib = L.load()
+        if seed is not None:
+            lib.nqpu_set_seed(seed)
+        out = np.empty(max(shots, 1), dtype=np.uint64)
+        L.check(lib.nqpu_sample(self.ptr, shots, out.ctypes.data), "sample")
+        return out[:shots]
+
+    def sample_counts(self, shots: int, seed: int | None = None) -> dict:
+        """MSB-first bitstring counts (sample_bitstrings, lib.rs:553-564)."""
+        idx = self.sample(shots, seed)
+        vals, cnt = np.unique(idx, return_counts=True)
+        return {format(int(v), f"0{self.num_qubits}b"): int(c) for v, c in zip(vals, cnt)}
+
+    def measure(self, q: int) -> tuple[int, float]:
+        m = L.NqpuMeasurement()
+        L.check(L.load().nqpu_measure(self.ptr, q, ct.byref(m)), "measure")
+        return int(m.outcome), float(m.probability)
+
+
+def simulate(circuit: Circuit, precision="c128", **run_kw) -> StateVector:
+    sv = StateVector(circuit.num_qubits, precision)
+    sv.run(circuit, **run_kw)
+    return sv

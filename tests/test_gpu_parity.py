@@ -1,0 +1,189 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle and the
+golden vectors.  Tolerances: complex128 amplitudes within 1e-12 of the
+oracle (and <Z_i> within 1e-12); complex64 amplitudes within 2e-6 per
+small circuit and infidelity <= 1e-5 on the large workloads.  Sampling is
+compared index-for-index with the oracle's restatement of the C ABI's seeded
+stream; measurement outcome, probability and collapsed state likewise."""
+import math
+
+import numpy as np
+import pytest
+
+from nqpu_amd import engine as E
+from nqpu_amd.engine import Circuit, Gate, StateVector
+from nqpu_amd.workloads import qft, random_u3_brickwork, tfim_trotter
+from tests.conftest import decode_gates
+
+pytestmark = pytest.mark.gpu
+
+FUSIONS = ["none", "reference", "tiled"]
+TOL = {"c128": 1e-12, "c64": 2e-6}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    n = E.device_count()
+    if n == 0:
+        pytest.fail("no HIP device visible: GPU tests must run on the MI355X")
+    return n
+
+
+def run_gpu(n, gates, prec="c128", fusion="tiled", semantics="standard", psi0=None, **kw):
+    c = Circuit(n)
+    c.extend(gates)
+    with StateVector(n, prec) as sv:
+        if psi0 is not None:
+            sv.set_amplitudes(psi0)
+        sv.run(c, fusion=fusion, semantics=semantics, **kw)
+        return sv.amplitudes()
+
+
+def infidelity(a, b):
+    return 1.0 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("fusion", FUSIONS)
+def test_every_kind_against_golden(golden, prec, fusion):
+    n = int(golden["single_n"])
+    gates = decode_gates(golden, "single")
+    for i, g in enumerate(gates):
+        out = run_gpu(n, [g], prec, fusion, psi0=golden["single_in"][i])
+        err = np.max(np.abs(out - golden["single_out"][i]))
+        assert err < TOL[prec], (g.kind, prec, fusion, err)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("fusion", FUSIONS)
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_random_circuits_against_golden(golden, prec, fusion, idx):
+    n = int(golden[f"circ{idx}_n"])
+    gates = decode_gates(golden, f"circ{idx}")
+    out = run_gpu(n, gates, prec, fusion)
+    tol = 1e-11 if prec == "c128" else 1e-5
+    assert np.max(np.abs(out - golden[f"circ{idx}_out"])) < tol
+
+
+@pytest.mark.parametrize("fusion", FUSIONS)
+def test_reference_semantics_against_oracle(oracle, golden, fusion):
+    """The dispatcher's quirks (CRx as CRz, CRz parallel branch, dropped
+    Rxx/Ryy/Rzz/CSWAP/CU/Custom) reproduced in reference mode."""
+    for idx in (0, 1, 2):
+        n = int(golden[f"circ{idx}_n"])
+        gates = decode_gates(golden, f"circ{idx}")
+        exp = oracle.run(n, gates, quirk=True)
+        out = run_gpu(n, gates, "c128", fusion, semantics="reference")
+        assert np.max(np.abs(out - exp)) < 1e-11
+
+
+@pytest.mark.parametrize("tile,low", [(5, 2), (8, 3), (10, 5), (12, 5), (13, 6)])
+def test_tile_shapes(oracle, tile, low):
+    c = random_u3_brickwork(16, 6, 7)
+    c.extend(qft(16).gates[:60])
+    c.extend(tfim_trotter(16, 2).gates)
+    exp = oracle.run(16, c.gates)
+    out = run_gpu(16, c.gates, "c128", "tiled", tile_qubits=tile, low_qubits=low)
+    assert np.max(np.abs(out - exp)) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def c1(oracle):
+    """20 qubits, 20 U3 + CNOT-brickwork layers, seed 101; the oracle state."""
+    c = random_u3_brickwork(20, 20, 101)
+    return c, oracle.run(20, c.gates)
+
+
+@pytest.mark.parametrize("fusion", FUSIONS)
+def test_c1_complex128_matches_oracle(oracle, c1, fusion):
+    c, exp = c1
+    with StateVector(20, "c128") as sv:
+        sv.run(c, fusion=fusion)
+        out = sv.amplitudes()
+        z = sv.expectation_z_all()
+    assert np.max(np.abs(out - exp)) < 1e-12
+    assert infidelity(out, exp) < 1e-12
+    zexp = np.array([oracle.expectation_z(exp, 20, q) for q in range(20)])
+    assert np.max(np.abs(z - zexp)) < 1e-12
+
+
+def test_c1_complex64_infidelity(c1):
+    c, exp = c1
+    out = run_gpu(20, c.gates, "c64", "tiled")
+    assert infidelity(out, exp) < 1e-5
+
+
+def test_reductions_against_oracle(oracle, c1):
+    c, exp = c1
+    rng = np.random.default_rng(11)
+    with StateVector(20, "c128") as sv:
+        sv.run(c)
+        assert abs(sv.norm() - oracle.norm(exp, 20)) < 1e-12
+        p = sv.probabilities()
+        assert np.max(np.abs(p - oracle.probabilities(exp, 20))) < 1e-15
+        for q in (0, 7, 19):
+            assert abs(sv.expectation_z(q) - oracle.expectation_z(exp, 20, q)) < 1e-12
+        for _ in range(12):
+            s = "".join("IXYZ"[k] for k in rng.integers(0, 4, 20))
+            x, z = E.pauli_masks(s)
+            assert abs(sv.expectation_pauli(s) - oracle.expectation_pauli(exp, 20, x, z)) < 1e-12, s
+        terms = [(-1.0, "ZZ" + "I" * 18), (-0.7, "X" + "I" * 19), (0.3, "I" * 19 + "Y")]
+        ref = sum(cf * oracle.expectation_pauli(exp, 20, *E.pauli_masks(s)) for cf, s in terms)
+        assert abs(sv.expectation_pauli_sum(terms) - ref) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_sampling_matches_oracle_stream(oracle, c1, prec):
+    c, exp = c1
+    with StateVector(20, prec) as sv:
+        sv.run(c)
+        got = sv.sample(20000, seed=2024)
+        amps = sv.amplitudes()
+    want = oracle.sample_cabi(np.ascontiguousarray(amps), 20, 20000, seed=2024)
+    assert np.array_equal(got, want)
+
+
+def test_sampling_edge_cases(oracle):
+    with StateVector(3, "c128") as sv:
+        sv.run(Circuit(3).x(1))
+        assert set(sv.sample(100, seed=1).tolist()) == {2}          # lib.rs:2585 basis state
+        assert len(sv.sample(0)) == 0
+    with StateVector(1, "c64") as sv:
+        sv.run(Circuit(1).h(0))
+        s = sv.sample(10000, seed=9)                               # lib.rs:2568
+        assert abs((s == 0).mean() - 0.5) < 0.05
+
+
+def test_measure_matches_oracle(oracle):
+    c = random_u3_brickwork(10, 4, 3)
+    exp = oracle.run(10, c.gates)
+    with StateVector(10, "c128") as sv:
+        sv.run(c)
+        E.set_seed(77)
+        oracle.rng_seed(77)
+        for q in (0, 4, 9):
+            o, p = sv.measure(q)
+            oo, pp = oracle.measure_qubit(exp, 10, q)
+            assert o == oo and abs(p - pp) < 1e-12
+        assert np.max(np.abs(sv.amplitudes() - exp)) < 1e-12
+
+
+def test_large_state_c64_vs_c128_and_fusion_modes():
+    """28 qubits, depth 24: tiled c64 against tiled c128 and the per-gate
+    c128 path; norm preserved."""
+    c = random_u3_brickwork(28, 24, 102)
+    with StateVector(28, "c128") as a:
+        a.run(c, fusion="tiled")
+        za = a.expectation_z_all()
+        amp_a = a.amplitudes()
+    with StateVector(28, "c128") as b:
+        b.run(c, fusion="none")
+        amp_b = b.amplitudes()
+    assert np.max(np.abs(amp_a - amp_b)) < 1e-12
+    del amp_b
+    with StateVector(28, "c64") as d:
+        d.run(c, fusion="tiled")
+        assert abs(d.norm() - 1) < 1e-4
+        zd = d.expectation_z_all()
+        amp_d = d.amplitudes()
+    assert infidelity(amp_d, amp_a) < 1e-5
+    assert np.max(np.abs(zd - za)) < 1e-5

@@ -1,0 +1,265 @@
+"""Pins the FP64 CPU oracle (oracle/cpu_ref.c) before it is trusted.
+
+Checks, per function: the independent dense brute force of
+tests/golden/make_golden.py, the reference's own known-answer tests
+(sdk/rust/src/lib.rs:2537-2680, c_ffi.rs:1646-2656, traits.rs:506-521),
+closed forms, and invariants.  No GPU needed.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from nqpu_amd.engine import Gate
+from tests.conftest import decode_gates
+
+TOL = 1e-12
+
+
+def G(kind, t=(), c=(), p=(), m=None):
+    return Gate(kind, tuple(t), tuple(c), tuple(p), m)
+
+
+def test_matrices_match_golden_matrices(oracle):
+    """orc_gate_matrix restates gates.rs:70-598; the golden script restates
+    the same table independently in numpy."""
+    from tests.golden import make_golden as mg
+    rng = np.random.default_rng(5)
+    for kind in mg.KINDS:
+        k, t, cq, p, cust = mg.random_gate(rng, 5, kind)
+        mine = oracle.gate_matrix(G(k, t, cq, p, cust))
+        ref = mg.matrix(k, p, cust)
+        assert mine.shape == ref.shape, kind
+        assert np.allclose(mine, ref, atol=1e-14), kind
+        assert np.allclose(mine @ mine.conj().T, np.eye(len(mine)), atol=1e-12), kind
+
+
+def test_every_kind_matches_brute_force(oracle, golden):
+    n = int(golden["single_n"])
+    gates = decode_gates(golden, "single")
+    for i, g in enumerate(gates):
+        a = np.ascontiguousarray(golden["single_in"][i])
+        oracle.apply_gate(a, n, g)
+        err = np.max(np.abs(a - golden["single_out"][i]))
+        assert err < TOL, (g.kind, err)
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_random_circuits_match_brute_force(oracle, golden, idx):
+    n = int(golden[f"circ{idx}_n"])
+    gates = decode_gates(golden, f"circ{idx}")
+    a = oracle.run(n, gates)
+    assert np.max(np.abs(a - golden[f"circ{idx}_out"])) < 1e-11
+    assert abs(oracle.norm(a, n) - 1) < 1e-12
+
+
+def test_pauli_and_z_match_dense(oracle, golden):
+    a = np.ascontiguousarray(golden["circ2_out"])
+    for s, v in zip(golden["pauli_strings"], golden["pauli_values"]):
+        x = sum(1 << q for q, ch in enumerate(str(s)) if ch in "XY")
+        z = sum(1 << q for q, ch in enumerate(str(s)) if ch in "YZ")
+        ny = str(s).count("Y")
+        assert abs(oracle.expectation_pauli(a, 8, x, z) - v) < 1e-12, s
+        # lib.rs:465-514 carries (-1)^{#Y}
+        assert abs(oracle.expectation_pauli(a, 8, x, z, quirk=True) - (-1) ** ny * v) < 1e-12, s
+    for q in range(8):
+        assert abs(oracle.expectation_z(a, 8, q) - golden["z_values"][q]) < 1e-12
+
+
+def test_x_y_expectations_match_pauli(oracle, golden):
+    a = np.ascontiguousarray(golden["circ1_out"])
+    n = 7
+    for q in range(n):
+        assert abs(oracle.expectation_x(a, n, q) - oracle.expectation_pauli(a, n, 1 << q, 0)) < 1e-12
+        assert abs(oracle.expectation_y(a, n, q) - oracle.expectation_pauli(a, n, 1 << q, 1 << q)) < 1e-12
+        assert abs(oracle.expectation_y(a, n, q, quirk=True) + oracle.expectation_y(a, n, q)) < 1e-12
+
+
+# ---- known answers from the reference's own tests
+
+
+def test_bell_known_answers(oracle):
+    # lib.rs:2610-2623, traits.rs:506-521, c_ffi.rs:1752
+    a = oracle.run(2, [G("h", (0,)), G("cnot", (1,), (0,))])
+    assert np.allclose(oracle.probabilities(a, 2), [0.5, 0, 0, 0.5], atol=TOL)
+    assert abs(oracle.expectation_pauli(a, 2, 0, 3) - 1) < TOL   # ZZ
+    assert abs(oracle.expectation_pauli(a, 2, 3, 0) - 1) < TOL   # XX
+    assert abs(oracle.expectation_pauli(a, 2, 3, 3) + 1) < TOL   # YY = -1 on |00>+|11>
+
+
+def test_single_qubit_known_answers(oracle):
+    a = oracle.run(1, [G("h", (0,))])
+    assert abs(oracle.expectation_x(a, 1, 0) - 1) < TOL          # lib.rs:2593
+    a = oracle.run(1, [])
+    assert abs(oracle.expectation_y(a, 1, 0)) < TOL               # lib.rs:2602
+    a = oracle.run(1, [G("sx", (0,)), G("sx", (0,))])
+    assert abs(oracle.probabilities(a, 1)[1] - 1) < TOL           # lib.rs:2635
+    a = oracle.run(1, [G("x", (0,)), G("phase", (0,), (), (math.pi,))])
+    assert abs(a[1].real + 1) < TOL                               # lib.rs:2645
+    a = oracle.run(2, [G("x", (0,)), G("iswap", (0, 1))])
+    assert abs(a[2] - 1j) < TOL                                   # lib.rs:2658
+    a = oracle.run(3, [G("x", (0,)), G("x", (1,)), G("x", (2,)), G("ccz", (2,), (0, 1))])
+    assert abs(a[7] + 1) < TOL                                    # lib.rs:2670
+    # c_ffi.rs:1775-1850: Rx(pi) = -iX, Ry(pi)|0> = |1>, Rz(pi/2) phases
+    a = oracle.run(1, [G("rx", (0,), (), (math.pi,))])
+    assert abs(a[1] + 1j) < TOL
+    a = oracle.run(1, [G("ry", (0,), (), (math.pi,))])
+    assert abs(a[1] - 1) < TOL
+    a = oracle.run(1, [G("h", (0,)), G("rz", (0,), (), (math.pi / 2,))])
+    assert abs(a[0] - np.exp(-1j * math.pi / 4) / math.sqrt(2)) < TOL
+    assert abs(a[1] - np.exp(1j * math.pi / 4) / math.sqrt(2)) < TOL
+    # |+i> has <Y> = +1; lib.rs:444 reports -1
+    a = oracle.run(1, [G("h", (0,)), G("s", (0,))])
+    assert abs(oracle.expectation_y(a, 1, 0) - 1) < TOL
+    assert abs(oracle.expectation_y(a, 1, 0, quirk=True) + 1) < TOL
+
+
+def test_ghz_and_qft_closed_forms(oracle):
+    n = 6
+    a = oracle.run(n, [G("h", (0,))] + [G("cnot", (q + 1,), (q,)) for q in range(n - 1)])
+    p = oracle.probabilities(a, n)
+    assert abs(p[0] - 0.5) < TOL and abs(p[-1] - 0.5) < TOL       # c_ffi.rs:2634
+    from nqpu_amd.workloads import qft
+    c = qft(n)
+    a = oracle.run(n, c.gates)
+    assert np.allclose(a, np.full(1 << n, 2 ** (-n / 2)), atol=1e-12)  # QFT|0> = uniform
+    for q in range(n):
+        assert abs(oracle.expectation_z(a, n, q)) < 1e-12
+
+
+def test_product_state_z_closed_form(oracle):
+    n = 5
+    th = [0.3, 1.1, 2.0, -0.7, 2.9]
+    a = oracle.run(n, [G("ry", (q,), (), (th[q],)) for q in range(n)])
+    for q in range(n):
+        assert abs(oracle.expectation_z(a, n, q) - math.cos(th[q])) < TOL
+        assert abs(oracle.expectation_x(a, n, q) - math.sin(th[q])) < TOL
+
+
+# ---- reference-quirk semantics, from the source's closed forms
+
+
+def test_quirk_crx_is_controlled_rz(oracle):
+    """lib.rs:1292-1358: CRx multiplies |c1 t0> by e^{-i th/2}, |c1 t1> by e^{+i th/2}."""
+    rng = np.random.default_rng(1)
+    a0 = rng.normal(size=8) + 1j * rng.normal(size=8)
+    a0 /= np.linalg.norm(a0)
+    th = 0.77
+    a = a0.copy()
+    oracle.apply_gate(a, 3, G("crx", (2,), (0,), (th,)), quirk=True)
+    b = a0.copy()
+    oracle.apply_gate(b, 3, G("crz", (2,), (0,), (th,)), quirk=False)
+    assert np.allclose(a, b, atol=TOL)
+
+
+def test_quirk_crz_parallel_branch(oracle):
+    """lib.rs:1439-1473: only |c1 t0> changes, by e^{+i th/2}."""
+    rng = np.random.default_rng(2)
+    a0 = rng.normal(size=4) + 1j * rng.normal(size=4)
+    th = 1.3
+    a = a0.copy()
+    oracle.apply_gate(a, 2, G("crz", (1,), (0,), (th,)), quirk=True)
+    exp = a0.copy()
+    exp[1] *= np.exp(1j * th / 2)  # control q0 = 1, target q1 = 0 -> index 1
+    assert np.allclose(a, exp, atol=TOL)
+
+
+@pytest.mark.parametrize("kind,t,c,p", [("rxx", (0, 1), (), (0.4,)), ("ryy", (0, 1), (), (0.4,)),
+                                        ("rzz", (0, 1), (), (0.4,)), ("cswap", (0, 1), (2,), ()),
+                                        ("cu", (0,), (1,), (0.1, 0.2, 0.3, 0.4))])
+def test_quirk_dropped_gates_are_noops(oracle, kind, t, c, p):
+    """ascii_viz.rs:166-177 drops these."""
+    rng = np.random.default_rng(3)
+    a0 = rng.normal(size=8) + 1j * rng.normal(size=8)
+    a = a0.copy()
+    oracle.apply_gate(a, 3, G(kind, t, c, p), quirk=True)
+    assert np.array_equal(a, a0)
+
+
+# ---- u / u2 index conventions (lib.rs:1493, 1536)
+
+
+def test_u2_convention(oracle):
+    rng = np.random.default_rng(4)
+    from tests.golden import make_golden as mg
+    m = mg.random_unitary(rng, 4)
+    a0 = rng.normal(size=16) + 1j * rng.normal(size=16)
+    a = a0.copy()
+    oracle.u2(a, 4, 1, 3, m)  # qubit_lo = 1 is sub-index bit 0
+    exp = mg.apply(a0, 4, m, [3, 1])
+    assert np.allclose(a, exp, atol=1e-12)
+
+
+def test_execute_diagonal_equals_gates(oracle):
+    rng = np.random.default_rng(6)
+    a0 = rng.normal(size=32) + 1j * rng.normal(size=32)
+    a = a0.copy()
+    single = [(0, 1.0, 1j), (3, np.exp(-0.2j), np.exp(0.2j))]
+    pairs = [(1, 4, -1.0), (0, 2, np.exp(0.5j))]
+    oracle.execute_diagonal(a, 5, single, pairs)
+    b = a0.copy()
+    for g in [G("s", (0,)), G("rz", (3,), (), (0.4,)), G("cz", (4,), (1,)), G("cr", (2,), (0,), (0.5,))]:
+        oracle.apply_gate(b, 5, g)
+    assert np.allclose(a, b, atol=TOL)
+
+
+# ---- sampling (c_ffi.rs:48-67, 506-525; lib.rs:527-550)
+
+
+def test_rng_properties(oracle):
+    oracle.rng_seed(0)
+    first = [oracle.rng_next() for _ in range(4)]
+    oracle.rng_seed(0x853c49e6748fea9b)  # seed 0 maps to the default state
+    assert first == [oracle.rng_next() for _ in range(4)]
+    oracle.rng_seed(42)
+    v = np.array([oracle.rng_next() for _ in range(200000)])
+    assert v.min() >= 0 and v.max() < 1
+    assert abs(v.mean() - 0.5) < 3e-3 and abs(v.var() - 1 / 12) < 2e-3
+
+
+def test_sample_bell_and_determinism(oracle):
+    a = oracle.run(2, [G("h", (0,)), G("cnot", (1,), (0,))])
+    s = oracle.sample_cabi(a, 2, 100, seed=42)        # c_ffi.rs:2240
+    assert set(np.unique(s)) == {0, 3}
+    a = oracle.run(2, [G("h", (0,)), G("h", (1,))])
+    r1 = oracle.sample_cabi(a, 2, 50, seed=12345)     # c_ffi.rs:2478
+    r2 = oracle.sample_cabi(a, 2, 50, seed=12345)
+    r3 = oracle.sample_cabi(a, 2, 50, seed=99999)
+    assert np.array_equal(r1, r2) and not np.array_equal(r1, r3)
+
+
+def test_sample_distribution_chi2(oracle, golden):
+    a = np.ascontiguousarray(golden["circ1_out"])
+    n = 7
+    p = oracle.probabilities(a, n)
+    shots = 200000
+    s = oracle.sample_cabi(a, n, shots, seed=7)
+    cnt = np.bincount(s.astype(np.int64), minlength=1 << n)
+    keep = p * shots > 5
+    chi2 = np.sum((cnt[keep] - shots * p[keep]) ** 2 / (shots * p[keep]))
+    dof = keep.sum() - 1
+    assert chi2 < dof + 6 * math.sqrt(2 * dof)
+
+
+def test_sample_cdf_search_semantics(oracle):
+    """linear-scan (first cdf > r) and lib.rs binary search (first cdf >= r)
+    agree away from exact ties and clamp r >= total to dim-1."""
+    a = np.array([0.6, 0.0, 0.8, 0.0], dtype=np.complex128)  # p = .36, 0, .64, 0
+    out = oracle.sample_cdf(a, 2, [0.0, 0.35, 0.36, 0.5, 0.9999, 1.5])
+    assert list(out) == [0, 0, 0, 2, 2, 3]
+
+
+def test_measure_qubit_collapse(oracle):
+    a = oracle.run(2, [G("h", (0,)), G("cnot", (1,), (0,))])
+    oracle.rng_seed(3)
+    o, p = oracle.measure_qubit(a, 2, 0)
+    assert abs(p - 0.5) < TOL
+    assert abs(abs(a[3 * o]) - 1) < TOL
+    assert abs(oracle.norm(a, 2) - 1) < TOL
+
+
+def test_fidelity(oracle, golden):
+    a = np.ascontiguousarray(golden["circ0_out"])
+    assert abs(oracle.fidelity(a, a, 6) - 1) < TOL
+    b = a * np.exp(0.3j)
+    assert abs(oracle.fidelity(a, b, 6) - 1) < TOL
