@@ -208,6 +208,9 @@ typedef struct {
 } NqpuRunOptions;
 
 #define NQPU_RUN_TIME_PASSES 1u
+/* run tiled passes with the LDS-resident tile kernel (one LDS sweep per gate)
+ * instead of the register-resident one */
+#define NQPU_RUN_LDS_TILES 2u
 
 /* Statistics of the last run on a state. */
 typedef struct {
@@ -265,6 +268,13 @@ typedef struct {
   uint64_t items;
 } NqpuFusionCounts;
 NqpuError nqpu_circuit_fusion_counts(const NqpuCircuit* circuit, NqpuFusionCounts* out);
+
+/* Host-only emulation of nqpu_run's tiled execution for a state of the given
+ * precision: the same plan and per-pass programs, interpreted on the CPU in
+ * FP64 over `amps` (len >= 2^n, updated in place).  For testing the planner
+ * without a device; slow. */
+NqpuError nqpu_debug_emulate(const NqpuCircuit* circuit, const NqpuRunOptions* opts, uint32_t precision,
+                             NqpuComplex* amps, uint64_t len);
 
 /* Detail string of the last error on the calling thread. */
 const char* nqpu_last_error_detail(void);

@@ -52,6 +52,7 @@ class NqpuRunOptions(ct.Structure):
 
 
 RUN_TIME_PASSES = 1
+RUN_LDS_TILES = 2
 
 
 class NqpuRunStats(ct.Structure):
@@ -125,6 +126,7 @@ SIGNATURES = {
     "nqpu_device_count": (ct.c_uint32, []),
     "nqpu_circuit_plan": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, ct.POINTER(NqpuRunStats)]),
     "nqpu_circuit_fusion_counts": (_E, [_P, ct.POINTER(NqpuFusionCounts)]),
+    "nqpu_debug_emulate": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, _P, ct.c_uint64]),
 }
 
 _lib = None

@@ -421,6 +421,20 @@ NqpuError nqpu_circuit_plan(const NqpuCircuit* c, const NqpuRunOptions* opts, ui
   return NQPU_OK;
 }
 
+NqpuError nqpu_debug_emulate(const NqpuCircuit* c, const NqpuRunOptions* opts, uint32_t precision,
+                             NqpuComplex* amps, uint64_t len) {
+  if (!c || !amps) return NQPU_NULL_POINTER;
+  if (precision != NQPU_PRECISION_C64 && precision != NQPU_PRECISION_C128) return NQPU_INVALID_PARAMETER;
+  if (Cc(c)->n > 26 || len < (1ull << Cc(c)->n)) return NQPU_INVALID_PARAMETER;
+  NqpuRunOptions o{NQPU_FUSION_TILED, NQPU_SEMANTICS_STANDARD, 0, 0, 0};
+  if (opts) o = *opts;
+  std::vector<cd> a(1ull << Cc(c)->n);
+  for (size_t i = 0; i < a.size(); ++i) a[i] = cd(amps[i].re, amps[i].im);
+  NqpuError e = emulate_tiled(*Cc(c), o, (int)precision, a);
+  for (size_t i = 0; i < a.size(); ++i) amps[i] = NqpuComplex{a[i].real(), a[i].imag()};
+  return e;
+}
+
 NqpuError nqpu_circuit_fusion_counts(const NqpuCircuit* c, NqpuFusionCounts* out) {
   if (!c || !out) return NQPU_NULL_POINTER;
   FusionResult fr = fuse_gates_reference(Cc(c)->gates);

@@ -46,6 +46,31 @@ struct DevOp {
   R m[32];  // interleaved re/im, up to 4x4
 };
 
+// Register-resident tile program (k_tile_reg).  Each of the 256 threads of a
+// tile workgroup holds 2^EB amplitudes; tile bits are "thread slots" (0..7,
+// the thread index bits) or "register slots" (0..EB-1, the register index
+// bits).  map[0..7] / map[8..8+EB) give the tile-local bit held by each slot.
+enum RegOpKind : int32_t {
+  R_REMAP = 0,  // exchange through LDS: slots change from map_old to map_new
+  R_U1 = 1,     // 2x2 on register slot s0
+  R_FLIP = 2,   // X on register slot s0
+  R_D1 = 3,     // diag(m[0], m[1]) on position s0 (>= 16: register slot s0-16, else thread slot)
+  R_D2 = 4,     // diag(m[0..3]) on positions s0 (index bit 0), s1 (index bit 1)
+  R_U2 = 5,     // 4x4 on register slots s0 (index bit 0), s1 (index bit 1)
+  R_SWAP = 6    // exchange register slots s0 and s1 (controlled SWAP)
+};
+
+template <typename R>
+struct RegOp {
+  int32_t kind, s0, s1, pad;
+  uint32_t ct;  // controls on thread slots (bits of the thread index)
+  uint32_t cr;  // controls on register slots (bits of the register index)
+  uint8_t map_old[16];
+  uint8_t map_new[16];
+  uint64_t pad2;
+  R m[32];
+};
+
 constexpr int kMaxTile = 14;  // maximum tile qubits (LDS: 2^13 * 16 B = 128 KiB)
 
 // One tile pass: tile qubits (global positions, ascending), the ops it
@@ -72,6 +97,8 @@ struct Launch {
   static void swap(void* st, int n, int a, int b, uint64_t cmask, hipStream_t s);
   static void diagk(void* st, int n, int k, const int32_t* q, const void* dlut, hipStream_t s);
   static void tile_pass(void* st, const PassArgs& pa, const void* dops, hipStream_t s);
+  // register-resident tile pass; pa.K = 8 + eb, program = RegOp<R>[pa.op0 ..]
+  static void tile_pass_reg(void* st, const PassArgs& pa, int eb, const void* prog, hipStream_t s);
   static void scale(void* st, int n, double f, hipStream_t s);
   // FP64 reductions: per-chunk partials, then deterministic final sums.
   static void sumsq_chunks(const void* st, int n, int chunk_bits, double* out, hipStream_t s);

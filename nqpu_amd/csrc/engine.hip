@@ -718,9 +718,330 @@ static std::vector<HostOp> diagonal_ops(const FusedItem& it) {
   return v;
 }
 
+// Program of one register-resident tile pass (k_tile_reg).  Slots: thread
+// slots 0..7 (code i) and register slots 0..eb-1 (code 16+j); map[] gives the
+// tile-local bit each slot holds.  Ops are issued in order; an op runs in the
+// current phase when its targets are register slots (diagonals and controls
+// can sit anywhere; an uncontrolled SWAP only relabels two slots), and no op
+// may overtake an earlier pending op on a shared qubit.  When nothing more
+// can run, an R_REMAP moves the targets of the next pending ops into
+// register slots.  The pass ends in the canonical slot assignment the store
+// expects.
+template <typename R>
+std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb) {
+  const int K = (int)p.qubits.size();
+  int loc[64];
+  for (int q = 0; q < 64; ++q) loc[q] = -1;
+  for (int j = 0; j < K; ++j) loc[p.qubits[j]] = j;
+  struct L {
+    int kind, a, b;
+    uint32_t cm;
+    const HostOp* h;
+    bool done;
+  };
+  std::vector<L> ops;
+  ops.reserve(p.ops.size());
+  for (const HostOp& o : p.ops) {
+    L l{o.kind, loc[o.a], 0, 0, &o, false};
+    if (o.kind == OP_U2 || o.kind == OP_SWAP || o.kind == OP_D2) l.b = loc[o.b];
+    for (int q = 0; q < 64; ++q)
+      if ((o.cmask >> q) & 1) l.cm |= 1u << loc[q];
+    ops.push_back(l);
+  }
+  auto two = [](int k) { return k == OP_U2 || k == OP_SWAP || k == OP_D2; };
+  auto qmask = [&](const L& o) { return o.cm | (1u << o.a) | (two(o.kind) ? (1u << o.b) : 0u); };
+  auto targets = [&](const L& o) -> uint32_t {
+    switch (o.kind) {
+      case OP_U1: case OP_FLIP: return 1u << o.a;
+      case OP_U2: return (1u << o.a) | (1u << o.b);
+      case OP_SWAP: return o.cm ? ((1u << o.a) | (1u << o.b)) : 0u;
+      default: return 0u;
+    }
+  };
+  uint8_t map[16], canon[16];
+  for (int i = 0; i < 16; ++i) map[i] = canon[i] = (uint8_t)(i < 8 + eb ? i : 0);
+  auto slot = [&](int lb) {
+    for (int i = 0; i < 8; ++i)
+      if (map[i] == lb) return i;
+    for (int j = 0; j < eb; ++j)
+      if (map[8 + j] == lb) return 16 + j;
+    return -1;
+  };
+  auto regmask = [&]() {
+    uint32_t m = 0;
+    for (int j = 0; j < eb; ++j) m |= 1u << map[8 + j];
+    return m;
+  };
+  std::vector<RegOp<R>> prog;
+  auto emit_remap = [&](const uint8_t* nm) {
+    RegOp<R> r{};
+    r.kind = R_REMAP;
+    for (int i = 0; i < 16; ++i) {
+      r.map_old[i] = map[i];
+      r.map_new[i] = nm[i];
+    }
+    prog.push_back(r);
+    for (int i = 0; i < 16; ++i) map[i] = nm[i];
+  };
+  auto emit = [&](const L& o) {
+    if (o.kind == OP_SWAP && o.cm == 0) {  // relabel
+      int sa = slot(o.a), sb = slot(o.b);
+      int ia = sa >= 16 ? 8 + (sa - 16) : sa, ib = sb >= 16 ? 8 + (sb - 16) : sb;
+      std::swap(map[ia], map[ib]);
+      return;
+    }
+    RegOp<R> r{};
+    for (int q = 0; q < K; ++q)
+      if ((o.cm >> q) & 1) {
+        int sq = slot(q);
+        if (sq >= 16) r.cr |= 1u << (sq - 16);
+        else r.ct |= 1u << sq;
+      }
+    const HostOp& h = *o.h;
+    int nm = 0;
+    switch (o.kind) {
+      case OP_U1: r.kind = R_U1; r.s0 = slot(o.a) - 16; nm = 4; break;
+      case OP_FLIP: r.kind = R_FLIP; r.s0 = slot(o.a) - 16; break;
+      case OP_D1: r.kind = R_D1; r.s0 = slot(o.a); nm = 2; break;
+      case OP_D2: r.kind = R_D2; r.s0 = slot(o.a); r.s1 = slot(o.b); nm = 4; break;
+      case OP_U2: r.kind = R_U2; r.s0 = slot(o.a) - 16; r.s1 = slot(o.b) - 16; nm = 16; break;
+      case OP_SWAP: r.kind = R_SWAP; r.s0 = slot(o.a) - 16; r.s1 = slot(o.b) - 16; break;
+      default: return;
+    }
+    for (int i = 0; i < nm; ++i) {
+      r.m[2 * i] = (R)h.m[i].real();
+      r.m[2 * i + 1] = (R)h.m[i].imag();
+    }
+    prog.push_back(r);
+  };
+  size_t remaining = ops.size();
+  while (true) {
+    // a relabelling SWAP can make earlier-skipped ops runnable: rescan
+    bool progress = true;
+    while (progress && remaining) {
+      progress = false;
+      uint32_t blocked = 0;
+      for (L& o : ops) {
+        if (o.done) continue;
+        const uint32_t qm = qmask(o);
+        if ((qm & blocked) == 0 && (targets(o) & ~regmask()) == 0) {
+          emit(o);
+          o.done = true;
+          --remaining;
+          progress = true;
+        } else {
+          blocked |= qm;
+        }
+      }
+    }
+    if (remaining == 0) break;
+    uint32_t blocked = 0;
+    // register set for the next phase: targets of the pending ops, in order
+    uint32_t need = 0;
+    for (const L& o : ops) {
+      if (o.done) continue;
+      const uint32_t qm = qmask(o), tg = targets(o);
+      if ((qm & blocked) == 0 && __builtin_popcount(need | tg) <= eb) need |= tg;
+      else blocked |= qm;
+      if (__builtin_popcount(need) == eb) break;
+    }
+    uint8_t nm[16];
+    for (int i = 0; i < 16; ++i) nm[i] = map[i];
+    std::vector<int> free_reg, evicted;
+    for (int j = 0; j < eb; ++j) {
+      if ((need >> map[8 + j]) & 1) need &= ~(1u << map[8 + j]);  // already in place
+      else free_reg.push_back(j);
+    }
+    size_t fi = 0;
+    for (int i = 0; i < 8 && need; ++i) {
+      const int lb = map[i];
+      if (!((need >> lb) & 1)) continue;
+      // thread slot i holds a needed bit: swap it with a register slot
+      const int j = free_reg[fi++];
+      nm[i] = map[8 + j];
+      nm[8 + j] = (uint8_t)lb;
+      need &= ~(1u << lb);
+    }
+    emit_remap(nm);
+  }
+  if (std::memcmp(map, canon, 16) != 0) emit_remap(canon);
+  return prog;
+}
+
+// Host emulation of one k_tile_reg pass over a complex128 vector, following
+// the kernel's thread / register / LDS semantics step by step.
+static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std::vector<RegOp<double>>& prog,
+                             int eb) {
+  const int K = 8 + eb, E = 1 << eb;
+  const uint64_t ntiles = 1ull << (n - K);
+  std::vector<cd> v(256 * E), lds(size_t(1) << K);
+  auto dep = [&](uint32_t t) {
+    uint64_t d = 0;
+    for (int j = 0; j < K; ++j)
+      if ((t >> j) & 1) d |= 1ull << p.qubits[j];
+    return d;
+  };
+  auto tmap = [&](const uint8_t* m, int tid, int r) {
+    uint32_t t = 0;
+    for (int i = 0; i < 8; ++i) t |= (uint32_t)((tid >> i) & 1) << m[i];
+    for (int j = 0; j < eb; ++j) t |= (uint32_t)((r >> j) & 1) << m[8 + j];
+    return t;
+  };
+  for (uint64_t tile = 0; tile < ntiles; ++tile) {
+    uint64_t base = tile;
+    for (int j = 0; j < K; ++j) {
+      const int q = p.qubits[j];
+      base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1));
+    }
+    for (int tid = 0; tid < 256; ++tid)
+      for (int r = 0; r < E; ++r) v[tid * E + r] = a[base | dep((uint32_t)tid | ((uint32_t)r << 8))];
+    for (const RegOp<double>& op : prog) {
+      if (op.kind == R_REMAP) {
+        for (int tid = 0; tid < 256; ++tid)
+          for (int r = 0; r < E; ++r) lds[tmap(op.map_old, tid, r)] = v[tid * E + r];
+        for (int tid = 0; tid < 256; ++tid)
+          for (int r = 0; r < E; ++r) v[tid * E + r] = lds[tmap(op.map_new, tid, r)];
+        continue;
+      }
+      cd m[16];
+      for (int i = 0; i < 16; ++i) m[i] = cd(op.m[2 * i], op.m[2 * i + 1]);
+      for (int tid = 0; tid < 256; ++tid) {
+        if ((tid & op.ct) != op.ct) continue;
+        cd* w = &v[tid * E];
+        for (int r = 0; r < E; ++r) {
+          if ((r & op.cr) != (int)op.cr) continue;
+          const int j0 = op.s0, j1 = op.s1;
+          switch (op.kind) {
+            case R_U1:
+              if (!((r >> j0) & 1)) {
+                cd x = w[r], y = w[r | (1 << j0)];
+                w[r] = m[0] * x + m[1] * y;
+                w[r | (1 << j0)] = m[2] * x + m[3] * y;
+              }
+              break;
+            case R_FLIP:
+              if (!((r >> j0) & 1)) std::swap(w[r], w[r | (1 << j0)]);
+              break;
+            case R_D1:
+            case R_D2: {
+              auto bitof = [&](int code) { return code >= 16 ? (r >> (code - 16)) & 1 : (tid >> code) & 1; };
+              int s = bitof(j0) | (op.kind == R_D2 ? bitof(j1) << 1 : 0);
+              w[r] *= m[s];
+              break;
+            }
+            case R_U2:
+              if (!((r >> j0) & 1) && !((r >> j1) & 1)) {
+                int idx[4] = {r, r | (1 << j0), r | (1 << j1), r | (1 << j0) | (1 << j1)};
+                cd x[4] = {w[idx[0]], w[idx[1]], w[idx[2]], w[idx[3]]};
+                for (int row = 0; row < 4; ++row) {
+                  cd acc = 0;
+                  for (int c = 0; c < 4; ++c) acc += m[4 * row + c] * x[c];
+                  w[idx[row]] = acc;
+                }
+              }
+              break;
+            case R_SWAP:
+              if (((r >> j0) & 1) && !((r >> j1) & 1)) std::swap(w[r], w[r ^ (1 << j0) ^ (1 << j1)]);
+              break;
+            default: break;
+          }
+        }
+      }
+    }
+    for (int tid = 0; tid < 256; ++tid)
+      for (int r = 0; r < E; ++r) a[base | dep((uint32_t)tid | ((uint32_t)r << 8))] = v[tid * E + r];
+  }
+}
+
+// Apply one op (global numbering) to a complex128 vector on the host.
+static void emulate_op(std::vector<cd>& a, int n, const HostOp& o) {
+  const uint64_t dim = 1ull << n;
+  for (uint64_t i = 0; i < dim; ++i) {
+    if ((i & o.cmask) != o.cmask) continue;
+    switch (o.kind) {
+      case OP_U1:
+        if (!((i >> o.a) & 1)) {
+          cd x = a[i], y = a[i | bit(o.a)];
+          a[i] = o.m[0] * x + o.m[1] * y;
+          a[i | bit(o.a)] = o.m[2] * x + o.m[3] * y;
+        }
+        break;
+      case OP_FLIP:
+        if (!((i >> o.a) & 1)) std::swap(a[i], a[i | bit(o.a)]);
+        break;
+      case OP_D1: a[i] *= o.m[(i >> o.a) & 1]; break;
+      case OP_D2: a[i] *= o.m[((i >> o.a) & 1) | (((i >> o.b) & 1) << 1)]; break;
+      case OP_U2:
+        if (!((i >> o.a) & 1) && !((i >> o.b) & 1)) {
+          uint64_t idx[4] = {i, i | bit(o.a), i | bit(o.b), i | bit(o.a) | bit(o.b)};
+          cd x[4] = {a[idx[0]], a[idx[1]], a[idx[2]], a[idx[3]]};
+          for (int r = 0; r < 4; ++r) a[idx[r]] = o.m[4 * r] * x[0] + o.m[4 * r + 1] * x[1] + o.m[4 * r + 2] * x[2] +
+                                                  o.m[4 * r + 3] * x[3];
+        }
+        break;
+      case OP_SWAP:
+        if (((i >> o.a) & 1) && !((i >> o.b) & 1)) std::swap(a[i], a[(i ^ bit(o.a)) | bit(o.b)]);
+        break;
+      default: break;
+    }
+  }
+}
+
+NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std::vector<cd>& a) {
+  const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
+  PlanOptions po = plan_options(o, prec);
+  std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
+  std::vector<Pass> passes = plan_tiles(ops, c.n, po);
+  const int K = std::min(po.tile_qubits, c.n);
+  const bool reg = po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13;
+  for (const Pass& p : passes) {
+    if (reg) emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8), K - 8);
+    else
+      for (const HostOp& op : p.ops) emulate_op(a, c.n, op);
+  }
+  return NQPU_OK;
+}
+
+template <typename R>
+static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb) {
+  std::vector<RegOp<R>> prog;
+  std::vector<PassArgs> args;
+  for (const Pass& p : passes) {
+    PassArgs pa{};
+    pa.K = (int)p.qubits.size();
+    pa.n = s->n;
+    pa.op0 = (int)prog.size();
+    for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
+    std::vector<RegOp<R>> pr = build_reg_program<R>(p, eb);
+    pa.nops = (int)pr.size();
+    prog.insert(prog.end(), pr.begin(), pr.end());
+    args.push_back(pa);
+  }
+  size_t ob = std::max<size_t>(prog.size(), 1) * sizeof(RegOp<R>);
+  HIPTRY(hipStreamSynchronize(s->stream), "sync before program upload");
+  if (!s->opbuf.ensure(ob)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  if (!prog.empty())
+    HIPTRY(hipMemcpy(s->opbuf.p, prog.data(), prog.size() * sizeof(RegOp<R>), hipMemcpyHostToDevice),
+           "program upload");
+  for (const PassArgs& pa : args) {
+    s->timer.mark(s->stream);
+    Launch<R>::tile_pass_reg(s->amps, pa, eb, s->opbuf.p, s->stream);
+    s->timer.mark(s->stream);
+    HIPTRY(hipGetLastError(), "register tile pass launch");
+  }
+  s->stats.passes += passes.size();
+  s->stats.bytes_moved += (double)passes.size() * 2.0 * (double)(s->amp_size() << s->n);
+  return NQPU_OK;
+}
+
 template <typename R>
 static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanOptions& po) {
   std::vector<Pass> passes = plan_tiles(ops, s->n, po);
+  const int K = std::min(po.tile_qubits, s->n);
+  if (po.registers && std::is_same<R, float>::value && K >= 11 && K <= 13) {
+    s->stats.device_ops += ops.size();
+    return run_tiled_reg<R>(s, passes, K - 8);
+  }
   std::vector<DevOp<R>> dops;
   std::vector<PassArgs> args;
   for (const Pass& p : passes) {
@@ -777,6 +1098,7 @@ PlanOptions plan_options(const NqpuRunOptions& o, int prec) {
   if (po.tile_qubits < 5) po.tile_qubits = 5;
   po.low_qubits = o.low_qubits ? (int)o.low_qubits : (prec == NQPU_PRECISION_C64 ? 5 : 4);
   po.low_qubits = std::min(po.low_qubits, po.tile_qubits);
+  po.registers = (o.flags & NQPU_RUN_LDS_TILES) == 0;
   return po;
 }
 

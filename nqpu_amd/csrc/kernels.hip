@@ -282,16 +282,14 @@ __device__ __forceinline__ void tile_apply_ops(T* sm, const int K, const DevOp<R
         sm[t] = cmul_(((t >> a) & 1) ? p1 : p0, sm[t]);
       }
     } else if (kind == OP_D2) {
-      T p[4];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        p[s].x = op.m[2 * s];
-        p[s].y = op.m[2 * s + 1];
-      }
       for (uint32_t t = tid; t < TT; t += NT) {
         if ((t & cm) != cm) continue;
-        uint32_t s = ((t >> a) & 1) | (((t >> b) & 1) << 1);
-        sm[t] = cmul_(p[s], sm[t]);
+        const bool ba = (t >> a) & 1, bb = (t >> b) & 1;
+        const int s = (bb ? 4 : 0) + (ba ? 2 : 0);  // index of the phase's real part
+        T ph;
+        ph.x = op.m[s];
+        ph.y = op.m[s + 1];
+        sm[t] = cmul_(ph, sm[t]);
       }
     } else if (kind == OP_U2) {
       const int lo = a < b ? a : b, hi = a < b ? b : a;
@@ -394,6 +392,229 @@ __global__ __launch_bounds__(NT) void k_tile_small(T* __restrict__ st, PassArgs 
       st[d] = sm[t];
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------- register tile pass
+//
+// Each thread keeps E = 2^EB amplitudes of the tile in registers: amplitude
+// r of thread tid is tile element t = sum_i bit_i(tid) << map[i] +
+// sum_j bit_j(r) << map[8 + j].  A gate whose targets are register slots is
+// applied in registers; controls and diagonal factors may sit on any slot
+// (they only select or scale elements); everything else waits for an
+// R_REMAP, which moves the tile through LDS into a new slot assignment.
+// LDS addresses are XOR-swizzled so that lanes spread over banks whichever
+// tile bits they vary.
+
+__device__ __forceinline__ uint32_t lds_swz(uint32_t t) { return t ^ (((t >> 5) ^ (t >> 10)) & 31u); }
+
+template <typename R, typename T, int EB, int J>
+__device__ __forceinline__ void reg_u1(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
+  const R m0r = op.m[0], m0i = op.m[1], m1r = op.m[2], m1i = op.m[3];
+  const R m2r = op.m[4], m2i = op.m[5], m3r = op.m[6], m3i = op.m[7];
+  const uint32_t cr = op.cr;
+#pragma unroll
+  for (int r0 = 0; r0 < (1 << EB); ++r0) {
+    if (r0 & (1 << J)) continue;
+    if ((r0 & cr) != cr) continue;
+    const int r1 = r0 | (1 << J);
+    T x = v[r0], y = v[r1];
+    T nx = mac2<R>(m0r, m0i, m1r, m1i, x, y);
+    T ny = mac2<R>(m2r, m2i, m3r, m3i, x, y);
+    v[r0] = on ? nx : x;
+    v[r1] = on ? ny : y;
+  }
+}
+
+template <typename R, typename T, int EB, int J>
+__device__ __forceinline__ void reg_flip(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
+  const uint32_t cr = op.cr;
+#pragma unroll
+  for (int r0 = 0; r0 < (1 << EB); ++r0) {
+    if (r0 & (1 << J)) continue;
+    if ((r0 & cr) != cr) continue;
+    const int r1 = r0 | (1 << J);
+    T x = v[r0], y = v[r1];
+    v[r0] = on ? y : x;
+    v[r1] = on ? x : y;
+  }
+}
+
+template <typename R, typename T, int EB, int J1, int J2>
+__device__ __forceinline__ void reg_u2(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
+  const uint32_t cr = op.cr;
+#pragma unroll
+  for (int r = 0; r < (1 << EB); ++r) {
+    if ((r & (1 << J1)) || (r & (1 << J2))) continue;
+    if ((r & cr) != cr) continue;
+    const int idx[4] = {r, r | (1 << J1), r | (1 << J2), r | (1 << J1) | (1 << J2)};
+    T x[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) x[s] = v[idx[s]];
+#pragma unroll
+    for (int row = 0; row < 4; ++row) {
+      T acc;
+      acc.x = 0;
+      acc.y = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const R mr = op.m[2 * (4 * row + c)], mi = op.m[2 * (4 * row + c) + 1];
+        acc.x += mr * x[c].x - mi * x[c].y;
+        acc.y += mr * x[c].y + mi * x[c].x;
+      }
+      v[idx[row]] = on ? acc : x[row];
+    }
+  }
+}
+
+template <typename R, typename T, int EB, int J1, int J2>
+__device__ __forceinline__ void reg_swap(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
+  const uint32_t cr = op.cr;
+#pragma unroll
+  for (int r = 0; r < (1 << EB); ++r) {
+    if (!(r & (1 << J1)) || (r & (1 << J2))) continue;
+    if ((r & cr) != cr) continue;
+    const int r2 = r ^ (1 << J1) ^ (1 << J2);
+    T x = v[r], y = v[r2];
+    v[r] = on ? y : x;
+    v[r2] = on ? x : y;
+  }
+}
+
+template <typename R, typename T, int EB>
+__device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, int tid, bool on) {
+  T p0, p1, p2, p3;
+  p0.x = op.m[0]; p0.y = op.m[1];
+  p1.x = op.m[2]; p1.y = op.m[3];
+  p2.x = op.m[4]; p2.y = op.m[5];
+  p3.x = op.m[6]; p3.y = op.m[7];
+  const bool two = op.kind == R_D2;
+  const int a = op.s0, b = op.s1;
+  const bool areg = a >= 16, breg = b >= 16;
+  const int as = a & 15, bs = b & 15;
+  const bool ta = (tid >> as) & 1, tb = two && ((tid >> bs) & 1);
+  const uint32_t cr = op.cr;
+#pragma unroll
+  for (int r = 0; r < (1 << EB); ++r) {
+    if ((r & cr) != cr) continue;
+    const bool ba = areg ? ((r >> as) & 1) : ta;
+    const bool bb = two && (breg ? ((r >> bs) & 1) : tb);
+    T lo, hi, ph;
+    lo.x = ba ? p1.x : p0.x;
+    lo.y = ba ? p1.y : p0.y;
+    hi.x = ba ? p3.x : p2.x;
+    hi.y = ba ? p3.y : p2.y;
+    ph.x = bb ? hi.x : lo.x;
+    ph.y = bb ? hi.y : lo.y;
+    T nv = cmul_(ph, v[r]);
+    v[r] = on ? nv : v[r];
+  }
+}
+
+template <typename R, typename T, int EB>
+__device__ __forceinline__ void reg_remap(T (&v)[1 << EB], T* sm, const RegOp<R>& op, int tid) {
+  uint32_t tp = 0, rp[EB];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) tp |= (uint32_t)((tid >> i) & 1) << op.map_old[i];
+#pragma unroll
+  for (int j = 0; j < EB; ++j) rp[j] = 1u << op.map_old[8 + j];
+#pragma unroll
+  for (int r = 0; r < (1 << EB); ++r) {
+    uint32_t t = tp;
+#pragma unroll
+    for (int j = 0; j < EB; ++j)
+      if ((r >> j) & 1) t |= rp[j];
+    sm[lds_swz(t)] = v[r];
+  }
+  __syncthreads();
+  tp = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) tp |= (uint32_t)((tid >> i) & 1) << op.map_new[i];
+#pragma unroll
+  for (int j = 0; j < EB; ++j) rp[j] = 1u << op.map_new[8 + j];
+#pragma unroll
+  for (int r = 0; r < (1 << EB); ++r) {
+    uint32_t t = tp;
+#pragma unroll
+    for (int j = 0; j < EB; ++j)
+      if ((r >> j) & 1) t |= rp[j];
+    v[r] = sm[lds_swz(t)];
+  }
+  __syncthreads();
+}
+
+#define NQPU_SLOT_CASES(FN, ...)                                      \
+  switch (op.s0) {                                                    \
+    case 0: FN<R, T, EB, 0>(__VA_ARGS__); break;                      \
+    case 1: if constexpr (EB > 1) FN<R, T, EB, 1>(__VA_ARGS__); break; \
+    case 2: if constexpr (EB > 2) FN<R, T, EB, 2>(__VA_ARGS__); break; \
+    case 3: if constexpr (EB > 3) FN<R, T, EB, 3>(__VA_ARGS__); break; \
+    case 4: if constexpr (EB > 4) FN<R, T, EB, 4>(__VA_ARGS__); break; \
+    default: break;                                                   \
+  }
+
+template <typename R, typename T, int EB, int J1>
+__device__ __forceinline__ void reg_pair_op(T (&v)[1 << EB], const RegOp<R>& op, bool on, bool is_u2) {
+#define NQPU_J2(J2)                                                           \
+  case J2:                                                                    \
+    if constexpr (J2 < EB && J2 != J1) {                                      \
+      if (is_u2) reg_u2<R, T, EB, J1, J2>(v, op, on);                         \
+      else reg_swap<R, T, EB, J1, J2>(v, op, on);                             \
+    }                                                                         \
+    break;
+  switch (op.s1) {
+    NQPU_J2(0) NQPU_J2(1) NQPU_J2(2) NQPU_J2(3) NQPU_J2(4)
+    default: break;
+  }
+#undef NQPU_J2
+}
+
+template <typename R, typename T, int EB>
+__global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArgs pa, uint64_t ntiles,
+                                                  const RegOp<R>* __restrict__ prog) {
+  constexpr int E = 1 << EB, NT = 256;
+  extern __shared__ unsigned char smraw[];
+  T* sm = reinterpret_cast<T*>(smraw);
+  const int tid = threadIdx.x;
+  uint64_t dlo = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) dlo |= (uint64_t)((tid >> j) & 1) << pa.q[j];
+  const RegOp<R>* ops = prog + pa.op0;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint64_t base = tile;
+#pragma unroll
+    for (int j = 0; j < 8 + EB; ++j) base = ins0(base, pa.q[j]);
+    T v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      uint64_t d = base | dlo;
+#pragma unroll
+      for (int j = 0; j < EB; ++j)
+        if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
+      v[r] = st[d];
+    }
+    for (int o = 0; o < pa.nops; ++o) {
+      const RegOp<R>& op = ops[o];
+      const bool on = (tid & op.ct) == op.ct;
+      switch (op.kind) {
+        case R_REMAP: reg_remap<R, T, EB>(v, sm, op, tid); break;
+        case R_U1: NQPU_SLOT_CASES(reg_u1, v, op, on) break;
+        case R_FLIP: NQPU_SLOT_CASES(reg_flip, v, op, on) break;
+        case R_D1:
+        case R_D2: reg_diag<R, T, EB>(v, op, tid, on); break;
+        case R_U2: NQPU_SLOT_CASES(reg_pair_op, v, op, on, true) break;
+        case R_SWAP: NQPU_SLOT_CASES(reg_pair_op, v, op, on, false) break;
+        default: break;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      uint64_t d = base | dlo;
+#pragma unroll
+      for (int j = 0; j < EB; ++j)
+        if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
+      st[d] = v[r];
+    }
   }
 }
 
@@ -772,6 +993,30 @@ void Launch<R>::tile_pass(void* st, const PassArgs& pa, const void* dops, hipStr
   const size_t lds = sizeof(T_) << pa.K;
   hipLaunchKernelGGL((k_tile_small<R, T_, kBlock>), dim3((unsigned)g), dim3(kBlock), lds, s, (T_*)st, pa, ntiles,
                      (const DevOp<R>*)dops);
+}
+
+template <typename R, int EB>
+static void launch_tile_reg(void* st, const PassArgs& pa, const void* prog, hipStream_t s) {
+  const uint64_t ntiles = 1ull << (pa.n - pa.K);
+  const uint64_t g = ntiles < (1ull << 20) ? ntiles : (1ull << 20);
+  const size_t lds = sizeof(T_) << pa.K;
+  auto fn = k_tile_reg<R, T_, EB>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(fn, dim3((unsigned)g), dim3(kBlock), lds, s, (T_*)st, pa, ntiles, (const RegOp<R>*)prog);
+}
+
+template <typename R>
+void Launch<R>::tile_pass_reg(void* st, const PassArgs& pa, int eb, const void* prog, hipStream_t s) {
+  switch (eb) {
+    case 3: launch_tile_reg<R, 3>(st, pa, prog, s); return;
+    case 4: launch_tile_reg<R, 4>(st, pa, prog, s); return;
+    case 5: launch_tile_reg<R, 5>(st, pa, prog, s); return;
+    default: return;
+  }
 }
 
 template <typename R>

@@ -52,6 +52,7 @@ struct Pass {
 struct PlanOptions {
   int tile_qubits = 12;
   int low_qubits = 6;
+  bool registers = true;  // register-resident tile kernel where it applies
 };
 
 // lowering and planning (pure host code; unit-testable without a device)
@@ -63,6 +64,9 @@ FusionResult fuse_gates_reference(const std::vector<Gate>& gates);
 PlanOptions plan_options(const NqpuRunOptions& o, int prec);
 // What run_circuit would launch, computed on the host.
 NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec);
+// Host emulation of the tiled execution (plan, per-pass programs, kernel
+// semantics) on a complex128 vector.
+NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std::vector<cd>& a);
 
 struct DeviceScratch {
   void* p = nullptr;

@@ -115,12 +115,23 @@ class NativeCircuit:
             ops[i] = _op(g)
         L.check(lib.nqpu_circuit_add_gates(self.ptr, ops, len(circuit.gates)), "nqpu_circuit_add_gates")
 
-    def plan(self, precision="c64", fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0):
+    def plan(self, precision="c64", fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
+             lds_tiles=False):
         lib = L.load()
         st = L.NqpuRunStats()
-        o = run_options(fusion, semantics, tile_qubits, low_qubits)
+        o = run_options(fusion, semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles)
         L.check(lib.nqpu_circuit_plan(self.ptr, ct.byref(o), _prec(precision), ct.byref(st)), "plan")
         return _stats(st)
+
+    def emulate(self, amps, precision="c64", semantics="standard", tile_qubits=0, low_qubits=0,
+                lds_tiles=False):
+        """Host emulation of the tiled GPU execution (plan, per-pass programs
+        and kernel semantics) in FP64; returns the new amplitudes."""
+        a = np.ascontiguousarray(amps, dtype=np.complex128).copy()
+        o = run_options("tiled", semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles)
+        L.check(L.load().nqpu_debug_emulate(self.ptr, ct.byref(o), _prec(precision), a.ctypes.data, len(a)),
+                "emulate")
+        return a
 
     def fusion_counts(self):
         lib = L.load()
@@ -146,13 +157,13 @@ _SEMANTICS = {"standard": L.SEMANTICS_STANDARD, "reference": L.SEMANTICS_REFEREN
 
 
 def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
-                time_passes=False) -> L.NqpuRunOptions:
+                time_passes=False, lds_tiles=False) -> L.NqpuRunOptions:
     o = L.NqpuRunOptions()
     o.fusion = _FUSION[fusion]
     o.semantics = _SEMANTICS[semantics]
     o.tile_qubits = tile_qubits
     o.low_qubits = low_qubits
-    o.flags = L.RUN_TIME_PASSES if time_passes else 0
+    o.flags = (L.RUN_TIME_PASSES if time_passes else 0) | (L.RUN_LDS_TILES if lds_tiles else 0)
     return o
 
 
@@ -230,10 +241,10 @@ class StateVector:
 
     # --- execution
     def run(self, circuit: Circuit | NativeCircuit, fusion="tiled", semantics="standard",
-            tile_qubits=0, low_qubits=0, time_passes=False) -> dict:
+            tile_qubits=0, low_qubits=0, time_passes=False, lds_tiles=False) -> dict:
         lib = L.load()
         nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
-        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes)
+        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles)
         L.check(lib.nqpu_run(nc.ptr, self.ptr, ct.byref(o)), "nqpu_run")
         return self.last_stats()
 

@@ -86,6 +86,21 @@ def test_tile_shapes(oracle, tile, low):
     assert np.max(np.abs(out - exp)) < 1e-12
 
 
+@pytest.mark.parametrize("tile,low", [(11, 3), (12, 5), (13, 6)])
+@pytest.mark.parametrize("lds", [False, True])
+def test_complex64_tile_kernels(oracle, tile, low, lds):
+    """Register-resident (default) and LDS tile kernels in complex64."""
+    from tests.test_planner_emulation import random_circuit
+    c = random_circuit(15, 300, 11)
+    c.extend(qft(15).gates)
+    exp = oracle.run(15, c.gates)
+    with StateVector(15, "c64") as sv:
+        sv.run(c, tile_qubits=tile, low_qubits=low, lds_tiles=lds)
+        out = sv.amplitudes()
+    assert np.max(np.abs(out - exp)) < 2e-5
+    assert infidelity(out, exp) < 1e-6
+
+
 @pytest.fixture(scope="module")
 def c1(oracle):
     """20 qubits, 20 U3 + CNOT-brickwork layers, seed 101; the oracle state."""

@@ -1,0 +1,55 @@
+"""The tiled execution plan on the CPU: nqpu_debug_emulate interprets the very
+plan and per-pass register programs nqpu_run launches (tile partition, slot
+remaps, relabelled SWAPs, control/diagonal handling) in FP64, and the result
+must equal the oracle's gate-by-gate state to 1e-12.  No GPU needed."""
+import numpy as np
+import pytest
+
+from nqpu_amd.engine import Circuit, Gate
+from nqpu_amd.workloads import qft, random_u3_brickwork, tfim_trotter
+from tests.golden import make_golden as mg
+
+
+def random_circuit(n, count, seed):
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    for _ in range(count):
+        k, t, cq, p, cust = mg.random_gate(rng, n)
+        c.gates.append(Gate(k, tuple(t), tuple(cq), tuple(p), cust))
+    return c
+
+
+def random_state(n, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return a / np.linalg.norm(a)
+
+
+CIRCUITS = {
+    "mixed13": lambda: random_circuit(13, 300, 1),
+    "mixed14": lambda: random_circuit(14, 200, 2),
+    "brick14": lambda: random_u3_brickwork(14, 12, 3),
+    "qft_trotter13": lambda: tfim_trotter(13, 3, circuit=qft(13)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CIRCUITS))
+@pytest.mark.parametrize("tile,low", [(11, 3), (12, 5), (13, 6), (12, 0)])
+@pytest.mark.parametrize("lds", [False, True])
+def test_emulated_tiles_match_oracle(oracle, name, tile, low, lds):
+    c = CIRCUITS[name]()
+    n = c.num_qubits
+    a0 = random_state(n, 7)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    got = c.native().emulate(a0, "c64", tile_qubits=tile, low_qubits=low, lds_tiles=lds)
+    assert np.max(np.abs(got - exp)) < 1e-12
+
+
+def test_emulated_reference_semantics(oracle):
+    c = random_circuit(12, 250, 5)
+    a0 = random_state(12, 8)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, 12, c.gates, quirk=True)
+    got = c.native().emulate(a0, "c64", semantics="reference", tile_qubits=12)
+    assert np.max(np.abs(got - exp)) < 1e-12
