@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the complex128 accuracy run")
     ap.add_argument("--no-per-gate", action="store_true")
+    ap.add_argument("--lds", action="store_true", help="LDS-resident tile kernel instead of the register one")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one plain state copy (k_copy) of known size, for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -80,10 +83,15 @@ def main():
     circ = random_u3_brickwork(n, depth, 102)
     nc = circ.native()
     gates = len(circ.gates)
-    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low)
+    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds)
     log(f"workload n={n} depth={depth} gates={gates} plan={nc.plan('c64', **run_kw)}")
 
     sv = E.StateVector(n, "c64")
+    if args.copy_probe:
+        tmp = E.StateVector(n, "c64")
+        tmp.copy_from(sv)
+        tmp.close()
+        log(f"copy probe: {2 * (8 << n)} bytes moved by k_copy")
     for w in range(args.warmup):
         sv.run(nc, **run_kw)
         log(f"warmup {w + 1}/{args.warmup} done")
@@ -124,7 +132,7 @@ def main():
         "hbm_frac_per_step": hbm_frac_step,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_tile", "avg_launch_ms": avg_pass_ms,
+                     "kernel": "k_tile" if args.lds else "k_tile_reg", "avg_launch_ms": avg_pass_ms,
                      "bytes_per_launch": bytes_per_pass},
     }
     zz = None
@@ -135,7 +143,7 @@ def main():
         sv.reset()
         sv.run(nc, **run_kw)
         ov = sv.overlap(ref)
-        result["infidelity_vs_c128"] = 1.0 - abs(ov) ** 2
+        result["infidelity_vs_c128"] = 1.0 - abs(ov) ** 2 / (sv.norm() * ref.norm())
         errs = []
         for i in range(n - 1):
             s = "I" * i + "ZZ" + "I" * (n - i - 2)

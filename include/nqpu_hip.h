@@ -220,6 +220,7 @@ typedef struct {
   double bytes_moved;       /* algorithmic HBM bytes of those sweeps */
   double pass_ms;           /* with NQPU_RUN_TIME_PASSES: summed kernel time of the passes */
   uint64_t timed_passes;    /* passes that pass_ms covers */
+  uint64_t remaps;          /* LDS exchanges inside register-resident tile passes */
 } NqpuRunStats;
 
 /* 1 <= n <= 34 (c64) or 33 (c128), bounded by free device memory. */
@@ -247,6 +248,9 @@ NqpuError nqpu_expectation_pauli_sum(const NqpuState* state, const uint64_t* xma
                                      uint64_t count, double* out);
 /* Scale the state to unit norm (FP64 norm). */
 NqpuError nqpu_state_normalize(NqpuState* state);
+/* Copy src's amplitudes into dst (same qubit count and precision), one
+ * device-side pass. */
+NqpuError nqpu_state_copy(NqpuState* dst, const NqpuState* src);
 /* <a|b> = sum conj(a_i) b_i in FP64 over two states of equal qubit count
  * (any precisions); |<a|b>|^2 is QuantumState::fidelity (lib.rs:388-403). */
 NqpuError nqpu_state_overlap(const NqpuState* a, const NqpuState* b, NqpuComplex* out);

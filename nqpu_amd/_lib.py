@@ -57,7 +57,8 @@ RUN_LDS_TILES = 2
 
 class NqpuRunStats(ct.Structure):
     _fields_ = [("gates", ct.c_uint64), ("device_ops", ct.c_uint64), ("passes", ct.c_uint64),
-                ("bytes_moved", ct.c_double), ("pass_ms", ct.c_double), ("timed_passes", ct.c_uint64)]
+                ("bytes_moved", ct.c_double), ("pass_ms", ct.c_double), ("timed_passes", ct.c_uint64),
+                ("remaps", ct.c_uint64)]
 
 
 class NqpuFusionCounts(ct.Structure):
@@ -120,6 +121,7 @@ SIGNATURES = {
     "nqpu_expectation_pauli_sum": (_E, [_P, _P, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),
     "nqpu_state_normalize": (_E, [_P]),
     "nqpu_state_overlap": (_E, [_P, _P, ct.POINTER(NqpuComplex)]),
+    "nqpu_state_copy": (_E, [_P, _P]),
     "nqpu_state_get_amplitudes": (_E, [_P, _P, ct.c_uint64]),
     "nqpu_state_set_amplitudes": (_E, [_P, _P, ct.c_uint64]),
     "nqpu_last_error_detail": (ct.c_char_p, []),

@@ -201,6 +201,19 @@ NqpuError nqpu_state_normalize(NqpuState* state) {
   return scale(S(state), 1.0 / std::sqrt(v));
 }
 
+NqpuError nqpu_state_copy(NqpuState* dst, const NqpuState* src) {
+  if (!dst || !src) return NQPU_NULL_POINTER;
+  State* d = S(dst);
+  const State* s = S(src);
+  if (d->n != s->n || d->prec != s->prec) return NQPU_INVALID_PARAMETER;
+  NqpuError e = hip_check(hipStreamSynchronize(s->stream), "sync");
+  if (e != NQPU_OK) return e;
+  launch_copy(d->amps, s->amps, d->dim(), d->prec == NQPU_PRECISION_C64, d->stream);
+  e = hip_check(hipGetLastError(), "copy launch");
+  if (e != NQPU_OK) return e;
+  return hip_check(hipStreamSynchronize(d->stream), "copy");
+}
+
 NqpuError nqpu_state_overlap(const NqpuState* a, const NqpuState* b, NqpuComplex* out) {
   if (!a || !b || !out) return NQPU_NULL_POINTER;
   if (S(a)->n != S(b)->n) return NQPU_INVALID_PARAMETER;

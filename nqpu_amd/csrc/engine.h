@@ -118,6 +118,8 @@ struct Launch {
 // Reduce per-chunk double partials [nchunks][width] -> [width] in a fixed
 // order (deterministic for a given n).
 void launch_reduce_rows(const double* in, uint64_t rows, int width, double* out, hipStream_t s);
+// dst[i] = src[i] over 2^n amplitudes (a plain streaming copy).
+void launch_copy(void* dst, const void* src, uint64_t dim, bool c64, hipStream_t s);
 // Per-chunk [Re, Im] of sum conj(a_i) b_i; a64/b64 select complex64 storage.
 void launch_overlap(const void* a, bool a64, const void* b, bool b64, int n, int cb, double* out,
                     hipStream_t s);

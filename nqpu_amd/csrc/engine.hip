@@ -1014,6 +1014,7 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
     std::vector<RegOp<R>> pr = build_reg_program<R>(p, eb);
     pa.nops = (int)pr.size();
+    for (const RegOp<R>& r : pr) s->stats.remaps += r.kind == R_REMAP;
     prog.insert(prog.end(), pr.begin(), pr.end());
     args.push_back(pa);
   }
@@ -1129,10 +1130,15 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
     }
   } else {
     std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
-    std::vector<Pass> passes = plan_tiles(ops, c.n, plan_options(o, prec));
+    PlanOptions po = plan_options(o, prec);
+    std::vector<Pass> passes = plan_tiles(ops, c.n, po);
     st.passes = passes.size();
     st.device_ops = ops.size();
     st.bytes_moved = (double)passes.size() * 2.0 * (double)(asz << c.n);
+    const int K = std::min(po.tile_qubits, c.n);
+    if (po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13)
+      for (const Pass& p : passes)
+        for (const RegOp<double>& r : build_reg_program<double>(p, K - 8)) st.remaps += r.kind == R_REMAP;
   }
   return st;
 }

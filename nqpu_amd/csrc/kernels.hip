@@ -228,6 +228,21 @@ __global__ void k_diagk(T* st, uint64_t dim, QList ql, const double2* lut) {
   }
 }
 
+template <typename T>
+__global__ void k_copy(T* __restrict__ dst, const T* __restrict__ src, uint64_t dim) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy(void* dst, const void* src, uint64_t dim, bool c64, hipStream_t s) {
+  if (c64)
+    hipLaunchKernelGGL(k_copy<float2>, dim3(grid_for(dim)), dim3(kBlock), 0, s, (float2*)dst, (const float2*)src, dim);
+  else
+    hipLaunchKernelGGL(k_copy<double2>, dim3(grid_for(dim)), dim3(kBlock), 0, s, (double2*)dst, (const double2*)src,
+                       dim);
+}
+
 template <typename R, typename T>
 __global__ void k_scale(T* st, uint64_t dim, double f) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
@@ -413,6 +428,17 @@ __device__ __forceinline__ void reg_u1(T (&v)[1 << EB], const RegOp<R>& op, bool
   const R m0r = op.m[0], m0i = op.m[1], m1r = op.m[2], m1i = op.m[3];
   const R m2r = op.m[4], m2i = op.m[5], m3r = op.m[6], m3i = op.m[7];
   const uint32_t cr = op.cr;
+  if (op.ct == 0 && cr == 0) {  // uncontrolled: straight-line math
+#pragma unroll
+    for (int r0 = 0; r0 < (1 << EB); ++r0) {
+      if (r0 & (1 << J)) continue;
+      const int r1 = r0 | (1 << J);
+      T x = v[r0], y = v[r1];
+      v[r0] = mac2<R>(m0r, m0i, m1r, m1i, x, y);
+      v[r1] = mac2<R>(m2r, m2i, m3r, m3i, x, y);
+    }
+    return;
+  }
 #pragma unroll
   for (int r0 = 0; r0 < (1 << EB); ++r0) {
     if (r0 & (1 << J)) continue;
@@ -429,6 +455,18 @@ __device__ __forceinline__ void reg_u1(T (&v)[1 << EB], const RegOp<R>& op, bool
 template <typename R, typename T, int EB, int J>
 __device__ __forceinline__ void reg_flip(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
   const uint32_t cr = op.cr;
+  if (op.ct == 0) {  // controls only on register slots: a uniform permutation
+#pragma unroll
+    for (int r0 = 0; r0 < (1 << EB); ++r0) {
+      if (r0 & (1 << J)) continue;
+      if ((r0 & cr) != cr) continue;
+      const int r1 = r0 | (1 << J);
+      T x = v[r0];
+      v[r0] = v[r1];
+      v[r1] = x;
+    }
+    return;
+  }
 #pragma unroll
   for (int r0 = 0; r0 < (1 << EB); ++r0) {
     if (r0 & (1 << J)) continue;
@@ -492,8 +530,23 @@ __device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, in
   const int a = op.s0, b = op.s1;
   const bool areg = a >= 16, breg = b >= 16;
   const int as = a & 15, bs = b & 15;
-  const bool ta = (tid >> as) & 1, tb = two && ((tid >> bs) & 1);
   const uint32_t cr = op.cr;
+  if (op.ct == 0 && areg && (!two || breg)) {
+    // every factor depends on register bits only: the phase of each element
+    // is chosen per register index, uniformly across the wavefront
+#pragma unroll
+    for (int r = 0; r < (1 << EB); ++r) {
+      if ((r & cr) != cr) continue;
+      const bool ba = (r >> as) & 1;
+      const bool bb = two && ((r >> bs) & 1);
+      T ph;
+      ph.x = bb ? (ba ? p3.x : p2.x) : (ba ? p1.x : p0.x);
+      ph.y = bb ? (ba ? p3.y : p2.y) : (ba ? p1.y : p0.y);
+      v[r] = cmul_(ph, v[r]);
+    }
+    return;
+  }
+  const bool ta = (tid >> as) & 1, tb = two && ((tid >> bs) & 1);
 #pragma unroll
   for (int r = 0; r < (1 << EB); ++r) {
     if ((r & cr) != cr) continue;

@@ -169,7 +169,7 @@ def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=
 
 def _stats(st: L.NqpuRunStats) -> dict:
     return {"gates": st.gates, "device_ops": st.device_ops, "passes": st.passes, "bytes": st.bytes_moved,
-            "pass_ms": st.pass_ms, "timed_passes": st.timed_passes}
+            "pass_ms": st.pass_ms, "timed_passes": st.timed_passes, "remaps": st.remaps}
 
 
 def _prec(p) -> int:
@@ -276,6 +276,10 @@ class StateVector:
 
     def normalize(self):
         L.check(L.load().nqpu_state_normalize(self.ptr), "normalize")
+
+    def copy_from(self, other: "StateVector") -> None:
+        """Device-side copy of another state of the same size and precision."""
+        L.check(L.load().nqpu_state_copy(self.ptr, other.ptr), "copy")
 
     def overlap(self, other: "StateVector") -> complex:
         """<self|other> in FP64 on the device."""
