@@ -29,6 +29,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(args):
+    """HBM bytes per launch of the dominant kernel, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/traffic.json (scaled by
+    the plain-copy calibration); None when no measurement matches this
+    kernel and workload shape."""
+    if args.lds or args.tile not in (0, 12) or args.low not in (0, 5) or args.qubits != 30 or args.depth != 40:
+        return None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)["k_tile_reg"]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(seconds: float, n: int = 26):
     """FP64 oracle (C + OpenMP, all host threads), gate by gate like the
     reference's GateOperations path, on the same circuit family at n qubits
@@ -131,7 +145,7 @@ def main():
                    "parallelism": "single GPU"},
         "hbm_frac_per_step": hbm_frac_step,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
                      "kernel": "k_tile" if args.lds else "k_tile_reg", "avg_launch_ms": avg_pass_ms,
                      "bytes_per_launch": bytes_per_pass},
     }

@@ -211,6 +211,8 @@ typedef struct {
 /* run tiled passes with the LDS-resident tile kernel (one LDS sweep per gate)
  * instead of the register-resident one */
 #define NQPU_RUN_LDS_TILES 2u
+/* nqpu_debug_emulate only: emulate the layout the batched kernel uses */
+#define NQPU_RUN_BATCH_LAYOUT 4u
 
 /* Statistics of the last run on a state. */
 typedef struct {
@@ -272,6 +274,42 @@ typedef struct {
   uint64_t items;
 } NqpuFusionCounts;
 NqpuError nqpu_circuit_fusion_counts(const NqpuCircuit* circuit, NqpuFusionCounts* out);
+
+/* ------------------------------------------------------------ batching */
+
+/* One weighted Pauli string: X on xmask&~zmask, Y on xmask&zmask, Z on
+ * zmask&~xmask. */
+typedef struct {
+  uint64_t xmask;
+  uint64_t zmask;
+  double coeff;
+} NqpuPauliTerm;
+
+typedef struct NqpuBatch NqpuBatch;
+
+typedef struct {
+  uint64_t circuits;
+  uint64_t resident;        /* circuits held entirely in registers/LDS */
+  uint64_t gate_passes;     /* tile passes over HBM-resident states */
+  uint64_t observable_passes;
+  double plan_ms;           /* host planning and upload */
+  double kernel_ms;         /* the persistent launch */
+  double state_bytes;       /* HBM reserved for HBM-resident states */
+} NqpuBatchStats;
+
+/* A reusable batch context (stream and device buffers).  Not thread-safe. */
+NqpuBatch* nqpu_batch_create(void);
+void nqpu_batch_destroy(NqpuBatch* batch);
+/* Run each circuit k from |0...0> as a complex64 state and write
+ * sum_t coeff_t <P_t> (FP64) for its terms terms[offsets[k] .. offsets[k+1])
+ * to out[k].  All circuits run in one persistent launch; every result is
+ * independent of which other circuits share the batch.  Circuits need at
+ * least 4 qubits; the jax_bridge vmap_simulate map (serial) is what this
+ * batches (sdk/rust/src/quantum_ml/bridges/jax_bridge.rs:447). */
+NqpuError nqpu_batch_expectation(NqpuBatch* batch, const NqpuCircuit* const* circuits, uint64_t count,
+                                 const NqpuPauliTerm* terms, const uint64_t* offsets,
+                                 const NqpuRunOptions* opts, double* out);
+NqpuError nqpu_batch_last_stats(const NqpuBatch* batch, NqpuBatchStats* out);
 
 /* Host-only emulation of nqpu_run's tiled execution for a state of the given
  * precision: the same plan and per-pass programs, interpreted on the CPU in

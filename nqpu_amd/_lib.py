@@ -61,6 +61,19 @@ class NqpuRunStats(ct.Structure):
                 ("remaps", ct.c_uint64)]
 
 
+class NqpuPauliTerm(ct.Structure):
+    _fields_ = [("xmask", ct.c_uint64), ("zmask", ct.c_uint64), ("coeff", ct.c_double)]
+
+
+class NqpuBatchStats(ct.Structure):
+    _fields_ = [("circuits", ct.c_uint64), ("resident", ct.c_uint64), ("gate_passes", ct.c_uint64),
+                ("observable_passes", ct.c_uint64), ("plan_ms", ct.c_double), ("kernel_ms", ct.c_double),
+                ("state_bytes", ct.c_double)]
+
+
+RUN_BATCH_LAYOUT = 4
+
+
 class NqpuFusionCounts(ct.Structure):
     _fields_ = [("original_count", ct.c_uint64), ("fused_count", ct.c_uint64),
                 ("gates_eliminated", ct.c_uint64), ("items", ct.c_uint64)]
@@ -129,6 +142,10 @@ SIGNATURES = {
     "nqpu_circuit_plan": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, ct.POINTER(NqpuRunStats)]),
     "nqpu_circuit_fusion_counts": (_E, [_P, ct.POINTER(NqpuFusionCounts)]),
     "nqpu_debug_emulate": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, _P, ct.c_uint64]),
+    "nqpu_batch_create": (_P, []),
+    "nqpu_batch_destroy": (None, [_P]),
+    "nqpu_batch_expectation": (_E, [_P, _P, ct.c_uint64, _P, _P, ct.POINTER(NqpuRunOptions), _P]),
+    "nqpu_batch_last_stats": (_E, [_P, ct.POINTER(NqpuBatchStats)]),
 }
 
 _lib = None

@@ -500,6 +500,42 @@ void nqpu_set_num_threads(uint32_t) {}
 
 const char* nqpu_last_error_detail(void) { return last_error(); }
 
+NqpuBatch* nqpu_batch_create(void) {
+  Batch* b = nullptr;
+  if (create_batch(&b) != NQPU_OK) return nullptr;
+  return reinterpret_cast<NqpuBatch*>(b);
+}
+
+void nqpu_batch_destroy(NqpuBatch* b) { delete reinterpret_cast<Batch*>(b); }
+
+NqpuError nqpu_batch_expectation(NqpuBatch* b, const NqpuCircuit* const* circuits, uint64_t count,
+                                 const NqpuPauliTerm* terms, const uint64_t* offsets, const NqpuRunOptions* opts,
+                                 double* out) {
+  if (!b || !out || (count && (!circuits || !offsets))) return NQPU_NULL_POINTER;
+  NqpuRunOptions o{NQPU_FUSION_TILED, NQPU_SEMANTICS_STANDARD, 0, 0, 0};
+  if (opts) o = *opts;
+  if (o.semantics > NQPU_SEMANTICS_REFERENCE) return NQPU_INVALID_PARAMETER;
+  std::vector<const Circuit*> cs(count);
+  std::vector<std::vector<PauliTerm>> ts(count);
+  for (uint64_t k = 0; k < count; ++k) {
+    if (!circuits[k]) return NQPU_NULL_POINTER;
+    cs[k] = Cc(circuits[k]);
+    if (offsets[k + 1] < offsets[k]) return NQPU_INVALID_PARAMETER;
+    if (offsets[k + 1] > offsets[k] && !terms) return NQPU_NULL_POINTER;
+    for (uint64_t t = offsets[k]; t < offsets[k + 1]; ++t) {
+      if (!std::isfinite(terms[t].coeff)) return NQPU_INVALID_PARAMETER;
+      ts[k].push_back(PauliTerm{terms[t].xmask, terms[t].zmask, terms[t].coeff});
+    }
+  }
+  return batch_expectation(reinterpret_cast<Batch*>(b), cs, ts, o, out);
+}
+
+NqpuError nqpu_batch_last_stats(const NqpuBatch* b, NqpuBatchStats* out) {
+  if (!b || !out) return NQPU_NULL_POINTER;
+  *out = reinterpret_cast<const Batch*>(b)->stats;
+  return NQPU_OK;
+}
+
 uint32_t nqpu_device_count(void) { return (uint32_t)device_count(); }
 
 }  // extern "C"

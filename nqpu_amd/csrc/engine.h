@@ -73,6 +73,43 @@ struct RegOp {
 
 constexpr int kMaxTile = 14;  // maximum tile qubits (LDS: 2^13 * 16 B = 128 KiB)
 
+// ------------------------------------------------------ batched execution
+// k_batch: one persistent launch over many complex64 circuits.  Each
+// workgroup takes whole circuits from an atomic queue; a circuit of at most
+// 8 + kBatchEB qubits lives in registers/LDS for its whole life, a larger
+// one streams register-tile passes over its own HBM region.  Observables
+// (Pauli sums) are evaluated on LDS tiles in FP64.
+constexpr int kBatchEB = 4;
+
+struct BatchCircuit {
+  int32_t n;
+  int32_t resident;      // 1: the whole state is one tile
+  int32_t pass0, npass;  // gate passes (BatchPass table)
+  int32_t red0, nred;    // observable passes (BatchPass table; resident: the gate pass)
+  int32_t out;           // output slot
+  int32_t pad;
+  uint64_t state_off;    // amplitude offset of the HBM region (not resident)
+};
+
+struct BatchPass {
+  int32_t K, tb;         // tile qubits, thread bits (K = tb + kBatchEB for gate passes)
+  int32_t op0, nops;     // register program
+  int32_t term0, nterm;  // observable terms evaluated on this tile set
+  int32_t q[16];         // tile qubits, ascending
+};
+
+struct BatchTerm {
+  uint32_t xl, zl;       // X/Z bits inside the tile (tile-local numbering)
+  uint64_t zg;           // Z bits outside the tile (global numbering)
+  double coeff;
+  int32_t ny;            // number of Y factors mod 4
+  int32_t pad;
+};
+
+void launch_batch(void* states, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
+                  const RegOp<float>* prog, const BatchTerm* terms, unsigned* counter, double* out,
+                  int grid, hipStream_t s);
+
 // One tile pass: tile qubits (global positions, ascending), the ops it
 // applies, and where they live in the uploaded op buffer.
 struct PassArgs {

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <map>
 
 namespace nqpu {
@@ -728,8 +729,8 @@ static std::vector<HostOp> diagonal_ops(const FusedItem& it) {
 // register slots.  The pass ends in the canonical slot assignment the store
 // expects.
 template <typename R>
-std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb) {
-  const int K = (int)p.qubits.size();
+std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
+  const int K = (int)p.qubits.size();  // == tb + eb
   int loc[64];
   for (int q = 0; q < 64; ++q) loc[q] = -1;
   for (int j = 0; j < K; ++j) loc[p.qubits[j]] = j;
@@ -758,10 +759,14 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb) {
       default: return 0u;
     }
   };
+  // canonical assignment: thread slot i holds tile bit i (i < tb), register
+  // slot j holds tile bit tb + j; thread slots >= tb do not exist
   uint8_t map[16], canon[16];
-  for (int i = 0; i < 16; ++i) map[i] = canon[i] = (uint8_t)(i < 8 + eb ? i : 0);
+  for (int i = 0; i < 16; ++i) map[i] = canon[i] = 0;
+  for (int i = 0; i < tb; ++i) map[i] = canon[i] = (uint8_t)i;
+  for (int j = 0; j < eb; ++j) map[8 + j] = canon[8 + j] = (uint8_t)(tb + j);
   auto slot = [&](int lb) {
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < tb; ++i)
       if (map[i] == lb) return i;
     for (int j = 0; j < eb; ++j)
       if (map[8 + j] == lb) return 16 + j;
@@ -853,7 +858,7 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb) {
       else free_reg.push_back(j);
     }
     size_t fi = 0;
-    for (int i = 0; i < 8 && need; ++i) {
+    for (int i = 0; i < tb && need; ++i) {
       const int lb = map[i];
       if (!((need >> lb) & 1)) continue;
       // thread slot i holds a needed bit: swap it with a register slot
@@ -871,8 +876,8 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb) {
 // Host emulation of one k_tile_reg pass over a complex128 vector, following
 // the kernel's thread / register / LDS semantics step by step.
 static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std::vector<RegOp<double>>& prog,
-                             int eb) {
-  const int K = 8 + eb, E = 1 << eb;
+                             int eb, int tb = 8) {
+  const int K = tb + eb, E = 1 << eb, NTH = 1 << tb;
   const uint64_t ntiles = 1ull << (n - K);
   std::vector<cd> v(256 * E), lds(size_t(1) << K);
   auto dep = [&](uint32_t t) {
@@ -893,19 +898,19 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std
       const int q = p.qubits[j];
       base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1));
     }
-    for (int tid = 0; tid < 256; ++tid)
-      for (int r = 0; r < E; ++r) v[tid * E + r] = a[base | dep((uint32_t)tid | ((uint32_t)r << 8))];
+    for (int tid = 0; tid < NTH; ++tid)
+      for (int r = 0; r < E; ++r) v[tid * E + r] = a[base | dep((uint32_t)tid | ((uint32_t)r << tb))];
     for (const RegOp<double>& op : prog) {
       if (op.kind == R_REMAP) {
-        for (int tid = 0; tid < 256; ++tid)
+        for (int tid = 0; tid < NTH; ++tid)
           for (int r = 0; r < E; ++r) lds[tmap(op.map_old, tid, r)] = v[tid * E + r];
-        for (int tid = 0; tid < 256; ++tid)
+        for (int tid = 0; tid < NTH; ++tid)
           for (int r = 0; r < E; ++r) v[tid * E + r] = lds[tmap(op.map_new, tid, r)];
         continue;
       }
       cd m[16];
       for (int i = 0; i < 16; ++i) m[i] = cd(op.m[2 * i], op.m[2 * i + 1]);
-      for (int tid = 0; tid < 256; ++tid) {
+      for (int tid = 0; tid < NTH; ++tid) {
         if ((tid & op.ct) != op.ct) continue;
         cd* w = &v[tid * E];
         for (int r = 0; r < E; ++r) {
@@ -948,8 +953,8 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std
         }
       }
     }
-    for (int tid = 0; tid < 256; ++tid)
-      for (int r = 0; r < E; ++r) a[base | dep((uint32_t)tid | ((uint32_t)r << 8))] = v[tid * E + r];
+    for (int tid = 0; tid < NTH; ++tid)
+      for (int r = 0; r < E; ++r) a[base | dep((uint32_t)tid | ((uint32_t)r << tb))] = v[tid * E + r];
   }
 }
 
@@ -989,13 +994,29 @@ static void emulate_op(std::vector<cd>& a, int n, const HostOp& o) {
 
 NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std::vector<cd>& a) {
   const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
-  PlanOptions po = plan_options(o, prec);
   std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
+  if (o.flags & NQPU_RUN_BATCH_LAYOUT) {  // the layout k_batch gives this circuit
+    constexpr int EB = kBatchEB, KH = 8 + kBatchEB;
+    if (c.n < EB) return NQPU_INVALID_PARAMETER;
+    if (c.n <= KH) {
+      Pass p;
+      for (int q = 0; q < c.n; ++q) p.qubits.push_back(q);
+      p.ops = ops;
+      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, EB, c.n - EB), EB, c.n - EB);
+    } else {
+      PlanOptions bpo;
+      bpo.tile_qubits = KH;
+      bpo.low_qubits = 5;
+      for (const Pass& p : plan_tiles(ops, c.n, bpo)) emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, EB, 8), EB);
+    }
+    return NQPU_OK;
+  }
+  PlanOptions po = plan_options(o, prec);
   std::vector<Pass> passes = plan_tiles(ops, c.n, po);
   const int K = std::min(po.tile_qubits, c.n);
   const bool reg = po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13;
   for (const Pass& p : passes) {
-    if (reg) emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8), K - 8);
+    if (reg) emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8), K - 8);
     else
       for (const HostOp& op : p.ops) emulate_op(a, c.n, op);
   }
@@ -1012,7 +1033,7 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     pa.n = s->n;
     pa.op0 = (int)prog.size();
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
-    std::vector<RegOp<R>> pr = build_reg_program<R>(p, eb);
+    std::vector<RegOp<R>> pr = build_reg_program<R>(p, eb, 8);
     pa.nops = (int)pr.size();
     for (const RegOp<R>& r : pr) s->stats.remaps += r.kind == R_REMAP;
     prog.insert(prog.end(), pr.begin(), pr.end());
@@ -1138,7 +1159,7 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
     const int K = std::min(po.tile_qubits, c.n);
     if (po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13)
       for (const Pass& p : passes)
-        for (const RegOp<double>& r : build_reg_program<double>(p, K - 8)) st.remaps += r.kind == R_REMAP;
+        for (const RegOp<double>& r : build_reg_program<double>(p, K - 8, 8)) st.remaps += r.kind == R_REMAP;
   }
   return st;
 }
@@ -1489,6 +1510,226 @@ NqpuError measure_qubit(State* s, int q, double r, uint32_t* outcome, double* pr
   HIPTRY(hipStreamSynchronize(s->stream), "collapse");
   *outcome = o;
   *prob = po;
+  return NQPU_OK;
+}
+
+// ----------------------------------------------------------------- batches
+
+Batch::~Batch() {
+  if (stream) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+  }
+}
+
+NqpuError create_batch(Batch** out) {
+  *out = nullptr;
+  if (device_count() == 0) {
+    set_error("no usable HIP device (or NQPU_HIP_FORCE_NO_DEVICE=1)");
+    return NQPU_UNAVAILABLE;
+  }
+  std::unique_ptr<Batch> b(new Batch());
+  HIPTRY(hipSetDevice(0), "hipSetDevice");
+  HIPTRY(hipStreamCreate(&b->stream), "hipStreamCreate");
+  *out = b.release();
+  return NQPU_OK;
+}
+
+// A term as the tile sees it: X/Z bits inside the tile in tile-local
+// numbering, Z bits outside it in global numbering.
+static BatchTerm tile_term(const PauliTerm& t, const std::vector<int>& q) {
+  BatchTerm bt{};
+  int loc[64];
+  for (int i = 0; i < 64; ++i) loc[i] = -1;
+  for (size_t j = 0; j < q.size(); ++j) loc[q[j]] = (int)j;
+  for (int b = 0; b < 64; ++b) {
+    if ((t.x >> b) & 1) bt.xl |= 1u << loc[b];
+    if ((t.z >> b) & 1) {
+      if (loc[b] >= 0) bt.zl |= 1u << loc[b];
+      else bt.zg |= bit(b);
+    }
+  }
+  bt.coeff = t.coeff;
+  bt.ny = __builtin_popcountll(t.x & t.z) & 3;
+  return bt;
+}
+
+NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuits,
+                            const std::vector<std::vector<PauliTerm>>& terms, const NqpuRunOptions& o,
+                            double* out) {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
+  const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
+  constexpr int EB = kBatchEB, KH = 8 + kBatchEB;
+  std::vector<BatchCircuit> bc;
+  std::vector<BatchPass> bp;
+  std::vector<RegOp<float>> prog;
+  std::vector<BatchTerm> bt;
+  std::vector<double> cost;
+  uint64_t state_amps = 0;
+  NqpuBatchStats st{};
+  for (size_t k = 0; k < circuits.size(); ++k) {
+    const Circuit& c = *circuits[k];
+    const int n = c.n;
+    if (n < EB) {
+      set_error("batched circuits need at least 4 qubits");
+      return NQPU_INVALID_PARAMETER;
+    }
+    const uint64_t lim = n >= 64 ? ~0ull : (bit(n) - 1);
+    for (const PauliTerm& t : terms[k])
+      if ((t.x | t.z) & ~lim) {
+        set_error("observable term acts on a qubit beyond the circuit");
+        return NQPU_INVALID_QUBIT;
+      }
+    std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
+    BatchCircuit C{};
+    C.n = n;
+    C.out = (int32_t)k;
+    if (n <= KH) {
+      C.resident = 1;
+      Pass p;
+      for (int q = 0; q < n; ++q) p.qubits.push_back(q);
+      p.ops = ops;
+      BatchPass P{};
+      P.K = n;
+      P.tb = n - EB;
+      for (int q = 0; q < n; ++q) P.q[q] = q;
+      std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, n - EB);
+      P.op0 = (int)prog.size();
+      P.nops = (int)pr.size();
+      prog.insert(prog.end(), pr.begin(), pr.end());
+      P.term0 = (int)bt.size();
+      for (const PauliTerm& t : terms[k]) bt.push_back(tile_term(t, p.qubits));
+      P.nterm = (int)terms[k].size();
+      C.pass0 = C.red0 = (int)bp.size();
+      C.npass = 1;
+      bp.push_back(P);
+      cost.push_back((double)(ops.size() + 1) * (double)(1ull << n) / 64.0);
+      st.resident++;
+    } else {
+      C.resident = 0;
+      C.state_off = state_amps;
+      state_amps += 1ull << n;
+      PlanOptions po;
+      po.tile_qubits = KH;
+      po.low_qubits = 5;
+      std::vector<Pass> passes = plan_tiles(ops, n, po);
+      C.pass0 = (int)bp.size();
+      C.npass = (int)passes.size();
+      for (const Pass& p : passes) {
+        BatchPass P{};
+        P.K = KH;
+        P.tb = 8;
+        for (int j = 0; j < KH; ++j) P.q[j] = p.qubits[j];
+        std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, 8);
+        P.op0 = (int)prog.size();
+        P.nops = (int)pr.size();
+        prog.insert(prog.end(), pr.begin(), pr.end());
+        bp.push_back(P);
+      }
+      // observable passes: each term on a tile set that holds its X bits
+      std::vector<std::vector<int>> sets(2);
+      for (int q = 0; q < KH; ++q) sets[0].push_back(q);
+      for (int q = n - KH; q < n; ++q) sets[1].push_back(q);
+      std::vector<std::vector<PauliTerm>> groups(2);
+      auto setmask = [](const std::vector<int>& s) {
+        uint64_t m = 0;
+        for (int q : s) m |= bit(q);
+        return m;
+      };
+      for (const PauliTerm& t : terms[k]) {
+        size_t g = 0;
+        for (; g < sets.size(); ++g)
+          if ((t.x & ~setmask(sets[g])) == 0) break;
+        if (g == sets.size()) {
+          if (__builtin_popcountll(t.x) > KH) {
+            set_error("an observable term has X/Y on more qubits than a tile holds");
+            return NQPU_INVALID_PARAMETER;
+          }
+          uint64_t m = t.x;
+          for (int q = 0; q < n && __builtin_popcountll(m) < KH; ++q) m |= bit(q);
+          std::vector<int> s;
+          for (int q = 0; q < n; ++q)
+            if ((m >> q) & 1) s.push_back(q);
+          sets.push_back(s);
+          groups.emplace_back();
+        }
+        groups[g].push_back(t);
+      }
+      C.red0 = (int)bp.size();
+      for (size_t g = 0; g < sets.size(); ++g) {
+        if (groups[g].empty()) continue;
+        BatchPass P{};
+        P.K = KH;
+        P.tb = 8;
+        for (int j = 0; j < KH; ++j) P.q[j] = sets[g][j];
+        P.term0 = (int)bt.size();
+        for (const PauliTerm& t : groups[g]) bt.push_back(tile_term(t, sets[g]));
+        P.nterm = (int)groups[g].size();
+        bp.push_back(P);
+        C.nred++;
+      }
+      st.gate_passes += C.npass;
+      st.observable_passes += C.nred;
+      cost.push_back((double)(C.npass + C.nred) * (double)(1ull << n));
+    }
+    bc.push_back(C);
+  }
+  // longest circuits first: the queue hands them out before the short ones
+  std::vector<size_t> order(bc.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+  std::vector<BatchCircuit> sorted;
+  sorted.reserve(bc.size());
+  for (size_t i : order) sorted.push_back(bc[i]);
+  // one table upload: [circuits][passes][program][terms][counter]
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t oc = 0, op_ = up(oc + sorted.size() * sizeof(BatchCircuit));
+  const size_t og = up(op_ + bp.size() * sizeof(BatchPass));
+  const size_t ot = up(og + prog.size() * sizeof(RegOp<float>));
+  const size_t ocnt = up(ot + bt.size() * sizeof(BatchTerm));
+  const size_t total = ocnt + 256;
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data() + oc, sorted.data(), sorted.size() * sizeof(BatchCircuit));
+  std::memcpy(host.data() + op_, bp.data(), bp.size() * sizeof(BatchPass));
+  if (!prog.empty()) std::memcpy(host.data() + og, prog.data(), prog.size() * sizeof(RegOp<float>));
+  if (!bt.empty()) std::memcpy(host.data() + ot, bt.data(), bt.size() * sizeof(BatchTerm));
+  HIPTRY(hipStreamSynchronize(b->stream), "sync");
+  if (!b->tables.ensure(total) || !b->out.ensure(circuits.size() * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  if (state_amps && !b->states.ensure(state_amps * 8)) {
+    set_error("batch states do not fit in device memory");
+    return NQPU_OUT_OF_DEVICE_MEMORY;
+  }
+  HIPTRY(hipMemcpy(b->tables.p, host.data(), total, hipMemcpyHostToDevice), "batch table upload");
+  char* base = (char*)b->tables.p;
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+  const int grid = (int)std::min<size_t>(circuits.size(), (size_t)cus * 3);  // 3 workgroups fit per CU
+  const auto t1 = clock::now();
+  hipEvent_t e0, e1;
+  HIPTRY(hipEventCreate(&e0), "event");
+  HIPTRY(hipEventCreate(&e1), "event");
+  (void)hipEventRecord(e0, b->stream);
+  if (grid > 0)
+    launch_batch(b->states.p, (const BatchCircuit*)(base + oc), (int)sorted.size(), (const BatchPass*)(base + op_),
+                 (const RegOp<float>*)(base + og), (const BatchTerm*)(base + ot), (unsigned*)(base + ocnt),
+                 (double*)b->out.p, grid, b->stream);
+  NqpuError le = hip_check(hipGetLastError(), "batch launch");
+  (void)hipEventRecord(e1, b->stream);
+  NqpuError se = hip_check(hipStreamSynchronize(b->stream), "batch");
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (le != NQPU_OK) return le;
+  if (se != NQPU_OK) return se;
+  if (!circuits.empty())
+    HIPTRY(hipMemcpy(out, b->out.p, circuits.size() * 8, hipMemcpyDeviceToHost), "batch results");
+  st.circuits = circuits.size();
+  st.plan_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  st.kernel_ms = ms;
+  st.state_bytes = (double)state_amps * 8.0;
+  b->stats = st;
   return NQPU_OK;
 }
 

@@ -564,20 +564,24 @@ __device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, in
   }
 }
 
+// `active` is false for threads beyond the tile's thread slots (small tiles
+// in batched execution); they only take part in the barriers.
 template <typename R, typename T, int EB>
-__device__ __forceinline__ void reg_remap(T (&v)[1 << EB], T* sm, const RegOp<R>& op, int tid) {
+__device__ __forceinline__ void reg_remap(T (&v)[1 << EB], T* sm, const RegOp<R>& op, int tid, bool active) {
   uint32_t tp = 0, rp[EB];
 #pragma unroll
   for (int i = 0; i < 8; ++i) tp |= (uint32_t)((tid >> i) & 1) << op.map_old[i];
 #pragma unroll
   for (int j = 0; j < EB; ++j) rp[j] = 1u << op.map_old[8 + j];
+  if (active) {
 #pragma unroll
-  for (int r = 0; r < (1 << EB); ++r) {
-    uint32_t t = tp;
+    for (int r = 0; r < (1 << EB); ++r) {
+      uint32_t t = tp;
 #pragma unroll
-    for (int j = 0; j < EB; ++j)
-      if ((r >> j) & 1) t |= rp[j];
-    sm[lds_swz(t)] = v[r];
+      for (int j = 0; j < EB; ++j)
+        if ((r >> j) & 1) t |= rp[j];
+      sm[lds_swz(t)] = v[r];
+    }
   }
   __syncthreads();
   tp = 0;
@@ -585,13 +589,15 @@ __device__ __forceinline__ void reg_remap(T (&v)[1 << EB], T* sm, const RegOp<R>
   for (int i = 0; i < 8; ++i) tp |= (uint32_t)((tid >> i) & 1) << op.map_new[i];
 #pragma unroll
   for (int j = 0; j < EB; ++j) rp[j] = 1u << op.map_new[8 + j];
+  if (active) {
 #pragma unroll
-  for (int r = 0; r < (1 << EB); ++r) {
-    uint32_t t = tp;
+    for (int r = 0; r < (1 << EB); ++r) {
+      uint32_t t = tp;
 #pragma unroll
-    for (int j = 0; j < EB; ++j)
-      if ((r >> j) & 1) t |= rp[j];
-    v[r] = sm[lds_swz(t)];
+      for (int j = 0; j < EB; ++j)
+        if ((r >> j) & 1) t |= rp[j];
+      v[r] = sm[lds_swz(t)];
+    }
   }
   __syncthreads();
 }
@@ -622,6 +628,27 @@ __device__ __forceinline__ void reg_pair_op(T (&v)[1 << EB], const RegOp<R>& op,
 #undef NQPU_J2
 }
 
+// Run a register-tile program on v (the standalone pass and the batched
+// kernel share it).
+template <typename R, typename T, int EB>
+__device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<R>* ops, int nops, int tid,
+                                            bool active) {
+  for (int o = 0; o < nops; ++o) {
+    const RegOp<R>& op = ops[o];
+    const bool on = (tid & op.ct) == op.ct;
+    switch (op.kind) {
+      case R_REMAP: reg_remap<R, T, EB>(v, sm, op, tid, active); break;
+      case R_U1: NQPU_SLOT_CASES(reg_u1, v, op, on) break;
+      case R_FLIP: NQPU_SLOT_CASES(reg_flip, v, op, on) break;
+      case R_D1:
+      case R_D2: reg_diag<R, T, EB>(v, op, tid, on); break;
+      case R_U2: NQPU_SLOT_CASES(reg_pair_op, v, op, on, true) break;
+      case R_SWAP: NQPU_SLOT_CASES(reg_pair_op, v, op, on, false) break;
+      default: break;
+    }
+  }
+}
+
 template <typename R, typename T, int EB>
 __global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArgs pa, uint64_t ntiles,
                                                   const RegOp<R>* __restrict__ prog) {
@@ -646,20 +673,7 @@ __global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArg
         if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
       v[r] = st[d];
     }
-    for (int o = 0; o < pa.nops; ++o) {
-      const RegOp<R>& op = ops[o];
-      const bool on = (tid & op.ct) == op.ct;
-      switch (op.kind) {
-        case R_REMAP: reg_remap<R, T, EB>(v, sm, op, tid); break;
-        case R_U1: NQPU_SLOT_CASES(reg_u1, v, op, on) break;
-        case R_FLIP: NQPU_SLOT_CASES(reg_flip, v, op, on) break;
-        case R_D1:
-        case R_D2: reg_diag<R, T, EB>(v, op, tid, on); break;
-        case R_U2: NQPU_SLOT_CASES(reg_pair_op, v, op, on, true) break;
-        case R_SWAP: NQPU_SLOT_CASES(reg_pair_op, v, op, on, false) break;
-        default: break;
-      }
-    }
+    reg_program<R, T, EB>(v, sm, ops, pa.nops, tid, true);
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       uint64_t d = base | dlo;
@@ -696,6 +710,152 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 template <typename R, typename T>
 __device__ __forceinline__ double nsq(T a) {
   return (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+}
+
+// ------------------------------------------------------ batched circuits
+
+// sum_k coeff_k <P_k> contributions of this thread over an LDS tile of TT
+// amplitudes in plain (unswizzled) order whose global index is base | dep(t).
+__device__ double tile_terms(const float2* sm, uint32_t TT, const BatchTerm* terms, int nterm, uint64_t base,
+                             int tid) {
+  double acc = 0;
+  for (int k = 0; k < nterm; ++k) {
+    const BatchTerm tm = terms[k];
+    const double gs = (__popcll(base & tm.zg) & 1) ? -1.0 : 1.0;
+    double sr = 0, si = 0;
+    if (tm.xl == 0) {
+      for (uint32_t t = tid; t < TT; t += 256) {
+        const float2 a = sm[t];
+        const double p = (double)a.x * a.x + (double)a.y * a.y;
+        sr += (__popc(t & tm.zl) & 1) ? -p : p;
+      }
+    } else {
+      const int h = 31 - __clz(tm.xl);
+      for (uint32_t k2 = tid; k2 < (TT >> 1); k2 += 256) {
+        const uint32_t i = ins0u(k2, h), j = i ^ tm.xl;
+        const float2 ai = sm[i], aj = sm[j];
+        const double s_i = (__popc(i & tm.zl) & 1) ? -1.0 : 1.0;
+        const double s_j = (__popc(j & tm.zl) & 1) ? -1.0 : 1.0;
+        const double cr = (double)aj.x * ai.x + (double)aj.y * ai.y;
+        const double ci = (double)aj.x * ai.y - (double)aj.y * ai.x;
+        sr += (s_i + s_j) * cr;
+        si += (s_i - s_j) * ci;
+      }
+    }
+    const double val = tm.ny == 0 ? sr : (tm.ny == 1 ? -si : (tm.ny == 2 ? -sr : si));
+    acc += tm.coeff * gs * val;
+  }
+  return acc;
+}
+
+template <int EB>
+__global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, const BatchCircuit* __restrict__ circs,
+                                                  int ncirc, const BatchPass* __restrict__ passes,
+                                                  const RegOp<float>* __restrict__ prog,
+                                                  const BatchTerm* __restrict__ terms, unsigned* counter,
+                                                  double* __restrict__ out) {
+  using T = float2;
+  constexpr int E = 1 << EB;
+  extern __shared__ unsigned char smraw[];
+  T* sm = reinterpret_cast<T*>(smraw);
+  __shared__ double red[4];
+  __shared__ unsigned s_c;
+  const int tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) s_c = atomicAdd(counter, 1u);
+    __syncthreads();
+    const unsigned c = s_c;
+    __syncthreads();
+    if (c >= (unsigned)ncirc) break;  // the same for every thread of the workgroup
+    const BatchCircuit C = circs[c];
+    double acc = 0;
+    if (C.resident) {
+      const BatchPass P = passes[C.pass0];
+      const bool active = tid < (1 << P.tb);
+      T v[E];
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        v[r].x = (tid == 0 && r == 0) ? 1.f : 0.f;
+        v[r].y = 0.f;
+      }
+      reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, active);
+      if (active) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) sm[tid | (r << P.tb)] = v[r];
+      }
+      __syncthreads();
+      acc = tile_terms(sm, 1u << P.K, terms + P.term0, P.nterm, 0, tid);
+      __syncthreads();
+    } else {
+      T* st = states + C.state_off;
+      for (int p = 0; p < C.npass; ++p) {
+        const BatchPass P = passes[C.pass0 + p];
+        const uint64_t ntiles = 1ull << (C.n - P.K);
+        uint64_t dlo = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) dlo |= (uint64_t)((tid >> j) & 1) << P.q[j];
+        for (uint64_t tile = 0; tile < ntiles; ++tile) {
+          uint64_t base = tile;
+#pragma unroll
+          for (int j = 0; j < 8 + EB; ++j) base = ins0(base, P.q[j]);
+          T v[E];
+#pragma unroll
+          for (int r = 0; r < E; ++r) {
+            uint64_t d = base | dlo;
+#pragma unroll
+            for (int j = 0; j < EB; ++j)
+              if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
+            if (p == 0) {  // the circuit starts from |0...0>
+              v[r].x = d == 0 ? 1.f : 0.f;
+              v[r].y = 0.f;
+            } else {
+              v[r] = st[d];
+            }
+          }
+          reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true);
+#pragma unroll
+          for (int r = 0; r < E; ++r) {
+            uint64_t d = base | dlo;
+#pragma unroll
+            for (int j = 0; j < EB; ++j)
+              if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
+            st[d] = v[r];
+          }
+        }
+        __threadfence_block();
+        __syncthreads();
+      }
+      for (int rp = 0; rp < C.nred; ++rp) {
+        const BatchPass* P = passes + C.red0 + rp;
+        const int K = P->K;
+        const int32_t* q = P->q;  // read from memory: indexed by a loop bound known only at run time
+        const uint32_t TT = 1u << K;
+        const uint64_t ntiles = 1ull << (C.n - K);
+        for (uint64_t tile = 0; tile < ntiles; ++tile) {
+          uint64_t base = tile;
+          for (int j = 0; j < K; ++j) base = ins0(base, q[j]);
+          for (uint32_t t = tid; t < TT; t += 256) {
+            uint64_t d = base;
+            for (int j = 0; j < K; ++j) d |= (uint64_t)((t >> j) & 1) << q[j];
+            sm[t] = st[d];
+          }
+          __syncthreads();
+          acc += tile_terms(sm, TT, terms + P->term0, P->nterm, base, tid);
+          __syncthreads();
+        }
+      }
+    }
+    const double s = block_sum<256>(acc, red);
+    if (tid == 0) out[C.out] = s;
+  }
+}
+
+void launch_batch(void* states, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
+                  const RegOp<float>* prog, const BatchTerm* terms, unsigned* counter, double* out, int grid,
+                  hipStream_t s) {
+  const size_t lds = sizeof(float2) << (8 + kBatchEB);
+  hipLaunchKernelGGL(k_batch<kBatchEB>, dim3(grid), dim3(256), lds, s, (float2*)states, circs, ncirc, passes, prog,
+                     terms, counter, out);
 }
 
 // out[c] = sum |a_i|^2 over chunk c of 2^cb amplitudes.

@@ -122,6 +122,22 @@ NqpuError overlap(const State* a, const State* b, NqpuComplex* out);
 NqpuError sample(const State* s, const std::vector<double>& r, uint64_t* out);
 NqpuError measure_qubit(State* s, int q, double r, uint32_t* outcome, double* prob);
 
+// Batched expectation values over many complex64 circuits (k_batch).
+struct PauliTerm {
+  uint64_t x, z;
+  double coeff;
+};
+struct Batch {
+  hipStream_t stream = nullptr;
+  DeviceScratch states, tables, out;
+  NqpuBatchStats stats{};
+  ~Batch();
+};
+NqpuError create_batch(Batch** out);
+NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuits,
+                            const std::vector<std::vector<PauliTerm>>& terms, const NqpuRunOptions& o,
+                            double* out);
+
 // The C ABI's thread-local xorshift64* stream (c_ffi.rs:48-67).
 void rng_seed(uint64_t seed);
 double rng_next();

@@ -124,11 +124,14 @@ class NativeCircuit:
         return _stats(st)
 
     def emulate(self, amps, precision="c64", semantics="standard", tile_qubits=0, low_qubits=0,
-                lds_tiles=False):
+                lds_tiles=False, batch_layout=False):
         """Host emulation of the tiled GPU execution (plan, per-pass programs
-        and kernel semantics) in FP64; returns the new amplitudes."""
+        and kernel semantics) in FP64; returns the new amplitudes.  With
+        batch_layout, emulates the layout the batched kernel uses."""
         a = np.ascontiguousarray(amps, dtype=np.complex128).copy()
         o = run_options("tiled", semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles)
+        if batch_layout:
+            o.flags |= L.RUN_BATCH_LAYOUT
         L.check(L.load().nqpu_debug_emulate(self.ptr, ct.byref(o), _prec(precision), a.ctypes.data, len(a)),
                 "emulate")
         return a
@@ -340,6 +343,64 @@ class StateVector:
         m = L.NqpuMeasurement()
         L.check(L.load().nqpu_measure(self.ptr, q, ct.byref(m)), "measure")
         return int(m.outcome), float(m.probability)
+
+
+def _terms(obs):
+    """[(coeff, pauli string)] or [(coeff, xmask, zmask)] -> [(x, z, coeff)]."""
+    out = []
+    for t in obs:
+        if len(t) == 2:
+            x, z = pauli_masks(t[1])
+            out.append((x, z, float(t[0])))
+        else:
+            out.append((int(t[1]), int(t[2]), float(t[0])))
+    return out
+
+
+class Batch:
+    """Many complex64 circuits in one persistent GPU launch, each returning
+    the FP64 expectation of its own Pauli sum (the batched counterpart of
+    the reference's serial vmap_simulate, jax_bridge.rs:447)."""
+
+    def __init__(self):
+        lib = L.load()
+        self.ptr = lib.nqpu_batch_create()
+        if not self.ptr:
+            raise L.NqpuError(9, "nqpu_batch_create")
+
+    def expectation(self, circuits, observables, semantics="standard") -> np.ndarray:
+        lib = L.load()
+        natives = [c if isinstance(c, NativeCircuit) else c.native() for c in circuits]
+        arr = (ct.c_void_p * max(1, len(natives)))(*[nc.ptr for nc in natives])
+        terms, offsets = [], [0]
+        for obs in observables:
+            terms.extend(_terms(obs))
+            offsets.append(len(terms))
+        tarr = (L.NqpuPauliTerm * max(1, len(terms)))()
+        for i, (x, z, c) in enumerate(terms):
+            tarr[i].xmask, tarr[i].zmask, tarr[i].coeff = x, z, c
+        off = np.array(offsets, dtype=np.uint64)
+        out = np.empty(max(1, len(natives)), dtype=np.float64)
+        o = run_options("tiled", semantics)
+        L.check(lib.nqpu_batch_expectation(self.ptr, arr, len(natives), tarr, off.ctypes.data, ct.byref(o),
+                                           out.ctypes.data), "nqpu_batch_expectation")
+        return out[: len(natives)]
+
+    def last_stats(self) -> dict:
+        st = L.NqpuBatchStats()
+        L.check(L.load().nqpu_batch_last_stats(self.ptr, ct.byref(st)), "batch stats")
+        return {k: getattr(st, k) for k, _ in L.NqpuBatchStats._fields_}
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            L.load().nqpu_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def simulate(circuit: Circuit, precision="c128", **run_kw) -> StateVector:

@@ -53,3 +53,15 @@ def test_emulated_reference_semantics(oracle):
     oracle.apply_circuit(exp, 12, c.gates, quirk=True)
     got = c.native().emulate(a0, "c64", semantics="reference", tile_qubits=12)
     assert np.max(np.abs(got - exp)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [4, 5, 8, 11, 12, 13, 15])
+def test_emulated_batch_layout_matches_oracle(oracle, n):
+    """The register programs the batched kernel runs: whole-state tiles with
+    fewer than 8 thread bits (n <= 12) and 12-qubit HBM passes (n >= 13)."""
+    c = random_circuit(n, 150, 20 + n)
+    a0 = random_state(n, n)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    got = c.native().emulate(a0, "c64", batch_layout=True)
+    assert np.max(np.abs(got - exp)) < 1e-12
