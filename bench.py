@@ -65,9 +65,102 @@ def cpu_baseline(seconds: float, n: int = 26):
                       f"one full-state pass per gate ({dt:.1f} s)"}
 
 
+def bench_batch(args):
+    """4096 hardware-efficient-ansatz circuits (8-20 qubits, 4-12 layers of
+    RY.RZ + CZ ladder, seed 103), each returning its transverse-field Ising
+    energy: one persistent launch versus plain per-circuit launches on the
+    same GPU and the FP64 oracle on the host."""
+    import numpy as np
+
+    from nqpu_amd import engine as E
+    from nqpu_amd.workloads import variational_batch
+    from oracle import cpu_ref
+
+    if E.device_count() == 0:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    t0 = time.perf_counter()
+    vcs = variational_batch(args.circuits, seed=103)
+    circs = [v.circuit.native() for v in vcs]
+    obs = [v.hamiltonian() for v in vcs]
+    log(f"built {len(circs)} circuits in {time.perf_counter() - t0:.1f} s "
+        f"({sum(len(v.circuit.gates) for v in vcs)} gates)")
+    b = E.Batch()
+    for w in range(args.warmup):
+        b.expectation(circs, obs)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    kernel_ms = plan_ms = 0.0
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        energies = b.expectation(circs, obs)
+        st = b.last_stats()
+        kernel_ms += st["kernel_ms"]
+        plan_ms += st["plan_ms"]
+    elapsed = time.perf_counter() - t1
+    value = len(circs) * args.steps / elapsed
+    log(f"batched: {elapsed / args.steps * 1e3:.1f} ms per batch (kernel {kernel_ms / args.steps:.1f} ms)")
+    # plain per-circuit launches: one state, one nqpu_run and one Pauli sum per circuit
+    sample = list(range(0, len(circs), max(1, len(circs) // 512)))
+    t2 = time.perf_counter()
+    per = []
+    for k in sample:
+        n = vcs[k].circuit.num_qubits
+        with E.StateVector(n, "c64") as sv:
+            sv.run(circs[k])
+            per.append(sv.expectation_pauli_sum(obs[k]))
+    per_rate = len(sample) / (time.perf_counter() - t2)
+    log(f"per-circuit launches: {per_rate:.1f} circuits/s over {len(sample)} circuits")
+    # parity against the FP64 oracle on circuits of up to 16 qubits
+    check = [k for k in sample if vcs[k].circuit.num_qubits <= 16][:48]
+    errs = []
+    for k in check:
+        c = vcs[k].circuit
+        a = cpu_ref.run(c.num_qubits, c.gates)
+        ref = sum(cf * cpu_ref.expectation_pauli(a, c.num_qubits, *E.pauli_masks(p)) for cf, p in obs[k])
+        errs.append(abs(ref - energies[k]))
+    result = {
+        "metric": "batched circuits/s (4096 variational circuits, 8-20 qubits)",
+        "value": value, "unit": "circuits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64", "data": "synthetic",
+        "config": {"workload": "4096 hardware-efficient ansatz circuits, n ~ U{8..20}, L ~ U{4..12}, "
+                               "RY.RZ layers + CZ ladder, TFIM energy with g ~ U[0.2,1.8], seed 103, "
+                               "one persistent launch", "circuits": len(circs)},
+        "kernel_ms_per_batch": kernel_ms / args.steps, "plan_ms_per_batch": plan_ms / args.steps,
+        "batch_stats": st,
+        "gpu_per_circuit": {"circuits_per_s": per_rate, "sampled_circuits": len(sample)},
+        "speedup_vs_per_circuit": value / per_rate,
+        "energy_max_abs_err_vs_fp64_oracle": max(errs) if errs else None,
+        "checked_circuits": len(check),
+        "per_circuit_vs_batched_max_abs_diff": float(np.max(np.abs(np.array(per) - energies[sample]))),
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        t3 = time.perf_counter()
+        done = 0
+        for k in sample:
+            c = vcs[k].circuit
+            a = cpu_ref.run(c.num_qubits, c.gates)
+            for cf, p in obs[k]:
+                cpu_ref.expectation_pauli(a, c.num_qubits, *E.pauli_masks(p))
+            done += 1
+            if time.perf_counter() - t3 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t3
+        result["cpu_baseline"] = {"value": done / dt, "unit": "circuits/s", "cores": cpu_ref.num_threads(),
+                                  "kind": "port",
+                                  "sample": f"{done} circuits of the same batch (every {len(circs) // len(sample)}th), "
+                                            f"complex128, gate by gate, serial Pauli sums ({dt:.1f} s)"}
+        result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="random30", choices=["random30", "batch"],
+                    help="random30: 30-qubit random circuit (default); batch: 4096 variational circuits")
+    ap.add_argument("--circuits", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
@@ -86,6 +179,8 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the statevector path runs on a single MI355X)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     from nqpu_amd import engine as E
     from nqpu_amd.workloads import random_u3_brickwork

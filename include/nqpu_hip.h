@@ -310,6 +310,11 @@ NqpuError nqpu_batch_expectation(NqpuBatch* batch, const NqpuCircuit* const* cir
                                  const NqpuPauliTerm* terms, const uint64_t* offsets,
                                  const NqpuRunOptions* opts, double* out);
 NqpuError nqpu_batch_last_stats(const NqpuBatch* batch, NqpuBatchStats* out);
+/* Host half of nqpu_batch_expectation only (planning, no device): fills the
+ * pass counts of `out` and the number of register-program operations. */
+NqpuError nqpu_debug_batch_plan(const NqpuCircuit* const* circuits, uint64_t count, const NqpuPauliTerm* terms,
+                                const uint64_t* offsets, const NqpuRunOptions* opts, NqpuBatchStats* out,
+                                uint64_t* program_ops);
 
 /* Host-only emulation of nqpu_run's tiled execution for a state of the given
  * precision: the same plan and per-pass programs, interpreted on the CPU in

@@ -508,15 +508,15 @@ NqpuBatch* nqpu_batch_create(void) {
 
 void nqpu_batch_destroy(NqpuBatch* b) { delete reinterpret_cast<Batch*>(b); }
 
-NqpuError nqpu_batch_expectation(NqpuBatch* b, const NqpuCircuit* const* circuits, uint64_t count,
-                                 const NqpuPauliTerm* terms, const uint64_t* offsets, const NqpuRunOptions* opts,
-                                 double* out) {
-  if (!b || !out || (count && (!circuits || !offsets))) return NQPU_NULL_POINTER;
-  NqpuRunOptions o{NQPU_FUSION_TILED, NQPU_SEMANTICS_STANDARD, 0, 0, 0};
+static NqpuError batch_args(const NqpuCircuit* const* circuits, uint64_t count, const NqpuPauliTerm* terms,
+                            const uint64_t* offsets, const NqpuRunOptions* opts, std::vector<const Circuit*>& cs,
+                            std::vector<std::vector<PauliTerm>>& ts, NqpuRunOptions& o) {
+  if (count && (!circuits || !offsets)) return NQPU_NULL_POINTER;
+  o = NqpuRunOptions{NQPU_FUSION_TILED, NQPU_SEMANTICS_STANDARD, 0, 0, 0};
   if (opts) o = *opts;
   if (o.semantics > NQPU_SEMANTICS_REFERENCE) return NQPU_INVALID_PARAMETER;
-  std::vector<const Circuit*> cs(count);
-  std::vector<std::vector<PauliTerm>> ts(count);
+  cs.assign(count, nullptr);
+  ts.assign(count, {});
   for (uint64_t k = 0; k < count; ++k) {
     if (!circuits[k]) return NQPU_NULL_POINTER;
     cs[k] = Cc(circuits[k]);
@@ -527,7 +527,36 @@ NqpuError nqpu_batch_expectation(NqpuBatch* b, const NqpuCircuit* const* circuit
       ts[k].push_back(PauliTerm{terms[t].xmask, terms[t].zmask, terms[t].coeff});
     }
   }
+  return NQPU_OK;
+}
+
+NqpuError nqpu_batch_expectation(NqpuBatch* b, const NqpuCircuit* const* circuits, uint64_t count,
+                                 const NqpuPauliTerm* terms, const uint64_t* offsets, const NqpuRunOptions* opts,
+                                 double* out) {
+  if (!b || !out) return NQPU_NULL_POINTER;
+  std::vector<const Circuit*> cs;
+  std::vector<std::vector<PauliTerm>> ts;
+  NqpuRunOptions o;
+  NqpuError e = batch_args(circuits, count, terms, offsets, opts, cs, ts, o);
+  if (e != NQPU_OK) return e;
   return batch_expectation(reinterpret_cast<Batch*>(b), cs, ts, o, out);
+}
+
+NqpuError nqpu_debug_batch_plan(const NqpuCircuit* const* circuits, uint64_t count, const NqpuPauliTerm* terms,
+                                const uint64_t* offsets, const NqpuRunOptions* opts, NqpuBatchStats* out,
+                                uint64_t* program_ops) {
+  if (!out) return NQPU_NULL_POINTER;
+  std::vector<const Circuit*> cs;
+  std::vector<std::vector<PauliTerm>> ts;
+  NqpuRunOptions o;
+  NqpuError e = batch_args(circuits, count, terms, offsets, opts, cs, ts, o);
+  if (e != NQPU_OK) return e;
+  BatchPlan plan;
+  e = plan_batch(cs, ts, o, plan);
+  if (e != NQPU_OK) return e;
+  *out = plan.st;
+  if (program_ops) *program_ops = plan.program_ops;
+  return NQPU_OK;
 }
 
 NqpuError nqpu_batch_last_stats(const NqpuBatch* b, NqpuBatchStats* out) {

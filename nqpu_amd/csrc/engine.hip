@@ -1554,11 +1554,8 @@ static BatchTerm tile_term(const PauliTerm& t, const std::vector<int>& q) {
   return bt;
 }
 
-NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuits,
-                            const std::vector<std::vector<PauliTerm>>& terms, const NqpuRunOptions& o,
-                            double* out) {
-  using clock = std::chrono::steady_clock;
-  const auto t0 = clock::now();
+NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vector<std::vector<PauliTerm>>& terms,
+                     const NqpuRunOptions& o, BatchPlan& plan) {
   const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
   constexpr int EB = kBatchEB, KH = 8 + kBatchEB;
   std::vector<BatchCircuit> bc;
@@ -1694,13 +1691,40 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   std::memcpy(host.data() + op_, bp.data(), bp.size() * sizeof(BatchPass));
   if (!prog.empty()) std::memcpy(host.data() + og, prog.data(), prog.size() * sizeof(RegOp<float>));
   if (!bt.empty()) std::memcpy(host.data() + ot, bt.data(), bt.size() * sizeof(BatchTerm));
+  plan.host = std::move(host);
+  plan.oc = oc;
+  plan.op = op_;
+  plan.og = og;
+  plan.ot = ot;
+  plan.ocnt = ocnt;
+  plan.total = total;
+  plan.state_amps = state_amps;
+  plan.ncirc = sorted.size();
+  plan.program_ops = prog.size();
+  st.circuits = circuits.size();
+  st.state_bytes = (double)state_amps * 8.0;
+  plan.st = st;
+  return NQPU_OK;
+}
+
+NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuits,
+                            const std::vector<std::vector<PauliTerm>>& terms, const NqpuRunOptions& o,
+                            double* out) {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
+  BatchPlan plan;
+  NqpuError pe = plan_batch(circuits, terms, o, plan);
+  if (pe != NQPU_OK) return pe;
+  const size_t oc = plan.oc, op_ = plan.op, og = plan.og, ot = plan.ot, ocnt = plan.ocnt, total = plan.total;
+  const uint64_t state_amps = plan.state_amps;
+  NqpuBatchStats st = plan.st;
   HIPTRY(hipStreamSynchronize(b->stream), "sync");
   if (!b->tables.ensure(total) || !b->out.ensure(circuits.size() * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
   if (state_amps && !b->states.ensure(state_amps * 8)) {
     set_error("batch states do not fit in device memory");
     return NQPU_OUT_OF_DEVICE_MEMORY;
   }
-  HIPTRY(hipMemcpy(b->tables.p, host.data(), total, hipMemcpyHostToDevice), "batch table upload");
+  HIPTRY(hipMemcpy(b->tables.p, plan.host.data(), total, hipMemcpyHostToDevice), "batch table upload");
   char* base = (char*)b->tables.p;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
@@ -1711,7 +1735,7 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   HIPTRY(hipEventCreate(&e1), "event");
   (void)hipEventRecord(e0, b->stream);
   if (grid > 0)
-    launch_batch(b->states.p, (const BatchCircuit*)(base + oc), (int)sorted.size(), (const BatchPass*)(base + op_),
+    launch_batch(b->states.p, (const BatchCircuit*)(base + oc), (int)plan.ncirc, (const BatchPass*)(base + op_),
                  (const RegOp<float>*)(base + og), (const BatchTerm*)(base + ot), (unsigned*)(base + ocnt),
                  (double*)b->out.p, grid, b->stream);
   NqpuError le = hip_check(hipGetLastError(), "batch launch");

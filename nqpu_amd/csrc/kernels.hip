@@ -761,12 +761,15 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
   __shared__ double red[4];
   __shared__ unsigned s_c;
   const int tid = threadIdx.x;
-  for (;;) {
-    if (tid == 0) s_c = atomicAdd(counter, 1u);
-    __syncthreads();
-    const unsigned c = s_c;
-    __syncthreads();
-    if (c >= (unsigned)ncirc) break;  // the same for every thread of the workgroup
+  // The queue index is made wave-uniform (readfirstlane): with a value the
+  // compiler must treat as divergent, the loop exit around the barriers
+  // below is restructured per lane and lanes other than the one doing the
+  // atomic keep iterating on a stale index.
+  if (tid == 0) s_c = atomicAdd(counter, 1u);
+  __syncthreads();
+  unsigned c = __builtin_amdgcn_readfirstlane(s_c);
+  __syncthreads();
+  while (c < (unsigned)ncirc) {
     const BatchCircuit C = circs[c];
     double acc = 0;
     if (C.resident) {
@@ -846,7 +849,13 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
       }
     }
     const double s = block_sum<256>(acc, red);
-    if (tid == 0) out[C.out] = s;
+    if (tid == 0) {
+      out[C.out] = s;
+      s_c = atomicAdd(counter, 1u);
+    }
+    __syncthreads();
+    c = __builtin_amdgcn_readfirstlane(s_c);
+    __syncthreads();
   }
 }
 

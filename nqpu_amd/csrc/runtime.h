@@ -133,6 +133,16 @@ struct Batch {
   NqpuBatchStats stats{};
   ~Batch();
 };
+// Host half of a batch: the tables k_batch reads, in one upload image.
+struct BatchPlan {
+  std::vector<char> host;
+  size_t oc = 0, op = 0, og = 0, ot = 0, ocnt = 0, total = 0;
+  uint64_t state_amps = 0;
+  size_t ncirc = 0, program_ops = 0;
+  NqpuBatchStats st{};
+};
+NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vector<std::vector<PauliTerm>>& terms,
+                     const NqpuRunOptions& o, BatchPlan& plan);
 NqpuError create_batch(Batch** out);
 NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuits,
                             const std::vector<std::vector<PauliTerm>>& terms, const NqpuRunOptions& o,
