@@ -74,7 +74,6 @@ def bench_batch(args):
 
     from nqpu_amd import engine as E
     from nqpu_amd.workloads import variational_batch
-    from oracle import cpu_ref
 
     if E.device_count() == 0:
         print("no HIP device visible", file=sys.stderr)
@@ -110,14 +109,6 @@ def bench_batch(args):
             per.append(sv.expectation_pauli_sum(obs[k]))
     per_rate = len(sample) / (time.perf_counter() - t2)
     log(f"per-circuit launches: {per_rate:.1f} circuits/s over {len(sample)} circuits")
-    # parity against the FP64 oracle on circuits of up to 16 qubits
-    check = [k for k in sample if vcs[k].circuit.num_qubits <= 16][:48]
-    errs = []
-    for k in check:
-        c = vcs[k].circuit
-        a = cpu_ref.run(c.num_qubits, c.gates)
-        ref = sum(cf * cpu_ref.expectation_pauli(a, c.num_qubits, *E.pauli_masks(p)) for cf, p in obs[k])
-        errs.append(abs(ref - energies[k]))
     result = {
         "metric": "batched circuits/s (4096 variational circuits, 8-20 qubits)",
         "value": value, "unit": "circuits/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -130,12 +121,21 @@ def bench_batch(args):
         "batch_stats": st,
         "gpu_per_circuit": {"circuits_per_s": per_rate, "sampled_circuits": len(sample)},
         "speedup_vs_per_circuit": value / per_rate,
-        "energy_max_abs_err_vs_fp64_oracle": max(errs) if errs else None,
-        "checked_circuits": len(check),
         "per_circuit_vs_batched_max_abs_diff": float(np.max(np.abs(np.array(per) - energies[sample]))),
     }
     if not args.no_cpu:
         log("cpu baseline")
+        from oracle import cpu_ref
+        # the oracle's energies of circuits of up to 16 qubits check the batched ones
+        check = [k for k in sample if vcs[k].circuit.num_qubits <= 16][:48]
+        errs = []
+        for k in check:
+            c = vcs[k].circuit
+            a = cpu_ref.run(c.num_qubits, c.gates)
+            ref = sum(cf * cpu_ref.expectation_pauli(a, c.num_qubits, *E.pauli_masks(p)) for cf, p in obs[k])
+            errs.append(abs(ref - energies[k]))
+        result["energy_max_abs_err_vs_fp64_oracle"] = max(errs) if errs else None
+        result["checked_circuits"] = len(check)
         t3 = time.perf_counter()
         done = 0
         for k in sample:
@@ -155,11 +155,116 @@ def bench_batch(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_qft_trotter(args):
+    """Memory-ceiling state: a 34-qubit complex64 state (128 GiB) runs a QFT
+    followed by 20 first-order Trotter steps (dt = 0.05, g = 1) of the 1D
+    transverse-field Ising chain, then a 33-qubit complex128 state (128 GiB)
+    runs the same circuit.  One step = reset to |0...0>, the whole circuit
+    and (complex64) the FP64 renormalisation of the result.  <Z0 Z1> after this circuit depends only on qubits 0..21 (each
+    Trotter step widens the light cone of Z0 Z1 by one qubit and the QFT of
+    |0...0> is a product state), so the FP64 oracle run at 22 qubits gives
+    the exact value for every size."""
+    from nqpu_amd import engine as E
+    from nqpu_amd.workloads import qft_trotter
+
+    if E.device_count() == 0:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    n = args.qubits if args.qubits != 30 else 34
+    n128 = n - 1
+    circ = qft_trotter(n, 20)
+    nc = circ.native()
+    gates = len(circ.gates)
+    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds)
+    log(f"workload n={n} gates={gates} plan={nc.plan('c64', **run_kw)}")
+    zz = "ZZ" + "I" * (n - 2)
+    sv = E.StateVector(n, "c64")
+    for w in range(args.warmup):
+        sv.reset()
+        sv.run(nc, **run_kw)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    pass_ms, timed_passes, passes = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sv.reset()
+        st = sv.run(nc, time_passes=True, **run_kw)
+        norm_raw = sv.norm()
+        sv.normalize()  # FP64 renormalisation on the GPU
+        pass_ms += st["pass_ms"]
+        timed_passes += st["timed_passes"]
+        passes = st["passes"]
+        log(f"step {k + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t0
+    dim = 1 << n
+    value = gates * dim * args.steps / elapsed
+    bytes_per_pass = 2.0 * dim * 8
+    avg_pass_ms = pass_ms / max(timed_passes, 1)
+    achieved = bytes_per_pass / (avg_pass_ms * 1e-3) / 1e9
+    z64 = sv.expectation_pauli(zz)
+    norm64 = sv.norm()
+    sv.close()
+    log(f"c64: {elapsed / args.steps:.3f} s per step, <Z0Z1> = {z64!r}, norm = {norm64!r}")
+
+    circ128 = qft_trotter(n128, 20)
+    nc128 = circ128.native()
+    sv = E.StateVector(n128, "c128")
+    t1 = time.perf_counter()
+    st128 = sv.run(nc128, time_passes=True, **run_kw)
+    dt128 = time.perf_counter() - t1
+    z128 = sv.expectation_pauli("ZZ" + "I" * (n128 - 2))
+    norm128 = sv.norm()
+    sv.close()
+    log(f"c128 {n128} qubits: {dt128:.3f} s, <Z0Z1> = {z128!r}, norm = {norm128!r}")
+    result = {
+        "metric": f"amp-updates/s ({n}-qubit complex64 QFT + 20 Trotter steps)",
+        "value": value, "unit": "amp-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64", "data": "synthetic",
+        "config": {"workload": f"QFT + 20 first-order TFIM Trotter steps (dt 0.05, g 1), {n} qubits complex64 "
+                               f"(128 GiB), {n128} qubits complex128 check", "qubits": n, "gates": gates,
+                   "passes_per_step": passes},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tile" if args.lds else "k_tile_reg", "avg_launch_ms": avg_pass_ms,
+                     "bytes_per_launch": bytes_per_pass},
+        "z0z1_c64": z64, "norm_c64": norm64, "norm_c64_before_renormalisation": norm_raw,
+        "c128": {"qubits": n128, "seconds": dt128, "passes": st128["passes"], "z0z1": z128, "norm": norm128,
+                 "amp_updates_per_s": len(circ128.gates) * float(1 << n128) / dt128},
+        "z0z1_abs_diff_c64_vs_c128": abs(z64 - z128),
+    }
+    if not args.no_cpu:
+        log("cpu baseline: FP64 oracle, same circuit at 22 and 26 qubits")
+        from oracle import cpu_ref
+        a = cpu_ref.run(22, qft_trotter(22, 20).gates)
+        zref = cpu_ref.expectation_pauli(a, 22, 0, 3)
+        result["z0z1_fp64_oracle_22q"] = zref
+        result["z0z1_abs_err_c64_vs_oracle"] = abs(z64 - zref)
+        result["z0z1_abs_err_c128_vs_oracle"] = abs(z128 - zref)
+        nb = 26
+        cb = qft_trotter(nb, 20)
+        a = cpu_ref.zero_state(nb)
+        done = 0
+        t3 = time.perf_counter()
+        for g in cb.gates:
+            cpu_ref.apply_gate(a, nb, g)
+            done += 1
+            if time.perf_counter() - t3 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t3
+        result["cpu_baseline"] = {"value": done * float(1 << nb) / dt, "unit": "amp-updates/s",
+                                  "cores": cpu_ref.num_threads(), "kind": "port",
+                                  "sample": f"first {done} gates of the same circuit at {nb} qubits, complex128, "
+                                            f"one full-state pass per gate ({dt:.1f} s)"}
+        result["speedup_vs_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="random30", choices=["random30", "batch"],
-                    help="random30: 30-qubit random circuit (default); batch: 4096 variational circuits")
+    ap.add_argument("--workload", default="random30", choices=["random30", "batch", "qft34"],
+                    help="random30: 30-qubit random circuit (default); batch: 4096 variational circuits; "
+                         "qft34: 34-qubit complex64 QFT + Trotter (128 GiB) with a 33-qubit complex128 check")
     ap.add_argument("--circuits", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -181,6 +286,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "qft34":
+        return bench_qft_trotter(args)
 
     from nqpu_amd import engine as E
     from nqpu_amd.workloads import random_u3_brickwork

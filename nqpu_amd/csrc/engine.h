@@ -35,14 +35,18 @@ struct HostOp {
   std::vector<int32_t> dq;   // OP_DIAGK qubits
   std::vector<cd> lut;       // OP_DIAGK phases, index = bits of dq
   uint64_t qubit_mask() const;
+  // Qubits that must be bits of a tile for the op to run inside a tile pass:
+  // the targets of non-diagonal ops.  Controls and diagonal factors on other
+  // qubits are fixed per tile and resolved from the tile's base index.
+  uint64_t tile_mask() const;
 };
 
 // Operation as the tile kernel reads it: qubit positions are tile-local bits.
 template <typename R>
 struct DevOp {
-  int32_t kind, a, b, pad;
+  int32_t kind, a, b, pad;  // OP_D1/OP_D2: a, b >= kGlobalBit name qubit a-kGlobalBit outside the tile
   uint64_t cmask;
-  uint64_t pad2;
+  uint64_t gc;              // controls outside the tile (global qubit mask)
   R m[32];  // interleaved re/im, up to 4x4
 };
 
@@ -54,7 +58,8 @@ enum RegOpKind : int32_t {
   R_REMAP = 0,  // exchange through LDS: slots change from map_old to map_new
   R_U1 = 1,     // 2x2 on register slot s0
   R_FLIP = 2,   // X on register slot s0
-  R_D1 = 3,     // diag(m[0], m[1]) on position s0 (>= 16: register slot s0-16, else thread slot)
+  R_D1 = 3,     // diag(m[0], m[1]) on position s0 (>= kGlobalBit: qubit s0-kGlobalBit outside the
+                //   tile; >= 16: register slot s0-16; else thread slot)
   R_D2 = 4,     // diag(m[0..3]) on positions s0 (index bit 0), s1 (index bit 1)
   R_U2 = 5,     // 4x4 on register slots s0 (index bit 0), s1 (index bit 1)
   R_SWAP = 6    // exchange register slots s0 and s1 (controlled SWAP)
@@ -67,9 +72,13 @@ struct RegOp {
   uint32_t cr;  // controls on register slots (bits of the register index)
   uint8_t map_old[16];
   uint8_t map_new[16];
-  uint64_t pad2;
+  uint64_t gc;  // controls outside the tile (global qubit mask): the op runs on tiles that satisfy them
   R m[32];
 };
+
+// Diagonal-op positions at or above kGlobalBit name qubit (pos - kGlobalBit)
+// outside the tile; its value is the same for the whole tile.
+constexpr int kGlobalBit = 64;
 
 constexpr int kMaxTile = 14;  // maximum tile qubits (LDS: 2^13 * 16 B = 128 KiB)
 

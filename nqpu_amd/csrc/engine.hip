@@ -41,6 +41,15 @@ uint64_t HostOp::qubit_mask() const {
   return m;
 }
 
+uint64_t HostOp::tile_mask() const {
+  switch (kind) {
+    case OP_U1: case OP_FLIP: return bit(a);
+    case OP_U2: case OP_SWAP: return bit(a) | bit(b);
+    case OP_D1: case OP_D2: return 0;
+    default: return qubit_mask();
+  }
+}
+
 uint64_t Gate::qubit_mask() const {
   uint64_t m = 0;
   for (int i = 0; i < nt; ++i) m |= bit(t[i]);
@@ -293,12 +302,12 @@ std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOp
     size_t i = 0;
     for (; i < rem.size(); ++i) {
       const HostOp& o = ops[rem[i]];
-      uint64_t qm = o.qubit_mask();
+      const uint64_t qm = o.qubit_mask(), tm = po.global_ops ? o.tile_mask() : qm;
       if (qm & blocked) {
         blocked |= qm;
         next.push_back(rem[i]);
-      } else if (__builtin_popcountll(S | qm) <= K) {
-        S |= qm;
+      } else if (__builtin_popcountll(S | tm) <= K) {
+        S |= tm;
         chosen.push_back(rem[i]);
       } else {
         blocked |= qm;
@@ -734,28 +743,37 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
   int loc[64];
   for (int q = 0; q < 64; ++q) loc[q] = -1;
   for (int j = 0; j < K; ++j) loc[p.qubits[j]] = j;
+  // a, b: tile-local bits, or kGlobalBit + qubit for a diagonal factor on a
+  // qubit outside the tile; gc: controls outside the tile
   struct L {
     int kind, a, b;
     uint32_t cm;
+    uint64_t gc;
     const HostOp* h;
     bool done;
   };
+  auto local = [&](int q) { return loc[q] >= 0 ? loc[q] : kGlobalBit + q; };
   std::vector<L> ops;
   ops.reserve(p.ops.size());
   for (const HostOp& o : p.ops) {
-    L l{o.kind, loc[o.a], 0, 0, &o, false};
-    if (o.kind == OP_U2 || o.kind == OP_SWAP || o.kind == OP_D2) l.b = loc[o.b];
+    L l{o.kind, local(o.a), 0, 0, 0, &o, false};
+    if (o.kind == OP_U2 || o.kind == OP_SWAP || o.kind == OP_D2) l.b = local(o.b);
     for (int q = 0; q < 64; ++q)
-      if ((o.cmask >> q) & 1) l.cm |= 1u << loc[q];
+      if ((o.cmask >> q) & 1) {
+        if (loc[q] >= 0) l.cm |= 1u << loc[q];
+        else l.gc |= bit(q);
+      }
     ops.push_back(l);
   }
   auto two = [](int k) { return k == OP_U2 || k == OP_SWAP || k == OP_D2; };
-  auto qmask = [&](const L& o) { return o.cm | (1u << o.a) | (two(o.kind) ? (1u << o.b) : 0u); };
+  // tile bits an op touches (bits outside the tile are constant over it)
+  auto lbit = [](int x) { return x < kGlobalBit ? 1u << x : 0u; };
+  auto qmask = [&](const L& o) { return o.cm | lbit(o.a) | (two(o.kind) ? lbit(o.b) : 0u); };
   auto targets = [&](const L& o) -> uint32_t {
     switch (o.kind) {
       case OP_U1: case OP_FLIP: return 1u << o.a;
       case OP_U2: return (1u << o.a) | (1u << o.b);
-      case OP_SWAP: return o.cm ? ((1u << o.a) | (1u << o.b)) : 0u;
+      case OP_SWAP: return (o.cm || o.gc) ? ((1u << o.a) | (1u << o.b)) : 0u;
       default: return 0u;
     }
   };
@@ -789,7 +807,7 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
     for (int i = 0; i < 16; ++i) map[i] = nm[i];
   };
   auto emit = [&](const L& o) {
-    if (o.kind == OP_SWAP && o.cm == 0) {  // relabel
+    if (o.kind == OP_SWAP && o.cm == 0 && o.gc == 0) {  // relabel
       int sa = slot(o.a), sb = slot(o.b);
       int ia = sa >= 16 ? 8 + (sa - 16) : sa, ib = sb >= 16 ? 8 + (sb - 16) : sb;
       std::swap(map[ia], map[ib]);
@@ -802,13 +820,15 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
         if (sq >= 16) r.cr |= 1u << (sq - 16);
         else r.ct |= 1u << sq;
       }
+    r.gc = o.gc;
+    auto dpos = [&](int x) { return x >= kGlobalBit ? x : slot(x); };  // diagonal factor position
     const HostOp& h = *o.h;
     int nm = 0;
     switch (o.kind) {
       case OP_U1: r.kind = R_U1; r.s0 = slot(o.a) - 16; nm = 4; break;
       case OP_FLIP: r.kind = R_FLIP; r.s0 = slot(o.a) - 16; break;
-      case OP_D1: r.kind = R_D1; r.s0 = slot(o.a); nm = 2; break;
-      case OP_D2: r.kind = R_D2; r.s0 = slot(o.a); r.s1 = slot(o.b); nm = 4; break;
+      case OP_D1: r.kind = R_D1; r.s0 = dpos(o.a); nm = 2; break;
+      case OP_D2: r.kind = R_D2; r.s0 = dpos(o.a); r.s1 = dpos(o.b); nm = 4; break;
       case OP_U2: r.kind = R_U2; r.s0 = slot(o.a) - 16; r.s1 = slot(o.b) - 16; nm = 16; break;
       case OP_SWAP: r.kind = R_SWAP; r.s0 = slot(o.a) - 16; r.s1 = slot(o.b) - 16; break;
       default: return;
@@ -873,6 +893,97 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
   return prog;
 }
 
+// A pass's ops in the LDS tile kernel's encoding: tile-local bit positions,
+// controls outside the tile in gc, diagonal factors outside the tile as
+// kGlobalBit + qubit.
+template <typename R>
+static void append_dev_ops(const Pass& p, std::vector<DevOp<R>>& out) {
+  int loc[64];
+  for (int q = 0; q < 64; ++q) loc[q] = -1;
+  for (size_t j = 0; j < p.qubits.size(); ++j) loc[p.qubits[j]] = (int)j;
+  auto pos = [&](int q) { return loc[q] >= 0 ? loc[q] : kGlobalBit + q; };
+  for (const HostOp& o : p.ops) {
+    DevOp<R> d{};
+    d.kind = o.kind;
+    d.a = pos(o.a);
+    d.b = (o.kind == OP_U2 || o.kind == OP_SWAP || o.kind == OP_D2) ? pos(o.b) : 0;
+    for (int q = 0; q < 64; ++q)
+      if ((o.cmask >> q) & 1) {
+        if (loc[q] >= 0) d.cmask |= bit(loc[q]);
+        else d.gc |= bit(q);
+      }
+    const int nm = (o.kind == OP_U2) ? 16 : (o.kind == OP_U1 ? 4 : (o.kind == OP_D2 ? 4 : 2));
+    for (int i = 0; i < nm; ++i) {
+      d.m[2 * i] = (R)o.m[i].real();
+      d.m[2 * i + 1] = (R)o.m[i].imag();
+    }
+    out.push_back(d);
+  }
+}
+
+// Host emulation of one k_tile / k_tile_small pass: per tile, the ops in the
+// kernel's encoding applied to the gathered tile.
+static void emulate_lds_pass(std::vector<cd>& a, int n, const Pass& p, const std::vector<DevOp<double>>& dops) {
+  const int K = (int)p.qubits.size();
+  const uint32_t TT = 1u << K;
+  const uint64_t ntiles = 1ull << (n - K);
+  std::vector<cd> t(TT);
+  for (uint64_t tile = 0; tile < ntiles; ++tile) {
+    uint64_t base = tile;
+    for (int j = 0; j < K; ++j) {
+      const int q = p.qubits[j];
+      base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1));
+    }
+    auto dep = [&](uint32_t e) {
+      uint64_t d = base;
+      for (int j = 0; j < K; ++j)
+        if ((e >> j) & 1) d |= 1ull << p.qubits[j];
+      return d;
+    };
+    for (uint32_t e = 0; e < TT; ++e) t[e] = a[dep(e)];
+    for (const DevOp<double>& op : dops) {
+      if ((base & op.gc) != op.gc) continue;
+      const uint32_t cm = (uint32_t)op.cmask;
+      auto bitof = [&](int x, uint32_t e) -> int {
+        return x >= kGlobalBit ? (int)((base >> (x - kGlobalBit)) & 1) : (int)((e >> x) & 1);
+      };
+      cd m[16];
+      for (int i = 0; i < 16; ++i) m[i] = cd(op.m[2 * i], op.m[2 * i + 1]);
+      for (uint32_t e = 0; e < TT; ++e) {
+        if ((e & cm) != cm) continue;
+        switch (op.kind) {
+          case OP_U1:
+            if (!((e >> op.a) & 1)) {
+              const uint32_t f = e | (1u << op.a);
+              cd x = t[e], y = t[f];
+              t[e] = m[0] * x + m[1] * y;
+              t[f] = m[2] * x + m[3] * y;
+            }
+            break;
+          case OP_FLIP:
+            if (!((e >> op.a) & 1)) std::swap(t[e], t[e | (1u << op.a)]);
+            break;
+          case OP_D1: t[e] *= m[bitof(op.a, e)]; break;
+          case OP_D2: t[e] *= m[bitof(op.a, e) | (bitof(op.b, e) << 1)]; break;
+          case OP_U2:
+            if (!((e >> op.a) & 1) && !((e >> op.b) & 1)) {
+              const uint32_t idx[4] = {e, e | (1u << op.a), e | (1u << op.b), e | (1u << op.a) | (1u << op.b)};
+              cd x[4] = {t[idx[0]], t[idx[1]], t[idx[2]], t[idx[3]]};
+              for (int r = 0; r < 4; ++r)
+                t[idx[r]] = m[4 * r] * x[0] + m[4 * r + 1] * x[1] + m[4 * r + 2] * x[2] + m[4 * r + 3] * x[3];
+            }
+            break;
+          case OP_SWAP:
+            if (((e >> op.a) & 1) && !((e >> op.b) & 1)) std::swap(t[e], t[(e ^ (1u << op.a)) | (1u << op.b)]);
+            break;
+          default: break;
+        }
+      }
+    }
+    for (uint32_t e = 0; e < TT; ++e) a[dep(e)] = t[e];
+  }
+}
+
 // Host emulation of one k_tile_reg pass over a complex128 vector, following
 // the kernel's thread / register / LDS semantics step by step.
 static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std::vector<RegOp<double>>& prog,
@@ -908,6 +1019,7 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std
           for (int r = 0; r < E; ++r) v[tid * E + r] = lds[tmap(op.map_new, tid, r)];
         continue;
       }
+      if ((base & op.gc) != op.gc) continue;  // a control outside the tile is 0 here
       cd m[16];
       for (int i = 0; i < 16; ++i) m[i] = cd(op.m[2 * i], op.m[2 * i + 1]);
       for (int tid = 0; tid < NTH; ++tid) {
@@ -929,7 +1041,10 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std
               break;
             case R_D1:
             case R_D2: {
-              auto bitof = [&](int code) { return code >= 16 ? (r >> (code - 16)) & 1 : (tid >> code) & 1; };
+              auto bitof = [&](int code) -> int {
+                if (code >= kGlobalBit) return (int)((base >> (code - kGlobalBit)) & 1);
+                return code >= 16 ? (r >> (code - 16)) & 1 : (tid >> code) & 1;
+              };
               int s = bitof(j0) | (op.kind == R_D2 ? bitof(j1) << 1 : 0);
               w[r] *= m[s];
               break;
@@ -1016,9 +1131,13 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
   const int K = std::min(po.tile_qubits, c.n);
   const bool reg = po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13;
   for (const Pass& p : passes) {
-    if (reg) emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8), K - 8);
-    else
-      for (const HostOp& op : p.ops) emulate_op(a, c.n, op);
+    if (reg) {
+      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8), K - 8);
+    } else {
+      std::vector<DevOp<double>> dops;
+      append_dev_ops<double>(p, dops);
+      emulate_lds_pass(a, c.n, p, dops);
+    }
   }
   return NQPU_OK;
 }
@@ -1072,28 +1191,8 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
     pa.n = s->n;
     pa.op0 = (int)dops.size();
     pa.nops = (int)p.ops.size();
-    int loc[64];
-    for (int q = 0; q < 64; ++q) loc[q] = -1;
-    for (int j = 0; j < pa.K; ++j) {
-      pa.q[j] = p.qubits[j];
-      loc[p.qubits[j]] = j;
-    }
-    for (const HostOp& o : p.ops) {
-      DevOp<R> d{};
-      d.kind = o.kind;
-      d.a = loc[o.a];
-      d.b = (o.kind == OP_U2 || o.kind == OP_SWAP || o.kind == OP_D2) ? loc[o.b] : 0;
-      uint64_t cm = 0;
-      for (int q = 0; q < 64; ++q)
-        if ((o.cmask >> q) & 1) cm |= bit(loc[q]);
-      d.cmask = cm;
-      int nm = (o.kind == OP_U2) ? 16 : (o.kind == OP_U1 ? 4 : (o.kind == OP_D2 ? 4 : 2));
-      for (int i = 0; i < nm; ++i) {
-        d.m[2 * i] = (R)o.m[i].real();
-        d.m[2 * i + 1] = (R)o.m[i].imag();
-      }
-      dops.push_back(d);
-    }
+    for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
+    append_dev_ops<R>(p, dops);
     args.push_back(pa);
   }
   size_t ob = std::max<size_t>(dops.size(), 1) * sizeof(DevOp<R>);

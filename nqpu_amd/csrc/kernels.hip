@@ -262,12 +262,21 @@ __global__ void k_scale(T* st, uint64_t dim, double f) {
 // the store; operations re-partition the tile among threads as pairs /
 // groups, so every operation ends with a barrier.
 
+// Bit x of tile element t, or of the tile's base index for x >= kGlobalBit.
+__device__ __forceinline__ bool tile_bit(uint32_t t, int x, uint64_t base) {
+  return x >= kGlobalBit ? ((base >> (x - kGlobalBit)) & 1) : ((t >> x) & 1);
+}
+
 template <typename R, typename T, int NT>
-__device__ __forceinline__ void tile_apply_ops(T* sm, const int K, const DevOp<R>* ops, int nops) {
+__device__ __forceinline__ void tile_apply_ops(T* sm, const int K, const DevOp<R>* ops, int nops, uint64_t base) {
   const uint32_t TT = 1u << K;
   const int tid = threadIdx.x;
   for (int o = 0; o < nops; ++o) {
     const DevOp<R>& op = ops[o];
+    // a control outside the tile is 0 on this tile: the op is the identity
+    // here (the test is uniform over the workgroup, so skipping the barrier
+    // is too)
+    if ((base & op.gc) != op.gc) continue;
     const int kind = op.kind, a = op.a, b = op.b;
     const uint32_t cm = (uint32_t)op.cmask;
     if (kind == OP_U1) {
@@ -294,12 +303,12 @@ __device__ __forceinline__ void tile_apply_ops(T* sm, const int K, const DevOp<R
       p1.x = op.m[2]; p1.y = op.m[3];
       for (uint32_t t = tid; t < TT; t += NT) {
         if ((t & cm) != cm) continue;
-        sm[t] = cmul_(((t >> a) & 1) ? p1 : p0, sm[t]);
+        sm[t] = cmul_(tile_bit(t, a, base) ? p1 : p0, sm[t]);
       }
     } else if (kind == OP_D2) {
       for (uint32_t t = tid; t < TT; t += NT) {
         if ((t & cm) != cm) continue;
-        const bool ba = (t >> a) & 1, bb = (t >> b) & 1;
+        const bool ba = tile_bit(t, a, base), bb = tile_bit(t, b, base);
         const int s = (bb ? 4 : 0) + (ba ? 2 : 0);  // index of the phase's real part
         T ph;
         ph.x = op.m[s];
@@ -373,7 +382,7 @@ __global__ __launch_bounds__(NT) void k_tile(T* __restrict__ st, PassArgs pa, ui
 #pragma unroll
     for (int r = 0; r < EPT; ++r) sm[tid + r * NT] = v[r];
     __syncthreads();
-    tile_apply_ops<R, T, NT>(sm, K, myops, pa.nops);
+    tile_apply_ops<R, T, NT>(sm, K, myops, pa.nops, base);
 #pragma unroll
     for (int r = 0; r < EPT; ++r) v[r] = sm[tid + r * NT];
 #pragma unroll
@@ -400,7 +409,7 @@ __global__ __launch_bounds__(NT) void k_tile_small(T* __restrict__ st, PassArgs 
       sm[t] = st[d];
     }
     __syncthreads();
-    tile_apply_ops<R, T, NT>(sm, K, ops + pa.op0, pa.nops);
+    tile_apply_ops<R, T, NT>(sm, K, ops + pa.op0, pa.nops, base);
     for (uint32_t t = threadIdx.x; t < TT; t += NT) {
       uint64_t d = base;
       for (int j = 0; j < K; ++j) d |= (uint64_t)((t >> j) & 1) << pa.q[j];
@@ -520,25 +529,50 @@ __device__ __forceinline__ void reg_swap(T (&v)[1 << EB], const RegOp<R>& op, bo
 }
 
 template <typename R, typename T, int EB>
-__device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, int tid, bool on) {
+__device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, int tid, bool on, uint64_t base) {
   T p0, p1, p2, p3;
   p0.x = op.m[0]; p0.y = op.m[1];
   p1.x = op.m[2]; p1.y = op.m[3];
   p2.x = op.m[4]; p2.y = op.m[5];
   p3.x = op.m[6]; p3.y = op.m[7];
   const bool two = op.kind == R_D2;
-  const int a = op.s0, b = op.s1;
+  int a = op.s0, b = two ? op.s1 : 16;
+  if (!two) {  // a single factor: the table does not depend on position b
+    p2 = p0;
+    p3 = p1;
+  }
+  // a factor on a qubit outside the tile is a fixed bit here: fold it into
+  // the phase table, leaving a diagonal on the other position (or a scalar,
+  // applied through register slot 0 with equal phases)
+  if (b >= kGlobalBit) {
+    if ((base >> (b - kGlobalBit)) & 1) {
+      p0 = p2;
+      p1 = p3;
+    }
+    b = 16;
+    p2 = p0;
+    p3 = p1;
+  }
+  if (a >= kGlobalBit) {
+    if ((base >> (a - kGlobalBit)) & 1) {
+      p0 = p1;
+      p2 = p3;
+    }
+    a = 16;
+    p1 = p0;
+    p3 = p2;
+  }
   const bool areg = a >= 16, breg = b >= 16;
   const int as = a & 15, bs = b & 15;
   const uint32_t cr = op.cr;
-  if (op.ct == 0 && areg && (!two || breg)) {
+  if (op.ct == 0 && areg && breg) {
     // every factor depends on register bits only: the phase of each element
     // is chosen per register index, uniformly across the wavefront
 #pragma unroll
     for (int r = 0; r < (1 << EB); ++r) {
       if ((r & cr) != cr) continue;
       const bool ba = (r >> as) & 1;
-      const bool bb = two && ((r >> bs) & 1);
+      const bool bb = (r >> bs) & 1;
       T ph;
       ph.x = bb ? (ba ? p3.x : p2.x) : (ba ? p1.x : p0.x);
       ph.y = bb ? (ba ? p3.y : p2.y) : (ba ? p1.y : p0.y);
@@ -546,12 +580,12 @@ __device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, in
     }
     return;
   }
-  const bool ta = (tid >> as) & 1, tb = two && ((tid >> bs) & 1);
+  const bool ta = (tid >> as) & 1, tb = (tid >> bs) & 1;
 #pragma unroll
   for (int r = 0; r < (1 << EB); ++r) {
     if ((r & cr) != cr) continue;
     const bool ba = areg ? ((r >> as) & 1) : ta;
-    const bool bb = two && (breg ? ((r >> bs) & 1) : tb);
+    const bool bb = breg ? ((r >> bs) & 1) : tb;
     T lo, hi, ph;
     lo.x = ba ? p1.x : p0.x;
     lo.y = ba ? p1.y : p0.y;
@@ -632,16 +666,18 @@ __device__ __forceinline__ void reg_pair_op(T (&v)[1 << EB], const RegOp<R>& op,
 // kernel share it).
 template <typename R, typename T, int EB>
 __device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<R>* ops, int nops, int tid,
-                                            bool active) {
+                                            bool active, uint64_t base) {
   for (int o = 0; o < nops; ++o) {
     const RegOp<R>& op = ops[o];
+    // controls outside the tile: uniform over the workgroup (remaps have none)
+    if ((base & op.gc) != op.gc) continue;
     const bool on = (tid & op.ct) == op.ct;
     switch (op.kind) {
       case R_REMAP: reg_remap<R, T, EB>(v, sm, op, tid, active); break;
       case R_U1: NQPU_SLOT_CASES(reg_u1, v, op, on) break;
       case R_FLIP: NQPU_SLOT_CASES(reg_flip, v, op, on) break;
       case R_D1:
-      case R_D2: reg_diag<R, T, EB>(v, op, tid, on); break;
+      case R_D2: reg_diag<R, T, EB>(v, op, tid, on, base); break;
       case R_U2: NQPU_SLOT_CASES(reg_pair_op, v, op, on, true) break;
       case R_SWAP: NQPU_SLOT_CASES(reg_pair_op, v, op, on, false) break;
       default: break;
@@ -673,7 +709,7 @@ __global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArg
         if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
       v[r] = st[d];
     }
-    reg_program<R, T, EB>(v, sm, ops, pa.nops, tid, true);
+    reg_program<R, T, EB>(v, sm, ops, pa.nops, tid, true, base);
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       uint64_t d = base | dlo;
@@ -781,7 +817,7 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
         v[r].x = (tid == 0 && r == 0) ? 1.f : 0.f;
         v[r].y = 0.f;
       }
-      reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, active);
+      reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, active, 0);
       if (active) {
 #pragma unroll
         for (int r = 0; r < E; ++r) sm[tid | (r << P.tb)] = v[r];
@@ -815,7 +851,7 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
               v[r] = st[d];
             }
           }
-          reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true);
+          reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true, base);
 #pragma unroll
           for (int r = 0; r < E; ++r) {
             uint64_t d = base | dlo;

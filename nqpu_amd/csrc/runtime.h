@@ -53,6 +53,9 @@ struct PlanOptions {
   int tile_qubits = 12;
   int low_qubits = 6;
   bool registers = true;  // register-resident tile kernel where it applies
+  // Controls and diagonal factors on qubits outside a tile are resolved per
+  // tile, so only the targets of non-diagonal ops need to be tile bits.
+  bool global_ops = true;
 };
 
 // lowering and planning (pure host code; unit-testable without a device)

@@ -263,3 +263,23 @@ def test_fidelity(oracle, golden):
     assert abs(oracle.fidelity(a, a, 6) - 1) < TOL
     b = a * np.exp(0.3j)
     assert abs(oracle.fidelity(a, b, 6) - 1) < TOL
+
+
+def test_trotter_light_cone(oracle):
+    """QFT|0...0> is |+>^n and each first-order TFIM Trotter step widens the
+    support of Z0 Z1 by one qubit, so after s steps <Z0 Z1> is the same for
+    every n >= s + 2; a dropped or misplaced RZZ/RX breaks the equality.  At
+    s = 20 the 22-qubit value is the reference for the 33/34-qubit runs."""
+    from nqpu_amd.workloads import qft_trotter
+    vals = {}
+    for n in (8, 9, 12):
+        a = oracle.run(n, qft_trotter(n, 6).gates)
+        vals[n] = oracle.expectation_pauli(a, n, 0, 3)
+    assert abs(vals[8] - vals[9]) < 1e-13 and abs(vals[8] - vals[12]) < 1e-13
+    # a chain shorter than the light cone gives a different value (the
+    # boundary's influence shrinks fast with distance, so take a short chain)
+    a = oracle.run(3, qft_trotter(3, 6).gates)
+    assert abs(oracle.expectation_pauli(a, 3, 0, 3) - vals[8]) > 1e-6
+    z22 = oracle.expectation_pauli(oracle.run(22, qft_trotter(22, 20).gates), 22, 0, 3)
+    z23 = oracle.expectation_pauli(oracle.run(23, qft_trotter(23, 20).gates), 23, 0, 3)
+    assert abs(z22 - z23) < 1e-12

@@ -30,6 +30,10 @@ CIRCUITS = {
     "mixed14": lambda: random_circuit(14, 200, 2),
     "brick14": lambda: random_u3_brickwork(14, 12, 3),
     "qft_trotter13": lambda: tfim_trotter(13, 3, circuit=qft(13)),
+    # 4-5 qubits outside every tile: controls and diagonal factors there are
+    # resolved from the tile's base index
+    "mixed16": lambda: random_circuit(16, 160, 9),
+    "qft_trotter15": lambda: tfim_trotter(15, 4, circuit=qft(15)),
 }
 
 
@@ -44,6 +48,29 @@ def test_emulated_tiles_match_oracle(oracle, name, tile, low, lds):
     oracle.apply_circuit(exp, n, c.gates)
     got = c.native().emulate(a0, "c64", tile_qubits=tile, low_qubits=low, lds_tiles=lds)
     assert np.max(np.abs(got - exp)) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["mixed16", "qft_trotter15"])
+def test_emulated_c128_tiles_match_oracle(oracle, name):
+    """complex128 runs the LDS tile kernel with its own op encoding."""
+    c = CIRCUITS[name]()
+    n = c.num_qubits
+    a0 = random_state(n, 11)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    got = c.native().emulate(a0, "c128", tile_qubits=11, low_qubits=3)
+    assert np.max(np.abs(got - exp)) < 1e-12
+
+
+def test_diagonal_gates_do_not_need_tile_bits():
+    """A QFT's controlled phases and a Trotter step's RZZ are diagonal, so a
+    pass needs only the Hadamard/RX targets as tile bits: a 30-qubit QFT fits
+    in ceil((30-12)/7)+1 passes of 12-qubit tiles with 5 fixed low qubits,
+    and 20 Trotter steps cost far fewer than one pass per step and tile."""
+    st = qft(30).native().plan("c64", tile_qubits=12, low_qubits=5)
+    assert st["passes"] <= 6
+    st = tfim_trotter(30, 20).native().plan("c64", tile_qubits=12, low_qubits=5)
+    assert st["passes"] <= 40
 
 
 def test_emulated_reference_semantics(oracle):
