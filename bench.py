@@ -160,8 +160,9 @@ def bench_qft_trotter(args):
     followed by 20 first-order Trotter steps (dt = 0.05, g = 1) of the 1D
     transverse-field Ising chain, then a 33-qubit complex128 state (128 GiB)
     runs the same circuit.  One step = reset to |0...0>, the whole circuit
-    and (complex64) the FP64 renormalisation of the result.  <Z0 Z1> after this circuit depends only on qubits 0..21 (each
-    Trotter step widens the light cone of Z0 Z1 by one qubit and the QFT of
+    and (complex64) the FP64 renormalisation of the result.  <Z0 Z1> after
+    this circuit depends only on qubits 0..21 (each Trotter step widens the
+    light cone of Z0 Z1 by one qubit and the QFT of
     |0...0> is a product state), so the FP64 oracle run at 22 qubits gives
     the exact value for every size."""
     from nqpu_amd import engine as E
@@ -259,12 +260,138 @@ def bench_qft_trotter(args):
     print(json.dumps(result), flush=True)
 
 
+def _percentile(xs, p):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), p)) if len(xs) else None
+
+
+def bench_serve(args):
+    """Served stream: Poisson arrivals at --qps of requests with n from
+    {10,14,18,22,26,28} (weights 30/25/20/12/8/5%), random U3+CNOT depth
+    U{10..100}, 1024 shots and 8 random Pauli strings each (seed 104).  The
+    request server admits them by slot and HBM arena and runs them
+    concurrently; latency is arrival -> result.  Compared with plain
+    per-circuit launches of the same requests, one after another, whose
+    results must be bit-identical."""
+    import threading
+
+    import numpy as np
+
+    from nqpu_amd import engine as E
+    from nqpu_amd.workloads import served_stream
+
+    if E.device_count() == 0:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    count = args.requests
+    reqs = served_stream(count, args.qps, seed=104)
+    natives = [r.circuit.native() for r in reqs]
+    gates = sum(len(r.circuit.gates) for r in reqs)
+    log(f"{count} requests, {gates} gates, arrivals over {reqs[-1].arrival:.1f} s at {args.qps}/s")
+    srv = E.Server(slots=args.slots, precision="c64", memory_bytes=64 << 30)
+    for r, nc in list(zip(reqs, natives))[: args.warmup * 4]:
+        srv.result(srv.submit(nc, r.terms, r.shots, r.seed))
+    log("warmup done")
+    ids = [None] * count
+    lag = [0.0] * count
+
+    def submitter(t0):
+        for k, (r, nc) in enumerate(zip(reqs, natives)):
+            now = time.perf_counter() - t0
+            if r.arrival > now:
+                time.sleep(r.arrival - now)
+            lag[k] = max(0.0, time.perf_counter() - t0 - r.arrival)
+            ids[k] = srv.submit(nc, r.terms, r.shots, r.seed)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter, args=(t0,))
+    th.start()
+    results = []
+    for k in range(count):
+        while ids[k] is None:
+            time.sleep(0.0005)
+        results.append(srv.result(ids[k], timeout=600))
+        if (k + 1) % 100 == 0:
+            log(f"{k + 1}/{count} served")
+    th.join()
+    elapsed = time.perf_counter() - t0
+    lat = [1e3 * lag[k] + results[k]["latency_ms"] for k in range(count)]
+    st = srv.stats()
+    # saturation: the same requests all at once
+    t1 = time.perf_counter()
+    burst = [srv.submit(nc, r.terms, r.shots, r.seed) for r, nc in zip(reqs, natives)]
+    for i in burst:
+        srv.result(i, timeout=600)
+    sat = count / (time.perf_counter() - t1)
+    srv.close()
+    log(f"served: p50 {_percentile(lat, 50):.1f} ms, p99 {_percentile(lat, 99):.1f} ms; burst {sat:.1f}/s")
+    # plain per-circuit launches, one after another (own state, run, 8 Pauli strings, shots)
+    service = []
+    diff = 0.0
+    same = True
+    for k, (r, nc) in enumerate(zip(reqs, natives)):
+        ta = time.perf_counter()
+        with E.StateVector(r.circuit.num_qubits, "c64") as sv:
+            sv.run(nc)
+            ex = np.array([c * sv.expectation_pauli(p) for c, p in r.terms])
+            smp = sv.sample(r.shots, seed=r.seed)
+        service.append(time.perf_counter() - ta)
+        diff = max(diff, float(np.max(np.abs(ex - results[k]["expectations"]))))
+        same = same and bool(np.array_equal(smp, results[k]["samples"]))
+    seq_rate = count / sum(service)
+    fin, seq_lat = 0.0, []
+    for r, s in zip(reqs, service):  # FIFO service of the same arrivals, one request at a time
+        fin = max(fin, r.arrival) + s
+        seq_lat.append(1e3 * (fin - r.arrival))
+    result = {
+        "metric": f"served circuits/s at {args.qps:g} req/s offered (Poisson), p50/p99 latency",
+        "value": count / elapsed, "unit": "circuits/s", "n_gpus": 1, "steps": count, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / count, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64", "data": "synthetic",
+        "config": {"workload": "served stream: Poisson arrivals, n in {10,14,18,22,26,28} (30/25/20/12/8/5%), "
+                               "random U3+CNOT depth U{10..100}, 1024 shots + 8 random Pauli strings, seed 104",
+                   "requests": count, "offered_qps": args.qps, "slots": args.slots, "gates": gates},
+        "latency_ms": {"p50": _percentile(lat, 50), "p99": _percentile(lat, 99), "max": max(lat)},
+        "saturation_circuits_per_s": sat,
+        "server_stats": st,
+        "per_circuit_launches": {"circuits_per_s": seq_rate,
+                                 "fifo_latency_ms_p50": _percentile(seq_lat, 50),
+                                 "fifo_latency_ms_p99": _percentile(seq_lat, 99)},
+        "speedup_vs_per_circuit": sat / seq_rate,
+        "served_vs_alone_max_abs_diff": diff, "served_vs_alone_samples_identical": same,
+    }
+    if not args.no_cpu:
+        log("cpu baseline")
+        from oracle import cpu_ref
+        t3 = time.perf_counter()
+        done = 0
+        for r in reqs:
+            n = r.circuit.num_qubits
+            a = cpu_ref.run(n, r.circuit.gates)
+            for c, p in r.terms:
+                cpu_ref.expectation_pauli(a, n, *E.pauli_masks(p))
+            cpu_ref.sample_cabi(a, n, r.shots, seed=r.seed)
+            done += 1
+            if time.perf_counter() - t3 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t3
+        result["cpu_baseline"] = {"value": done / dt, "unit": "circuits/s", "cores": cpu_ref.num_threads(),
+                                  "kind": "port",
+                                  "sample": f"first {done} requests of the same stream served one after another, "
+                                            f"complex128 ({dt:.1f} s)"}
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="random30", choices=["random30", "batch", "qft34"],
+    ap.add_argument("--workload", default="random30", choices=["random30", "batch", "qft34", "serve"],
                     help="random30: 30-qubit random circuit (default); batch: 4096 variational circuits; "
-                         "qft34: 34-qubit complex64 QFT + Trotter (128 GiB) with a 33-qubit complex128 check")
+                         "qft34: 34-qubit complex64 QFT + Trotter (128 GiB) with a 33-qubit complex128 check; "
+                         "serve: Poisson request stream through the request server")
+    ap.add_argument("--qps", type=float, default=40.0, help="serve: offered requests per second")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
+    ap.add_argument("--slots", type=int, default=8, help="serve: concurrent requests on the GPU")
     ap.add_argument("--circuits", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -288,6 +415,8 @@ def main():
         return bench_batch(args)
     if args.workload == "qft34":
         return bench_qft_trotter(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     from nqpu_amd import engine as E
     from nqpu_amd.workloads import random_u3_brickwork

@@ -329,6 +329,67 @@ const char* nqpu_last_error_detail(void);
 /* Number of usable devices (0 when none or when NQPU_HIP_FORCE_NO_DEVICE=1). */
 uint32_t nqpu_device_count(void);
 
+/* ---- Request server (continuous batching) ----
+ * Replaces the one-request-at-a-time serving path of ExecuteResponse::execute
+ * and SampleResponse::sample (sdk/rust/src/web/mod.rs:288-419), which
+ * re-simulates the circuit for every shot.  Each request is a circuit run from
+ * |0...0>, a list of weighted Pauli strings and a number of shots.
+ * Requests are admitted in arrival order as soon as a slot and room in the
+ * server's HBM arena are free.  Admitted requests run concurrently, one per
+ * slot, each on its own stream.  A request's results do not depend on which
+ * other requests share the device: its shots come from its own seeded
+ * xorshift64* stream (seed 0 picks the C ABI's default state) and its
+ * reductions are summed in a fixed order.
+ * Every call may be made from any thread. */
+typedef struct NqpuServer NqpuServer;
+
+typedef struct {
+  uint32_t slots;          /* requests on the device at once (0 = 8, at most 64) */
+  uint32_t precision;      /* NqpuPrecision of every request's state (0 = complex64) */
+  uint64_t memory_bytes;   /* HBM reserved for request states (0 = half the free memory) */
+  NqpuRunOptions run;      /* how each circuit runs (all zero = defaults: tiled, standard) */
+} NqpuServerConfig;
+
+typedef enum {
+  NQPU_REQUEST_QUEUED = 0,
+  NQPU_REQUEST_RUNNING = 1,
+  NQPU_REQUEST_DONE = 2,
+  NQPU_REQUEST_FAILED = 3
+} NqpuRequestState;
+
+typedef struct {
+  uint32_t state;      /* NqpuRequestState */
+  int32_t error;       /* NqpuError of a failed request */
+  double queued_ms;    /* submission to admission (so far, while queued) */
+  double run_ms;       /* admission to completion (so far, while running) */
+  double latency_ms;   /* submission to completion (so far, while unfinished) */
+} NqpuRequestStatus;
+
+typedef struct {
+  uint64_t submitted, completed, failed;
+  uint64_t peak_running;   /* most requests admitted at once */
+  uint64_t peak_bytes;     /* most arena bytes held by admitted requests at once */
+  uint64_t memory_bytes;   /* size of the arena */
+} NqpuServerStats;
+
+/* NULL on failure (nqpu_last_error_detail says why). */
+NqpuServer* nqpu_server_create(const NqpuServerConfig* config);
+/* Finishes the running requests, fails the queued ones, frees the arena. */
+void nqpu_server_destroy(NqpuServer* server);
+/* Queue a request; the circuit and terms are copied.  A state larger than the
+ * whole arena is refused with NQPU_ERROR_OUT_OF_DEVICE_MEMORY. */
+NqpuError nqpu_server_submit(NqpuServer* server, const NqpuCircuit* circuit, const NqpuPauliTerm* terms,
+                             uint32_t num_terms, uint32_t shots, uint64_t seed, uint64_t* request_id);
+NqpuError nqpu_server_poll(NqpuServer* server, uint64_t request_id, NqpuRequestStatus* out);
+/* Block until the request finishes (timeout_ms = UINT32_MAX: no limit);
+ * NQPU_ERROR_SIMULATION_FAILED when the timeout passes first. */
+NqpuError nqpu_server_wait(NqpuServer* server, uint64_t request_id, uint32_t timeout_ms, NqpuRequestStatus* out);
+/* Copy a finished request's results (coeff * <P_k> per term; sampled basis
+ * indices) and forget the request.  A failed request returns its error. */
+NqpuError nqpu_server_take(NqpuServer* server, uint64_t request_id, double* expectations, uint32_t num_terms,
+                           uint64_t* samples, uint32_t shots);
+NqpuError nqpu_server_stats(const NqpuServer* server, NqpuServerStats* out);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -79,6 +79,24 @@ class NqpuFusionCounts(ct.Structure):
                 ("gates_eliminated", ct.c_uint64), ("items", ct.c_uint64)]
 
 
+class NqpuServerConfig(ct.Structure):
+    _fields_ = [("slots", ct.c_uint32), ("precision", ct.c_uint32), ("memory_bytes", ct.c_uint64),
+                ("run", NqpuRunOptions)]
+
+
+class NqpuRequestStatus(ct.Structure):
+    _fields_ = [("state", ct.c_uint32), ("error", ct.c_int32), ("queued_ms", ct.c_double),
+                ("run_ms", ct.c_double), ("latency_ms", ct.c_double)]
+
+
+class NqpuServerStats(ct.Structure):
+    _fields_ = [("submitted", ct.c_uint64), ("completed", ct.c_uint64), ("failed", ct.c_uint64),
+                ("peak_running", ct.c_uint64), ("peak_bytes", ct.c_uint64), ("memory_bytes", ct.c_uint64)]
+
+
+REQUEST_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+WAIT_FOREVER = 0xFFFFFFFF
+
 # name -> (restype, argtypes)
 _P = ct.c_void_p
 _E = ct.c_int
@@ -148,6 +166,13 @@ SIGNATURES = {
     "nqpu_batch_last_stats": (_E, [_P, ct.POINTER(NqpuBatchStats)]),
     "nqpu_debug_batch_plan": (_E, [_P, ct.c_uint64, _P, _P, ct.POINTER(NqpuRunOptions),
                                    ct.POINTER(NqpuBatchStats), ct.POINTER(ct.c_uint64)]),
+    "nqpu_server_create": (_P, [ct.POINTER(NqpuServerConfig)]),
+    "nqpu_server_destroy": (None, [_P]),
+    "nqpu_server_submit": (_E, [_P, _P, _P, ct.c_uint32, ct.c_uint32, ct.c_uint64, ct.POINTER(ct.c_uint64)]),
+    "nqpu_server_poll": (_E, [_P, ct.c_uint64, ct.POINTER(NqpuRequestStatus)]),
+    "nqpu_server_wait": (_E, [_P, ct.c_uint64, ct.c_uint32, ct.POINTER(NqpuRequestStatus)]),
+    "nqpu_server_take": (_E, [_P, ct.c_uint64, _P, ct.c_uint32, _P, ct.c_uint32]),
+    "nqpu_server_stats": (_E, [_P, ct.POINTER(NqpuServerStats)]),
 }
 
 _lib = None

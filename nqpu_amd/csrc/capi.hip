@@ -567,4 +567,53 @@ NqpuError nqpu_batch_last_stats(const NqpuBatch* b, NqpuBatchStats* out) {
 
 uint32_t nqpu_device_count(void) { return (uint32_t)device_count(); }
 
+NqpuServer* nqpu_server_create(const NqpuServerConfig* cfg) {
+  NqpuServerConfig c{};
+  if (cfg) c = *cfg;
+  Server* s = nullptr;
+  if (create_server(c, &s) != NQPU_OK) return nullptr;
+  return reinterpret_cast<NqpuServer*>(s);
+}
+
+void nqpu_server_destroy(NqpuServer* s) {
+  if (s) destroy_server(reinterpret_cast<Server*>(s));
+}
+
+NqpuError nqpu_server_submit(NqpuServer* s, const NqpuCircuit* c, const NqpuPauliTerm* terms, uint32_t num_terms,
+                             uint32_t shots, uint64_t seed, uint64_t* id) {
+  if (!s || !c || !id) return NQPU_NULL_POINTER;
+  if (num_terms && !terms) return NQPU_NULL_POINTER;
+  const Circuit* cc = Cc(c);
+  const uint64_t lim = cc->n >= 64 ? ~0ull : ((1ull << cc->n) - 1);
+  std::vector<PauliTerm> ts;
+  ts.reserve(num_terms);
+  for (uint32_t k = 0; k < num_terms; ++k) {
+    if (!std::isfinite(terms[k].coeff)) return NQPU_INVALID_PARAMETER;
+    if ((terms[k].xmask | terms[k].zmask) & ~lim) return NQPU_INVALID_QUBIT;
+    ts.push_back(PauliTerm{terms[k].xmask, terms[k].zmask, terms[k].coeff});
+  }
+  return server_submit(reinterpret_cast<Server*>(s), *cc, ts, shots, seed, id);
+}
+
+NqpuError nqpu_server_poll(NqpuServer* s, uint64_t id, NqpuRequestStatus* out) {
+  if (!s || !out) return NQPU_NULL_POINTER;
+  return server_poll(reinterpret_cast<Server*>(s), id, out);
+}
+
+NqpuError nqpu_server_wait(NqpuServer* s, uint64_t id, uint32_t timeout_ms, NqpuRequestStatus* out) {
+  if (!s) return NQPU_NULL_POINTER;
+  return server_wait(reinterpret_cast<Server*>(s), id, timeout_ms, out);
+}
+
+NqpuError nqpu_server_take(NqpuServer* s, uint64_t id, double* expectations, uint32_t num_terms, uint64_t* samples,
+                           uint32_t shots) {
+  if (!s) return NQPU_NULL_POINTER;
+  return server_take(reinterpret_cast<Server*>(s), id, expectations, num_terms, samples, shots);
+}
+
+NqpuError nqpu_server_stats(const NqpuServer* s, NqpuServerStats* out) {
+  if (!s || !out) return NQPU_NULL_POINTER;
+  return server_stats(reinterpret_cast<const Server*>(s), out);
+}
+
 }  // extern "C"

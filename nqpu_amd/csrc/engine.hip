@@ -23,11 +23,18 @@ NqpuError hip_check(hipError_t e, const char* what) {
   set_error(std::string(what) + ": " + hipGetErrorString(e));
   return e == hipErrorOutOfMemory ? NQPU_OUT_OF_DEVICE_MEMORY : NQPU_DEVICE_ERROR;
 }
-#define HIPTRY(expr, what)                              \
-  do {                                                  \
-    NqpuError e_ = hip_check((expr), what);             \
-    if (e_ != NQPU_OK) return e_;                       \
-  } while (0)
+// A copy ordered on the caller's stream and waited for.  Plain hipMemcpy runs
+// on the null stream, which would serialise against every other state's
+// stream in the process (concurrent requests of a server, for one).
+static hipError_t copy_on(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
+// Every stream the library creates: default priority, and not synchronised
+// with the null stream, so work of different handles can overlap.
+hipError_t new_stream(hipStream_t* st) { return hipStreamCreateWithFlags(st, hipStreamNonBlocking); }
 
 // ------------------------------------------------------------------- masks
 
@@ -566,7 +573,7 @@ bool DeviceScratch::ensure(size_t b) {
 
 State::~State() {
   if (stream) (void)hipStreamSynchronize(stream);
-  if (amps) (void)hipFree(amps);
+  if (amps && owns_amps) (void)hipFree(amps);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -594,7 +601,7 @@ NqpuError create_state(int n, int prec, State** out) {
     set_error("state of " + std::to_string(bytes) + " bytes does not fit in free device memory");
     return NQPU_OUT_OF_DEVICE_MEMORY;
   }
-  HIPTRY(hipStreamCreate(&s->stream), "hipStreamCreate");
+  HIPTRY(new_stream(&s->stream), "hipStreamCreate");
   HIPTRY(hipMalloc(&s->amps, bytes), "hipMalloc(state)");
   NqpuError e = reset_state(s.get());
   if (e != NQPU_OK) return e;
@@ -619,7 +626,7 @@ NqpuError clone_state(const State* s, State** out) {
   c->prec = s->prec;
   c->device = s->device;
   size_t bytes = s->amp_size() << s->n;
-  HIPTRY(hipStreamCreate(&c->stream), "hipStreamCreate");
+  HIPTRY(new_stream(&c->stream), "hipStreamCreate");
   HIPTRY(hipMalloc(&c->amps, bytes), "hipMalloc(clone)");
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
   HIPTRY(hipMemcpyAsync(c->amps, s->amps, bytes, hipMemcpyDeviceToDevice, c->stream), "clone copy");
@@ -657,7 +664,7 @@ static NqpuError exec_direct(State* s, const HostOp& o) {
     size_t lb = o.lut.size() * sizeof(double) * 2;
     HIPTRY(hipStreamSynchronize(q), "sync before LUT upload");
     if (!s->scratch2.ensure(lb)) return NQPU_OUT_OF_DEVICE_MEMORY;
-    HIPTRY(hipMemcpy(s->scratch2.p, o.lut.data(), lb, hipMemcpyHostToDevice), "LUT upload");
+    HIPTRY(copy_on(s->stream, s->scratch2.p, o.lut.data(), lb, hipMemcpyHostToDevice), "LUT upload");
   }
   s->timer.mark(q);
   switch (o.kind) {
@@ -1162,7 +1169,7 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
   HIPTRY(hipStreamSynchronize(s->stream), "sync before program upload");
   if (!s->opbuf.ensure(ob)) return NQPU_OUT_OF_DEVICE_MEMORY;
   if (!prog.empty())
-    HIPTRY(hipMemcpy(s->opbuf.p, prog.data(), prog.size() * sizeof(RegOp<R>), hipMemcpyHostToDevice),
+    HIPTRY(copy_on(s->stream, s->opbuf.p, prog.data(), prog.size() * sizeof(RegOp<R>), hipMemcpyHostToDevice),
            "program upload");
   for (const PassArgs& pa : args) {
     s->timer.mark(s->stream);
@@ -1199,7 +1206,7 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
   HIPTRY(hipStreamSynchronize(s->stream), "sync before op upload");
   if (!s->opbuf.ensure(ob)) return NQPU_OUT_OF_DEVICE_MEMORY;
   if (!dops.empty())
-    HIPTRY(hipMemcpy(s->opbuf.p, dops.data(), dops.size() * sizeof(DevOp<R>), hipMemcpyHostToDevice), "op upload");
+    HIPTRY(copy_on(s->stream, s->opbuf.p, dops.data(), dops.size() * sizeof(DevOp<R>), hipMemcpyHostToDevice), "op upload");
   for (const PassArgs& pa : args) {
     s->timer.mark(s->stream);
     Launch<R>::tile_pass(s->amps, pa, s->opbuf.p, s->stream);
@@ -1351,7 +1358,7 @@ NqpuError overlap(const State* a0, const State* b0, NqpuComplex* out) {
   HIPTRY(hipGetLastError(), "overlap");
   HIPTRY(hipStreamSynchronize(a->stream), "overlap");
   std::vector<double> rows(nch * 2);
-  HIPTRY(hipMemcpy(rows.data(), a->scratch.p, nch * 16, hipMemcpyDeviceToHost), "overlap copy");
+  HIPTRY(copy_on(a->stream, rows.data(), a->scratch.p, nch * 16, hipMemcpyDeviceToHost), "overlap copy");
   double re = 0, im = 0;
   for (uint64_t c = 0; c < nch; ++c) {
     re += rows[2 * c];
@@ -1382,7 +1389,7 @@ NqpuError get_amplitudes(const State* s0, uint64_t off, uint64_t count, NqpuComp
   State* s = const_cast<State*>(s0);
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
   if (s->prec == NQPU_PRECISION_C128) {
-    HIPTRY(hipMemcpy(out, (char*)s->amps + off * 16, count * 16, hipMemcpyDeviceToHost), "amplitude copy");
+    HIPTRY(copy_on(s->stream, out, (char*)s->amps + off * 16, count * 16, hipMemcpyDeviceToHost), "amplitude copy");
     return NQPU_OK;
   }
   for (uint64_t b = 0; b < count; b += kXferChunk) {
@@ -1391,7 +1398,7 @@ NqpuError get_amplitudes(const State* s0, uint64_t off, uint64_t count, NqpuComp
     Launch<float>::to_c128(s->amps, c, off + b, s->scratch.p, s->stream);
     HIPTRY(hipGetLastError(), "to_c128");
     HIPTRY(hipStreamSynchronize(s->stream), "to_c128");
-    HIPTRY(hipMemcpy(out + b, s->scratch.p, c * 16, hipMemcpyDeviceToHost), "amplitude copy");
+    HIPTRY(copy_on(s->stream, out + b, s->scratch.p, c * 16, hipMemcpyDeviceToHost), "amplitude copy");
   }
   return NQPU_OK;
 }
@@ -1399,13 +1406,13 @@ NqpuError get_amplitudes(const State* s0, uint64_t off, uint64_t count, NqpuComp
 NqpuError set_amplitudes(State* s, uint64_t off, uint64_t count, const NqpuComplex* in) {
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
   if (s->prec == NQPU_PRECISION_C128) {
-    HIPTRY(hipMemcpy((char*)s->amps + off * 16, in, count * 16, hipMemcpyHostToDevice), "amplitude copy");
+    HIPTRY(copy_on(s->stream, (char*)s->amps + off * 16, in, count * 16, hipMemcpyHostToDevice), "amplitude copy");
     return NQPU_OK;
   }
   for (uint64_t b = 0; b < count; b += kXferChunk) {
     uint64_t c = std::min(kXferChunk, count - b);
     if (!s->scratch.ensure(c * 16)) return NQPU_OUT_OF_DEVICE_MEMORY;
-    HIPTRY(hipMemcpy(s->scratch.p, in + b, c * 16, hipMemcpyHostToDevice), "amplitude copy");
+    HIPTRY(copy_on(s->stream, s->scratch.p, in + b, c * 16, hipMemcpyHostToDevice), "amplitude copy");
     Launch<float>::from_c128(s->amps, c, off + b, s->scratch.p, s->stream);
     HIPTRY(hipGetLastError(), "from_c128");
     HIPTRY(hipStreamSynchronize(s->stream), "from_c128");
@@ -1429,7 +1436,7 @@ NqpuError probabilities(const State* s0, double* out) {
     });
     HIPTRY(hipGetLastError(), "probs");
     HIPTRY(hipStreamSynchronize(s->stream), "probs");
-    HIPTRY(hipMemcpy(out, s->scratch.p, dim * 8, hipMemcpyDeviceToHost), "probs copy");
+    HIPTRY(copy_on(s->stream, out, s->scratch.p, dim * 8, hipMemcpyDeviceToHost), "probs copy");
     return NQPU_OK;
   }
   // large states: convert slices to c128 on the host side of the copy
@@ -1455,7 +1462,7 @@ static NqpuError chunk_sums(const State* s0, std::vector<double>* sums, int* cb_
   HIPTRY(hipGetLastError(), "sumsq");
   HIPTRY(hipStreamSynchronize(s->stream), "sumsq");
   sums->resize(nch);
-  HIPTRY(hipMemcpy(sums->data(), s->scratch.p, nch * 8, hipMemcpyDeviceToHost), "sumsq copy");
+  HIPTRY(copy_on(s->stream, sums->data(), s->scratch.p, nch * 8, hipMemcpyDeviceToHost), "sumsq copy");
   *cb_out = cb;
   return NQPU_OK;
 }
@@ -1500,7 +1507,7 @@ NqpuError zsum_all(const State* s0, double* tot, std::vector<double>* s1) {
   HIPTRY(hipGetLastError(), "zsum");
   HIPTRY(hipStreamSynchronize(s->stream), "zsum");
   std::vector<double> rows(nch * w);
-  HIPTRY(hipMemcpy(rows.data(), s->scratch.p, nch * w * 8, hipMemcpyDeviceToHost), "zsum copy");
+  HIPTRY(copy_on(s->stream, rows.data(), s->scratch.p, nch * w * 8, hipMemcpyDeviceToHost), "zsum copy");
   double t = 0;
   for (uint64_t c = 0; c < nch; ++c) {
     const double* r = &rows[c * w];
@@ -1527,7 +1534,7 @@ NqpuError pauli(const State* s0, uint64_t x, uint64_t z, double* out) {
   HIPTRY(hipGetLastError(), "pauli");
   HIPTRY(hipStreamSynchronize(s->stream), "pauli");
   std::vector<double> rows(nch * 2);
-  HIPTRY(hipMemcpy(rows.data(), s->scratch.p, nch * 16, hipMemcpyDeviceToHost), "pauli copy");
+  HIPTRY(copy_on(s->stream, rows.data(), s->scratch.p, nch * 16, hipMemcpyDeviceToHost), "pauli copy");
   double sr = 0, si = 0;
   for (uint64_t c = 0; c < nch; ++c) {
     sr += rows[2 * c];
@@ -1577,8 +1584,8 @@ NqpuError sample(const State* s0, const std::vector<double>& r, uint64_t* out) {
   double* dsub = dpre + (nch + 1);
   double* dr = dsub + nsub;
   uint64_t* dout = (uint64_t*)(dr + shots);
-  HIPTRY(hipMemcpy(dpre, pre.data(), (nch + 1) * 8, hipMemcpyHostToDevice), "prefix upload");
-  HIPTRY(hipMemcpy(dr, r.data(), shots * 8, hipMemcpyHostToDevice), "uniforms upload");
+  HIPTRY(copy_on(s->stream, dpre, pre.data(), (nch + 1) * 8, hipMemcpyHostToDevice), "prefix upload");
+  HIPTRY(copy_on(s->stream, dr, r.data(), shots * 8, hipMemcpyHostToDevice), "uniforms upload");
   dispatch(s, [&](auto* tag) {
     using R = std::remove_pointer_t<decltype(tag)>;
     Launch<R>::sub_sums(s->amps, s->n, sb, dsub, s->stream);
@@ -1586,7 +1593,7 @@ NqpuError sample(const State* s0, const std::vector<double>& r, uint64_t* out) {
   });
   HIPTRY(hipGetLastError(), "sample");
   HIPTRY(hipStreamSynchronize(s->stream), "sample");
-  HIPTRY(hipMemcpy(out, dout, shots * 8, hipMemcpyDeviceToHost), "sample copy");
+  HIPTRY(copy_on(s->stream, out, dout, shots * 8, hipMemcpyDeviceToHost), "sample copy");
   return NQPU_OK;
 }
 
@@ -1629,7 +1636,7 @@ NqpuError create_batch(Batch** out) {
   }
   std::unique_ptr<Batch> b(new Batch());
   HIPTRY(hipSetDevice(0), "hipSetDevice");
-  HIPTRY(hipStreamCreate(&b->stream), "hipStreamCreate");
+  HIPTRY(new_stream(&b->stream), "hipStreamCreate");
   *out = b.release();
   return NQPU_OK;
 }
@@ -1823,7 +1830,7 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
     set_error("batch states do not fit in device memory");
     return NQPU_OUT_OF_DEVICE_MEMORY;
   }
-  HIPTRY(hipMemcpy(b->tables.p, plan.host.data(), total, hipMemcpyHostToDevice), "batch table upload");
+  HIPTRY(copy_on(b->stream, b->tables.p, plan.host.data(), total, hipMemcpyHostToDevice), "batch table upload");
   char* base = (char*)b->tables.p;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
@@ -1847,7 +1854,7 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   if (le != NQPU_OK) return le;
   if (se != NQPU_OK) return se;
   if (!circuits.empty())
-    HIPTRY(hipMemcpy(out, b->out.p, circuits.size() * 8, hipMemcpyDeviceToHost), "batch results");
+    HIPTRY(copy_on(b->stream, out, b->out.p, circuits.size() * 8, hipMemcpyDeviceToHost), "batch results");
   st.circuits = circuits.size();
   st.plan_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   st.kernel_ms = ms;
@@ -1858,15 +1865,18 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
 
 // ------------------------------------------------------------------- RNG
 
-static thread_local uint64_t g_rng = 0x853c49e6748fea9bULL;
-void rng_seed(uint64_t seed) { g_rng = seed == 0 ? 0x853c49e6748fea9bULL : seed; }
-double rng_next() {
-  uint64_t x = g_rng;
+uint64_t rng_init(uint64_t seed) { return seed == 0 ? 0x853c49e6748fea9bULL : seed; }
+double rng_step(uint64_t& state) {
+  uint64_t x = state;
   x ^= x >> 12;
   x ^= x << 25;
   x ^= x >> 27;
-  g_rng = x;
+  state = x;
   return (double)((x * 0x2545F4914F6CDD1DULL) >> 11) / (double)(1ull << 53);
 }
+
+static thread_local uint64_t g_rng = 0x853c49e6748fea9bULL;
+void rng_seed(uint64_t seed) { g_rng = rng_init(seed); }
+double rng_next() { return rng_step(g_rng); }
 
 }  // namespace nqpu

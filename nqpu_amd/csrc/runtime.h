@@ -93,6 +93,7 @@ struct State {
   int n = 0;
   int prec = NQPU_PRECISION_C128;
   void* amps = nullptr;
+  bool owns_amps = true;  // false: amps lives in memory the owner of the State manages (a server's arena)
   hipStream_t stream = nullptr;
   int device = 0;
   NqpuRunStats stats{};
@@ -107,6 +108,11 @@ struct State {
 void set_error(const std::string& s);
 const char* last_error();
 NqpuError hip_check(hipError_t e, const char* what);
+#define HIPTRY(expr, what)                              \
+  do {                                                  \
+    NqpuError e_ = ::nqpu::hip_check((expr), what);     \
+    if (e_ != NQPU_OK) return e_;                       \
+  } while (0)
 
 int device_count();
 NqpuError create_state(int n, int prec, State** out);
@@ -152,7 +158,27 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
                             double* out);
 
 // The C ABI's thread-local xorshift64* stream (c_ffi.rs:48-67).
+// The C ABI's xorshift64* generator (sdk/rust/src/infra/bindings/c_ffi.rs:48-67):
+// a thread-local stream for the nqpu_* calls and explicit streams for callers
+// that own one (a server request).
 void rng_seed(uint64_t seed);
 double rng_next();
+uint64_t rng_init(uint64_t seed);
+double rng_step(uint64_t& state);
+
+// A stream of default priority that does not synchronise with the null stream.
+hipError_t new_stream(hipStream_t* st);
+
+// Continuous-batching request server (server.hip).
+struct Server;
+NqpuError create_server(const NqpuServerConfig& cfg, Server** out);
+void destroy_server(Server* s);
+NqpuError server_submit(Server* s, const Circuit& c, const std::vector<PauliTerm>& terms, uint32_t shots,
+                        uint64_t seed, uint64_t* id);
+NqpuError server_poll(Server* s, uint64_t id, NqpuRequestStatus* out);
+NqpuError server_wait(Server* s, uint64_t id, uint32_t timeout_ms, NqpuRequestStatus* out);
+NqpuError server_take(Server* s, uint64_t id, double* expectations, uint32_t num_terms, uint64_t* samples,
+                      uint32_t shots);
+NqpuError server_stats(const Server* s, NqpuServerStats* out);
 
 }  // namespace nqpu

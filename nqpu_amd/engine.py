@@ -403,6 +403,94 @@ class Batch:
             pass
 
 
+class Server:
+    """Continuous-batching request server on the GPU (nqpu_server_*): each
+    request runs a circuit from |0...0>, returns coeff * <P> for its Pauli
+    terms and `shots` sampled basis indices.  Requests are admitted in
+    arrival order as slots and arena memory free up, and run concurrently;
+    a request's results are the same whatever else is running (serving path
+    replaced: sdk/rust/src/web/mod.rs:288-419)."""
+
+    def __init__(self, slots=8, precision="c64", memory_bytes=0, fusion="tiled", semantics="standard",
+                 tile_qubits=0, low_qubits=0):
+        lib = L.load()
+        cfg = L.NqpuServerConfig()
+        cfg.slots = slots
+        cfg.precision = _prec(precision)
+        cfg.memory_bytes = int(memory_bytes)
+        cfg.run = run_options(fusion, semantics, tile_qubits, low_qubits)
+        self.ptr = lib.nqpu_server_create(ct.byref(cfg))
+        if not self.ptr:
+            detail = (lib.nqpu_last_error_detail() or b"").decode()
+            raise L.NqpuError(9 if "no usable" in detail else 8, "nqpu_server_create")
+        self._shape = {}
+
+    def submit(self, circuit, terms=(), shots=0, seed=0) -> int:
+        nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
+        ts = _terms(terms)
+        tarr = (L.NqpuPauliTerm * max(1, len(ts)))()
+        for i, (x, z, c) in enumerate(ts):
+            tarr[i].xmask, tarr[i].zmask, tarr[i].coeff = x, z, c
+        rid = ct.c_uint64()
+        L.check(L.load().nqpu_server_submit(self.ptr, nc.ptr, tarr, len(ts), int(shots), int(seed),
+                                            ct.byref(rid)), "nqpu_server_submit")
+        self._shape[rid.value] = (len(ts), int(shots))
+        return rid.value
+
+    @staticmethod
+    def _status(st: L.NqpuRequestStatus) -> dict:
+        return {"state": L.REQUEST_STATES[st.state], "error": st.error, "queued_ms": st.queued_ms,
+                "run_ms": st.run_ms, "latency_ms": st.latency_ms}
+
+    def poll(self, rid: int) -> dict:
+        st = L.NqpuRequestStatus()
+        L.check(L.load().nqpu_server_poll(self.ptr, rid, ct.byref(st)), "nqpu_server_poll")
+        return self._status(st)
+
+    def wait(self, rid: int, timeout: float | None = None) -> dict:
+        st = L.NqpuRequestStatus()
+        ms = L.WAIT_FOREVER if timeout is None else max(0, int(timeout * 1000))
+        L.check(L.load().nqpu_server_wait(self.ptr, rid, ms, ct.byref(st)), "nqpu_server_wait")
+        return self._status(st)
+
+    def take(self, rid: int):
+        """(expectations, samples) of a finished request; forgets it."""
+        nterms, shots = self._shape.pop(rid)
+        e = np.zeros(max(1, nterms), dtype=np.float64)
+        s = np.zeros(max(1, shots), dtype=np.uint64)
+        L.check(L.load().nqpu_server_take(self.ptr, rid, e.ctypes.data, nterms, s.ctypes.data, shots),
+                "nqpu_server_take")
+        return e[:nterms], s[:shots]
+
+    def result(self, rid: int, timeout: float | None = None) -> dict:
+        st = self.wait(rid, timeout)
+        e, s = self.take(rid)
+        st.update(expectations=e, samples=s)
+        return st
+
+    def stats(self) -> dict:
+        st = L.NqpuServerStats()
+        L.check(L.load().nqpu_server_stats(self.ptr, ct.byref(st)), "nqpu_server_stats")
+        return {k: getattr(st, k) for k, _ in L.NqpuServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            L.load().nqpu_server_destroy(self.ptr)
+            self.ptr = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def simulate(circuit: Circuit, precision="c128", **run_kw) -> StateVector:
     sv = StateVector(circuit.num_qubits, precision)
     sv.run(circuit, **run_kw)

@@ -124,3 +124,42 @@ def tfim_trotter(n: int, steps: int = 20, dt: float = 0.05, g: float = 1.0,
 def qft_trotter(n: int, steps: int = 20) -> Circuit:
     c = qft(n)
     return tfim_trotter(n, steps, circuit=c)
+
+
+@dataclass
+class ServedRequest:
+    arrival: float          # seconds after the stream starts
+    circuit: Circuit
+    terms: list             # [(1.0, pauli string)]
+    shots: int
+    seed: int               # seed of the request's shot stream
+
+
+SERVED_QUBITS = (10, 14, 18, 22, 26, 28)
+SERVED_WEIGHTS = (0.30, 0.25, 0.20, 0.12, 0.08, 0.05)
+
+
+def served_stream(count: int, qps: float, seed: int = 104, shots: int = 1024, nterms: int = 8,
+                  qubits=SERVED_QUBITS, weights=SERVED_WEIGHTS, dmin: int = 10, dmax: int = 100):
+    """Poisson arrivals at `qps` requests/s.  Per request, in stream order:
+    the exponential inter-arrival gap, the qubit count (drawn with `weights`),
+    the depth ~ U{dmin..dmax} of random U3 + CNOT-brickwork layers (circuit
+    seed from the stream), `nterms` Pauli strings with every qubit ~ U{I,X,Y,Z},
+    and the seed of its shot stream."""
+    rng = SplitMix64(seed)
+    cum = []
+    acc = 0.0
+    for w in weights:
+        acc += w
+        cum.append(acc)
+    t = 0.0
+    out = []
+    for _ in range(count):
+        t += -math.log(1.0 - rng.uniform()) / qps
+        u = rng.uniform() * acc
+        n = qubits[next(i for i, c in enumerate(cum) if u < c or i == len(cum) - 1)]
+        depth = rng.randint(dmin, dmax)
+        circ = random_u3_brickwork(n, depth, rng.next_u64())
+        terms = [(1.0, "".join("IXYZ"[rng.randint(0, 3)] for _q in range(n))) for _k in range(nterms)]
+        out.append(ServedRequest(t, circ, terms, shots, rng.next_u64()))
+    return out

@@ -52,6 +52,9 @@ def test_no_device_is_reported_not_faked():
     code = ("import ctypes as ct; from nqpu_amd import _lib as L; lib=L.load();"
             "p=lib.nqpu_state_create(3); assert not p;"
             "assert b'no usable' in lib.nqpu_last_error_detail();"
+            "assert not lib.nqpu_server_create(None);"
+            "assert b'no usable' in lib.nqpu_last_error_detail();"
+            "assert not lib.nqpu_batch_create();"
             "assert lib.nqpu_device_count()==0; print('ok')")
     env = dict(os.environ, NQPU_HIP_FORCE_NO_DEVICE="1")
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
@@ -79,6 +82,13 @@ def test_invalid_create_and_null_handles(lib):
     assert lib.nqpu_circuit_gate_count(None) == 0
     assert lib.nqpu_circuit_depth(None) == 0
     assert lib.nqpu_expectation_pauli(None, 0, 0, None) == NP
+    rid = ct.c_uint64()
+    assert lib.nqpu_server_submit(None, None, None, 0, 0, 0, ct.byref(rid)) == NP
+    assert lib.nqpu_server_poll(None, 1, None) == NP
+    assert lib.nqpu_server_wait(None, 1, 0, None) == NP
+    assert lib.nqpu_server_take(None, 1, None, 0, None, 0) == NP
+    assert lib.nqpu_server_stats(None, None) == NP
+    lib.nqpu_server_destroy(None)
 
 
 def test_circuit_api(lib):
