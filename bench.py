@@ -176,7 +176,8 @@ def bench_qft_trotter(args):
     circ = qft_trotter(n, 20)
     nc = circ.native()
     gates = len(circ.gates)
-    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds)
+    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds,
+                  mfma=not args.no_mfma)
     log(f"workload n={n} gates={gates} plan={nc.plan('c64', **run_kw)}")
     zz = "ZZ" + "I" * (n - 2)
     sv = E.StateVector(n, "c64")
@@ -404,6 +405,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the complex128 accuracy run")
     ap.add_argument("--no-per-gate", action="store_true")
     ap.add_argument("--lds", action="store_true", help="LDS-resident tile kernel instead of the register one")
+    ap.add_argument("--no-mfma", action="store_true",
+                    help="keep every gate on the vector ALUs (no fused blocks on the matrix cores)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain state copy (k_copy) of known size, for counter calibration")
     args = ap.parse_args()
@@ -428,7 +431,8 @@ def main():
     circ = random_u3_brickwork(n, depth, 102)
     nc = circ.native()
     gates = len(circ.gates)
-    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds)
+    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds,
+                  mfma=not args.no_mfma)
     log(f"workload n={n} depth={depth} gates={gates} plan={nc.plan('c64', **run_kw)}")
 
     sv = E.StateVector(n, "c64")

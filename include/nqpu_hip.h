@@ -213,6 +213,9 @@ typedef struct {
 #define NQPU_RUN_LDS_TILES 2u
 /* nqpu_debug_emulate only: emulate the layout the batched kernel uses */
 #define NQPU_RUN_BATCH_LAYOUT 4u
+/* complex64 register tiles: apply every gate on the vector ALUs instead of
+ * fusing runs of gates on four qubits into dense blocks for the matrix cores */
+#define NQPU_RUN_NO_MFMA 8u
 
 /* Statistics of the last run on a state. */
 typedef struct {
@@ -223,6 +226,8 @@ typedef struct {
   double pass_ms;           /* with NQPU_RUN_TIME_PASSES: summed kernel time of the passes */
   uint64_t timed_passes;    /* passes that pass_ms covers */
   uint64_t remaps;          /* LDS exchanges inside register-resident tile passes */
+  uint64_t mfma_blocks;     /* fused 4-qubit blocks applied on the matrix cores */
+  uint64_t mfma_gates;      /* device ops those blocks absorbed */
 } NqpuRunStats;
 
 /* 1 <= n <= 34 (c64) or 33 (c128), bounded by free device memory. */

@@ -62,8 +62,19 @@ enum RegOpKind : int32_t {
                 //   tile; >= 16: register slot s0-16; else thread slot)
   R_D2 = 4,     // diag(m[0..3]) on positions s0 (index bit 0), s1 (index bit 1)
   R_U2 = 5,     // 4x4 on register slots s0 (index bit 0), s1 (index bit 1)
-  R_SWAP = 6    // exchange register slots s0 and s1 (controlled SWAP)
+  R_SWAP = 6,   // exchange register slots s0 and s1 (controlled SWAP)
+  // Dense 16x16 block on four tile bits, on the matrix cores (complex64, EB = 4
+  // only): block index bits a0, a1, a3 are register slots 0, 1, 2 and a2 is
+  // thread slot 5 (lane bit 5); register slot 3 is not in the block.  The
+  // result leaves a1 on thread slot 5 and a2 on register slot 1.  s0 indexes
+  // the op's operand blob (kMfmaBlobBytes each): per lane l and K-step s, the
+  // fp16 high and low parts of A[l % 32][16 s + 8 (l / 32) + i], i = 0..7,
+  // where A is the 32x32 real form of the block matrix (row 2a'+c', column
+  // 2a+c; c = 0 real, 1 imaginary part).
+  R_MFMA = 7
 };
+
+constexpr int kMfmaBlobBytes = 64 * 64;  // 64 lanes x (2 K-steps x (hi, lo) x 8 fp16)
 
 template <typename R>
 struct RegOp {
@@ -127,7 +138,12 @@ struct PassArgs {
   int32_t nops;
   int32_t op0;
   int32_t q[kMaxTile];
+  // nonzero: the pass holds R_MFMA ops, so amplitudes are kept multiplied by
+  // 2^15 in registers (loaded x 2^15, stored x 2^-15) and their fp16 parts
+  // stay clear of the subnormal range
+  int32_t scaled;
 };
+constexpr float kMfmaScale = 32768.0f;
 
 // ----------------------------------------------------------- kernel launches
 // All launches are asynchronous on `s`; R = float (c64) or double (c128).
@@ -144,7 +160,8 @@ struct Launch {
   static void diagk(void* st, int n, int k, const int32_t* q, const void* dlut, hipStream_t s);
   static void tile_pass(void* st, const PassArgs& pa, const void* dops, hipStream_t s);
   // register-resident tile pass; pa.K = 8 + eb, program = RegOp<R>[pa.op0 ..]
-  static void tile_pass_reg(void* st, const PassArgs& pa, int eb, const void* prog, hipStream_t s);
+  static void tile_pass_reg(void* st, const PassArgs& pa, int eb, const void* prog, const void* blob,
+                            hipStream_t s);
   static void scale(void* st, int n, double f, hipStream_t s);
   // FP64 reductions: per-chunk partials, then deterministic final sums.
   static void sumsq_chunks(const void* st, int n, int chunk_bits, double* out, hipStream_t s);

@@ -302,31 +302,73 @@ std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOp
   const uint64_t all = (n >= 64) ? ~0ull : (bit(n) - 1);
   std::vector<size_t> rem(ops.size());
   for (size_t i = 0; i < ops.size(); ++i) rem[i] = i;
-  while (!rem.empty()) {
-    uint64_t S = (L > 0) ? (bit(L) - 1) : 0, blocked = 0;
-    std::vector<size_t> chosen, next;
-    next.reserve(rem.size());
+  const uint64_t low = (L > 0) ? (bit(L) - 1) : 0;
+  // vector work of an op, in dense-2x2 units: what a pass should absorb most of
+  auto weight = [](const HostOp& o) {
+    switch (o.kind) {
+      case OP_U1: return 1.0;
+      case OP_U2: return 2.0;
+      case OP_DIAGK: return 1.0;
+      case OP_SWAP: return 0.2;
+      default: return 0.4;
+    }
+  };
+  // The ops of `rem` one pass takes: with `grow`, the tile set starts at the
+  // low qubits and takes the targets of ops in circuit order while they fit;
+  // otherwise the tile set is S as given.  An op joins when none of its
+  // qubits is blocked by an earlier op left behind and its targets are tile
+  // bits; an op left behind blocks its qubits.
+  auto take = [&](uint64_t& S, bool grow, std::vector<size_t>* chosen, std::vector<size_t>* next) {
+    uint64_t blocked = 0;
+    double w = 0;
     size_t i = 0;
     for (; i < rem.size(); ++i) {
       const HostOp& o = ops[rem[i]];
       const uint64_t qm = o.qubit_mask(), tm = po.global_ops ? o.tile_mask() : qm;
-      if (qm & blocked) {
-        blocked |= qm;
-        next.push_back(rem[i]);
-      } else if (__builtin_popcountll(S | tm) <= K) {
+      bool join = (qm & blocked) == 0;
+      if (join) {
+        if (grow) join = __builtin_popcountll(S | tm) <= K;
+        else join = (tm & ~S) == 0;
+      }
+      if (join) {
         S |= tm;
-        chosen.push_back(rem[i]);
+        w += weight(o);
+        if (chosen) chosen->push_back(rem[i]);
       } else {
         blocked |= qm;
-        next.push_back(rem[i]);
+        if (next) next->push_back(rem[i]);
       }
       if ((blocked & all) == all) {  // nothing further can join this pass
         ++i;
         break;
       }
     }
-    for (; i < rem.size(); ++i) next.push_back(rem[i]);
-    if (chosen.empty()) {  // op wider than a tile cannot happen (ops touch <= 4 qubits)
+    if (next)
+      for (; i < rem.size(); ++i) next->push_back(rem[i]);
+    return w;
+  };
+  while (!rem.empty()) {
+    // candidates: the greedy set, and the low qubits plus every window of
+    // K - L consecutive qubits (circuits with 1D structure sweep well with these)
+    uint64_t best = low;
+    double bw = take(best, true, nullptr, nullptr);
+    for (int q = 0; q < n && __builtin_popcountll(best) < K; ++q) best |= bit(q);
+    const int w = K - L;
+    if (w > 0 && n > K)
+      for (int s = L; s + w <= n; ++s) {
+        uint64_t S = low | (((bit(w) - 1)) << s);
+        const double cw = take(S, false, nullptr, nullptr);
+        if (cw > bw + 1e-9) {
+          bw = cw;
+          best = S;
+        }
+      }
+    uint64_t S = best;
+    std::vector<size_t> chosen, next;
+    next.reserve(rem.size());
+    take(S, false, &chosen, &next);
+    if (chosen.empty()) {  // cannot happen: the greedy candidate always takes the first op
+      S |= ops[next.front()].tile_mask();
       chosen.push_back(next.front());
       next.erase(next.begin());
     }
@@ -744,8 +786,21 @@ static std::vector<HostOp> diagonal_ops(const FusedItem& it) {
 // can run, an R_REMAP moves the targets of the next pending ops into
 // register slots.  The pass ends in the canonical slot assignment the store
 // expects.
+// A register-tile program: the ops, and for each R_MFMA op (s0 = index) its
+// 16x16 block matrix in block-index order (a0, a1, a2, a3; see R_MFMA).
 template <typename R>
-std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
+struct RegProgram {
+  std::vector<RegOp<R>> ops;
+  std::vector<std::array<cd, 256>> blocks;
+  uint64_t fused_ops = 0;  // device ops inside those blocks
+};
+
+// Fusion of dense runs into matrix-core blocks pays when the block replaces at
+// least this much vector-ALU work, in units of one dense 2x2 gate.
+constexpr double kMfmaMinWeight = 3.0;
+
+template <typename R>
+RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false) {
   const int K = (int)p.qubits.size();  // == tb + eb
   int loc[64];
   for (int q = 0; q < 64; ++q) loc[q] = -1;
@@ -802,7 +857,8 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
     for (int j = 0; j < eb; ++j) m |= 1u << map[8 + j];
     return m;
   };
-  std::vector<RegOp<R>> prog;
+  RegProgram<R> out;
+  std::vector<RegOp<R>>& prog = out.ops;
   auto emit_remap = [&](const uint8_t* nm) {
     RegOp<R> r{};
     r.kind = R_REMAP;
@@ -846,38 +902,237 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
     }
     prog.push_back(r);
   };
-  size_t remaining = ops.size();
+  // ---- schedule items: single ops, or (mfma) blocks of ops on <= 4 tile bits
+  struct Item {
+    int op;     // index into ops, or -1 for a block
+    int blk;    // index into blks
+    uint32_t qm;
+    bool done;
+  };
+  // A block acts on <= 4 tile bits (q) and may depend on <= kMfmaMaxSel
+  // qubits outside the tile (g: their controls and diagonal factors); it is
+  // applied with one of 2^|g| matrices, chosen per tile from its base index.
+  struct Blk {
+    uint32_t q = 0;
+    uint64_t g = 0;
+    std::vector<int> ops;
+  };
+  constexpr int kMfmaMaxSel = 2;
+  auto gmask = [&](const L& o) {
+    uint64_t m = o.gc;
+    if (o.a >= kGlobalBit) m |= bit(o.a - kGlobalBit);
+    if (two(o.kind) && o.b >= kGlobalBit) m |= bit(o.b - kGlobalBit);
+    return m;
+  };
+  std::vector<Item> items;
+  std::vector<Blk> blks;
+  const bool use_mfma = mfma && eb == 4 && tb == 8;
+  if (use_mfma) {
+    auto fusable = [&](const L& o) {
+      if (__builtin_popcountll(gmask(o)) > kMfmaMaxSel) return false;
+      return o.kind == OP_U1 || o.kind == OP_FLIP || o.kind == OP_D1 || o.kind == OP_D2 || o.kind == OP_U2 ||
+             o.kind == OP_SWAP;
+    };
+    // vector-ALU work of an op, in dense-2x2 units (an uncontrolled SWAP is a free relabelling)
+    auto weight = [&](const L& o) {
+      switch (o.kind) {
+        case OP_U1: return 1.0;
+        case OP_U2: return 4.0;
+        case OP_SWAP: return (o.cm || o.gc) ? 0.5 : 0.0;
+        default: return 0.35;
+      }
+    };
+    std::vector<Blk> open;  // pairwise disjoint; each commutes with everything emitted after it opened
+    auto flush = [&](size_t bi) {
+      Blk b = std::move(open[bi]);
+      open.erase(open.begin() + (long)bi);
+      double w = 0;
+      for (int k : b.ops) w += weight(ops[k]);
+      if (w >= kMfmaMinWeight) {
+        blks.push_back(b);
+        items.push_back(Item{-1, (int)blks.size() - 1, b.q, false});
+      } else {
+        for (int k : b.ops) items.push_back(Item{k, -1, qmask(ops[k]), false});
+      }
+    };
+    for (int k = 0; k < (int)ops.size(); ++k) {
+      const uint32_t S = qmask(ops[k]);
+      const uint64_t G = gmask(ops[k]);
+      uint32_t U = S;
+      uint64_t GU = G;
+      for (const Blk& b : open)
+        if (b.q & S) {
+          U |= b.q;
+          GU |= b.g;
+        }
+      if (!fusable(ops[k]) || __builtin_popcount(U) > 4 || __builtin_popcountll(GU) > kMfmaMaxSel) {
+        for (size_t bi = 0; bi < open.size();)
+          if (open[bi].q & S) flush(bi);
+          else ++bi;
+        if (!fusable(ops[k]) || S == 0) {
+          items.push_back(Item{k, -1, S, false});
+          continue;
+        }
+        U = S;
+        GU = G;
+      }
+      Blk m;
+      m.q = U;
+      m.g = GU;
+      for (size_t bi = 0; bi < open.size();) {
+        if (open[bi].q & S) {
+          m.ops.insert(m.ops.end(), open[bi].ops.begin(), open[bi].ops.end());
+          open.erase(open.begin() + (long)bi);
+        } else {
+          ++bi;
+        }
+      }
+      std::sort(m.ops.begin(), m.ops.end());  // circuit order is a valid order for disjoint merged blocks
+      m.ops.push_back(k);
+      open.push_back(std::move(m));
+    }
+    while (!open.empty()) flush(0);
+  } else {
+    for (int k = 0; k < (int)ops.size(); ++k) items.push_back(Item{k, -1, qmask(ops[k]), false});
+  }
+  // positions a block uses: block bits a0, a1, a3 on register slots 0, 1, 2 and a2 on thread slot 5
+  auto block_slots = [&]() {
+    return (1u << map[8]) | (1u << map[9]) | (1u << map[10]) | (1u << map[5]);
+  };
+  auto emit_block = [&](const Blk& b) {
+    int pos[32];
+    for (int i = 0; i < 32; ++i) pos[i] = -1;
+    pos[map[8]] = 0;
+    pos[map[9]] = 1;
+    pos[map[5]] = 2;
+    pos[map[10]] = 3;
+    int sel[kMfmaMaxSel], ns = 0;
+    for (int q = 0; q < 64; ++q)
+      if ((b.g >> q) & 1) sel[ns++] = q;
+    RegOp<R> r{};
+    r.kind = R_MFMA;
+    r.s0 = (int32_t)out.blocks.size();
+    r.pad = ns;
+    for (int i = 0; i < ns; ++i) r.map_old[i] = (uint8_t)sel[i];
+    for (int var = 0; var < (1 << ns); ++var) {
+      uint64_t gset = 0;  // qubits outside the tile that are 1 in this variant
+      for (int i = 0; i < ns; ++i)
+        if ((var >> i) & 1) gset |= bit(sel[i]);
+      // bit of block index i at op position x (a tile bit, or a qubit outside the tile)
+      auto bitat = [&](int x, int i) -> int {
+        return x >= kGlobalBit ? (int)((gset >> (x - kGlobalBit)) & 1) : (i >> pos[x]) & 1;
+      };
+      std::array<cd, 256> M{};
+      for (int c = 0; c < 16; ++c) {  // column c = the block applied to basis vector c
+        cd v[16];
+        for (int i = 0; i < 16; ++i) v[i] = (i == c) ? cd(1, 0) : cd(0, 0);
+        for (int k : b.ops) {
+          const L& o = ops[k];
+          if ((o.gc & ~gset) != 0) continue;  // a control outside the tile is 0 in this variant
+          const HostOp& h = *o.h;
+          uint32_t cmk = 0;
+          for (int q = 0; q < K; ++q)
+            if ((o.cm >> q) & 1) cmk |= 1u << pos[q];
+          const int pa = o.a < kGlobalBit ? pos[o.a] : 0, pb = (two(o.kind) && o.b < kGlobalBit) ? pos[o.b] : 0;
+          for (int i = 0; i < 16; ++i) {
+            if ((i & cmk) != cmk) continue;
+            switch (o.kind) {
+              case OP_U1:
+                if (!((i >> pa) & 1)) {
+                  const int j = i | (1 << pa);
+                  const cd x = v[i], y = v[j];
+                  v[i] = h.m[0] * x + h.m[1] * y;
+                  v[j] = h.m[2] * x + h.m[3] * y;
+                }
+                break;
+              case OP_FLIP:
+                if (!((i >> pa) & 1)) std::swap(v[i], v[i | (1 << pa)]);
+                break;
+              case OP_D1: v[i] *= h.m[bitat(o.a, i)]; break;
+              case OP_D2: v[i] *= h.m[bitat(o.a, i) | (bitat(o.b, i) << 1)]; break;
+              case OP_U2:
+                if (!((i >> pa) & 1) && !((i >> pb) & 1)) {
+                  const int idx[4] = {i, i | (1 << pa), i | (1 << pb), i | (1 << pa) | (1 << pb)};
+                  const cd x[4] = {v[idx[0]], v[idx[1]], v[idx[2]], v[idx[3]]};
+                  for (int r = 0; r < 4; ++r)
+                    v[idx[r]] = h.m[4 * r] * x[0] + h.m[4 * r + 1] * x[1] + h.m[4 * r + 2] * x[2] +
+                                h.m[4 * r + 3] * x[3];
+                }
+                break;
+              case OP_SWAP:
+                if (((i >> pa) & 1) && !((i >> pb) & 1)) std::swap(v[i], v[(i ^ (1 << pa)) | (1 << pb)]);
+                break;
+              default: break;
+            }
+          }
+        }
+        for (int i = 0; i < 16; ++i) M[16 * i + c] = v[i];
+      }
+      out.blocks.push_back(M);
+    }
+    out.fused_ops += b.ops.size();
+    prog.push_back(r);
+    std::swap(map[5], map[9]);  // the result holds a1 on thread slot 5 and a2 on register slot 1
+  };
+  size_t remaining = items.size();
   while (true) {
     // a relabelling SWAP can make earlier-skipped ops runnable: rescan
     bool progress = true;
     while (progress && remaining) {
       progress = false;
       uint32_t blocked = 0;
-      for (L& o : ops) {
-        if (o.done) continue;
-        const uint32_t qm = qmask(o);
-        if ((qm & blocked) == 0 && (targets(o) & ~regmask()) == 0) {
-          emit(o);
-          o.done = true;
+      for (Item& it : items) {
+        if (it.done) continue;
+        bool ready = (it.qm & blocked) == 0;
+        if (ready) ready = it.op >= 0 ? (targets(ops[it.op]) & ~regmask()) == 0
+                                      : (blks[it.blk].q & ~block_slots()) == 0;
+        if (ready) {
+          if (it.op >= 0) emit(ops[it.op]);
+          else emit_block(blks[it.blk]);
+          it.done = true;
           --remaining;
           progress = true;
         } else {
-          blocked |= qm;
+          blocked |= it.qm;
         }
       }
     }
     if (remaining == 0) break;
+    uint8_t nm[16];
+    for (int i = 0; i < 16; ++i) nm[i] = map[i];
+    const Item* first = nullptr;
+    for (const Item& it : items)
+      if (!it.done) {
+        first = &it;
+        break;
+      }
+    if (first->op < 0) {
+      // move the block's bits onto register slots 0..2 and thread slot 5,
+      // keeping those already there
+      const int P[4] = {8, 9, 10, 5};
+      uint32_t miss = blks[first->blk].q & ~block_slots();
+      for (int pi = 0; pi < 4 && miss; ++pi) {
+        if ((blks[first->blk].q >> nm[P[pi]]) & 1) continue;  // holds a block bit already
+        const int lb = __builtin_ctz(miss);
+        miss &= miss - 1;
+        int from = -1;
+        for (int i = 0; i < 16; ++i)
+          if ((i < tb || (i >= 8 && i < 8 + eb)) && nm[i] == lb) from = i;
+        std::swap(nm[from], nm[P[pi]]);
+      }
+      emit_remap(nm);
+      continue;
+    }
     uint32_t blocked = 0;
     // register set for the next phase: targets of the pending ops, in order
     uint32_t need = 0;
-    for (const L& o : ops) {
-      if (o.done) continue;
-      const uint32_t qm = qmask(o), tg = targets(o);
-      if ((qm & blocked) == 0 && __builtin_popcount(need | tg) <= eb) need |= tg;
-      else blocked |= qm;
+    for (const Item& it : items) {
+      if (it.done) continue;
+      const uint32_t tg = it.op >= 0 ? targets(ops[it.op]) : ~0u;
+      if ((it.qm & blocked) == 0 && it.op >= 0 && __builtin_popcount(need | tg) <= eb) need |= tg;
+      else blocked |= it.qm;
       if (__builtin_popcount(need) == eb) break;
     }
-    uint8_t nm[16];
     for (int i = 0; i < 16; ++i) nm[i] = map[i];
     std::vector<int> free_reg, evicted;
     for (int j = 0; j < eb; ++j) {
@@ -897,7 +1152,59 @@ std::vector<RegOp<R>> build_reg_program(const Pass& p, int eb, int tb) {
     emit_remap(nm);
   }
   if (std::memcmp(map, canon, 16) != 0) emit_remap(canon);
-  return prog;
+  return out;
+}
+
+// IEEE binary16 of x, round to nearest even (normals, subnormals, overflow to
+// infinity); the operand blobs of R_MFMA ops are built with it.
+static uint16_t to_half(double x) {
+  const float f = (float)x;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const int exp = (int)((u >> 23) & 0xff) - 127;
+  uint32_t mant = u & 0x7fffffu;
+  if (exp == 128) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0u));  // inf / nan
+  if (exp > 15) return (uint16_t)(sign | 0x7c00u);
+  if (exp >= -14) {  // normal: keep 10 of 23 mantissa bits
+    uint32_t h = ((uint32_t)(exp + 15) << 10) | (mant >> 13);
+    const uint32_t rest = mant & 0x1fffu;
+    if (rest > 0x1000u || (rest == 0x1000u && (h & 1u))) ++h;  // may carry into the exponent: still right
+    return (uint16_t)(sign | h);
+  }
+  if (exp < -25) return (uint16_t)sign;
+  // subnormal: value = m * 2^-24 with m = (1.mant) * 2^(exp + 24)
+  mant |= 0x800000u;
+  const int shift = -exp - 1;  // 14 .. 24: bits dropped from the 24-bit significand to reach 2^-24 units
+  uint32_t h = mant >> shift;
+  const uint32_t rest = mant & ((1u << shift) - 1), half = 1u << (shift - 1);
+  if (rest > half || (rest == half && (h & 1u))) ++h;
+  return (uint16_t)(sign | h);
+}
+
+static double from_half(uint16_t h) {
+  const int sign = (h >> 15) & 1, exp = (h >> 10) & 0x1f, mant = h & 0x3ff;
+  double v = exp == 0 ? std::ldexp((double)mant, -24) : std::ldexp((double)(mant | 0x400), exp - 25);
+  if (exp == 31) v = mant ? NAN : INFINITY;
+  return sign ? -v : v;
+}
+
+// Operand blob of one R_MFMA op from its 16x16 block matrix (block-index order).
+static void mfma_blob(const std::array<cd, 256>& M, uint16_t* out /* kMfmaBlobBytes / 2 */) {
+  auto A = [&](int m, int k) {  // 32x32 real form
+    const cd u = M[16 * (m >> 1) + (k >> 1)];
+    const int cr = m & 1, cc = k & 1;
+    return cr == 0 ? (cc == 0 ? u.real() : -u.imag()) : (cc == 0 ? u.imag() : u.real());
+  };
+  for (int l = 0; l < 64; ++l)
+    for (int s = 0; s < 2; ++s)
+      for (int i = 0; i < 8; ++i) {
+        const double x = A(l % 32, 16 * s + 8 * (l / 32) + i);
+        const uint16_t hi = to_half(x);
+        const uint16_t lo = to_half(x - from_half(hi));
+        out[l * 32 + s * 16 + i] = hi;
+        out[l * 32 + s * 16 + 8 + i] = lo;
+      }
 }
 
 // A pass's ops in the LDS tile kernel's encoding: tile-local bit positions,
@@ -993,8 +1300,9 @@ static void emulate_lds_pass(std::vector<cd>& a, int n, const Pass& p, const std
 
 // Host emulation of one k_tile_reg pass over a complex128 vector, following
 // the kernel's thread / register / LDS semantics step by step.
-static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std::vector<RegOp<double>>& prog,
-                             int eb, int tb = 8) {
+static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const RegProgram<double>& rp, int eb,
+                             int tb = 8) {
+  const std::vector<RegOp<double>>& prog = rp.ops;
   const int K = tb + eb, E = 1 << eb, NTH = 1 << tb;
   const uint64_t ntiles = 1ull << (n - K);
   std::vector<cd> v(256 * E), lds(size_t(1) << K);
@@ -1024,6 +1332,35 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const std
           for (int r = 0; r < E; ++r) lds[tmap(op.map_old, tid, r)] = v[tid * E + r];
         for (int tid = 0; tid < NTH; ++tid)
           for (int r = 0; r < E; ++r) v[tid * E + r] = lds[tmap(op.map_new, tid, r)];
+        continue;
+      }
+      if (op.kind == R_MFMA) {
+        // per wavefront, column (lane bits 0..4) and register half e: the 16
+        // block amplitudes are a0, a1, a3 = register bits 0, 1, 2 and a2 =
+        // lane bit 5 on input; a0, a2, a3 = register bits 0, 1, 2 and a1 =
+        // lane bit 5 on output
+        int var = 0;  // variant chosen by the qubits outside the tile the block depends on
+        for (int i = 0; i < op.pad; ++i) var |= (int)((base >> op.map_old[i]) & 1) << i;
+        const std::array<cd, 256>& M = rp.blocks[op.s0 + var];
+        std::vector<cd> nv(v);
+        for (int w = 0; w < NTH; w += 64)
+          for (int col = 0; col < 32; ++col)
+            for (int e = 0; e < 2; ++e) {
+              cd x[16];
+              for (int ai = 0; ai < 16; ++ai) {
+                const int tid = w + col + 32 * ((ai >> 2) & 1);
+                const int r = (ai & 1) + 2 * ((ai >> 1) & 1) + 4 * ((ai >> 3) & 1) + 8 * e;
+                x[ai] = v[tid * E + r];
+              }
+              for (int ao = 0; ao < 16; ++ao) {
+                cd y = 0;
+                for (int ai = 0; ai < 16; ++ai) y += M[16 * ao + ai] * x[ai];
+                const int tid = w + col + 32 * ((ao >> 1) & 1);
+                const int r = (ao & 1) + 2 * ((ao >> 2) & 1) + 4 * ((ao >> 3) & 1) + 8 * e;
+                nv[tid * E + r] = y;
+              }
+            }
+        v.swap(nv);
         continue;
       }
       if ((base & op.gc) != op.gc) continue;  // a control outside the tile is 0 here
@@ -1139,7 +1476,7 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
   const bool reg = po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13;
   for (const Pass& p : passes) {
     if (reg) {
-      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8), K - 8);
+      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8, po.mfma), K - 8);
     } else {
       std::vector<DevOp<double>> dops;
       append_dev_ops<double>(p, dops);
@@ -1150,8 +1487,9 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
 }
 
 template <typename R>
-static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb) {
+static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb, bool mfma) {
   std::vector<RegOp<R>> prog;
+  std::vector<uint16_t> blob;  // R_MFMA operand blobs of the whole run
   std::vector<PassArgs> args;
   for (const Pass& p : passes) {
     PassArgs pa{};
@@ -1159,21 +1497,37 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     pa.n = s->n;
     pa.op0 = (int)prog.size();
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
-    std::vector<RegOp<R>> pr = build_reg_program<R>(p, eb, 8);
-    pa.nops = (int)pr.size();
-    for (const RegOp<R>& r : pr) s->stats.remaps += r.kind == R_REMAP;
-    prog.insert(prog.end(), pr.begin(), pr.end());
+    RegProgram<R> pr = build_reg_program<R>(p, eb, 8, mfma);
+    const int b0 = (int)(blob.size() / (kMfmaBlobBytes / 2));
+    for (RegOp<R>& r : pr.ops) {
+      s->stats.remaps += r.kind == R_REMAP;
+      if (r.kind == R_MFMA) r.s0 += b0;
+    }
+    for (const auto& M : pr.blocks) {
+      const size_t at = blob.size();
+      blob.resize(at + kMfmaBlobBytes / 2);
+      mfma_blob(M, blob.data() + at);
+    }
+    s->stats.mfma_blocks += pr.blocks.size();
+    s->stats.mfma_gates += pr.fused_ops;
+    pa.scaled = pr.blocks.empty() ? 0 : 1;
+    pa.nops = (int)pr.ops.size();
+    prog.insert(prog.end(), pr.ops.begin(), pr.ops.end());
     args.push_back(pa);
   }
-  size_t ob = std::max<size_t>(prog.size(), 1) * sizeof(RegOp<R>);
+  const size_t pb = (std::max<size_t>(prog.size(), 1) * sizeof(RegOp<R>) + 255) & ~size_t(255);
+  const size_t ob = pb + blob.size() * 2;
   HIPTRY(hipStreamSynchronize(s->stream), "sync before program upload");
   if (!s->opbuf.ensure(ob)) return NQPU_OUT_OF_DEVICE_MEMORY;
   if (!prog.empty())
     HIPTRY(copy_on(s->stream, s->opbuf.p, prog.data(), prog.size() * sizeof(RegOp<R>), hipMemcpyHostToDevice),
            "program upload");
+  if (!blob.empty())
+    HIPTRY(copy_on(s->stream, (char*)s->opbuf.p + pb, blob.data(), blob.size() * 2, hipMemcpyHostToDevice),
+           "block upload");
   for (const PassArgs& pa : args) {
     s->timer.mark(s->stream);
-    Launch<R>::tile_pass_reg(s->amps, pa, eb, s->opbuf.p, s->stream);
+    Launch<R>::tile_pass_reg(s->amps, pa, eb, s->opbuf.p, (char*)s->opbuf.p + pb, s->stream);
     s->timer.mark(s->stream);
     HIPTRY(hipGetLastError(), "register tile pass launch");
   }
@@ -1188,7 +1542,7 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
   const int K = std::min(po.tile_qubits, s->n);
   if (po.registers && std::is_same<R, float>::value && K >= 11 && K <= 13) {
     s->stats.device_ops += ops.size();
-    return run_tiled_reg<R>(s, passes, K - 8);
+    return run_tiled_reg<R>(s, passes, K - 8, po.mfma);
   }
   std::vector<DevOp<R>> dops;
   std::vector<PassArgs> args;
@@ -1227,6 +1581,7 @@ PlanOptions plan_options(const NqpuRunOptions& o, int prec) {
   po.low_qubits = o.low_qubits ? (int)o.low_qubits : (prec == NQPU_PRECISION_C64 ? 5 : 4);
   po.low_qubits = std::min(po.low_qubits, po.tile_qubits);
   po.registers = (o.flags & NQPU_RUN_LDS_TILES) == 0;
+  po.mfma = (o.flags & NQPU_RUN_NO_MFMA) == 0;
   return po;
 }
 
@@ -1264,8 +1619,12 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
     st.bytes_moved = (double)passes.size() * 2.0 * (double)(asz << c.n);
     const int K = std::min(po.tile_qubits, c.n);
     if (po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13)
-      for (const Pass& p : passes)
-        for (const RegOp<double>& r : build_reg_program<double>(p, K - 8, 8)) st.remaps += r.kind == R_REMAP;
+      for (const Pass& p : passes) {
+        const RegProgram<double> rp = build_reg_program<double>(p, K - 8, 8, po.mfma);
+        for (const RegOp<double>& r : rp.ops) st.remaps += r.kind == R_REMAP;
+        st.mfma_blocks += rp.blocks.size();
+        st.mfma_gates += rp.fused_ops;
+      }
   }
   return st;
 }
@@ -1697,7 +2056,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
       P.K = n;
       P.tb = n - EB;
       for (int q = 0; q < n; ++q) P.q[q] = q;
-      std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, n - EB);
+      std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, n - EB).ops;
       P.op0 = (int)prog.size();
       P.nops = (int)pr.size();
       prog.insert(prog.end(), pr.begin(), pr.end());
@@ -1724,7 +2083,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
         P.K = KH;
         P.tb = 8;
         for (int j = 0; j < KH; ++j) P.q[j] = p.qubits[j];
-        std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, 8);
+        std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, 8).ops;
         P.op0 = (int)prog.size();
         P.nops = (int)pr.size();
         prog.insert(prog.end(), pr.begin(), pr.end());

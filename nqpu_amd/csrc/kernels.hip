@@ -602,19 +602,25 @@ __device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, in
 // in batched execution); they only take part in the barriers.
 template <typename R, typename T, int EB>
 __device__ __forceinline__ void reg_remap(T (&v)[1 << EB], T* sm, const RegOp<R>& op, int tid, bool active) {
-  uint32_t tp = 0, rp[EB];
+  // lds_swz is linear over GF(2), and the thread and register parts of a tile
+  // index occupy different bits, so the swizzled byte offset of register r is
+  // the thread's part XOR a combination of per-register-slot parts that is
+  // the same for every lane: one vector XOR per register.
+  char* lds = reinterpret_cast<char*>(sm);
+  uint32_t tp = 0, rs[EB];
 #pragma unroll
   for (int i = 0; i < 8; ++i) tp |= (uint32_t)((tid >> i) & 1) << op.map_old[i];
 #pragma unroll
-  for (int j = 0; j < EB; ++j) rp[j] = 1u << op.map_old[8 + j];
+  for (int j = 0; j < EB; ++j) rs[j] = lds_swz(1u << op.map_old[8 + j]) * (uint32_t)sizeof(T);
+  uint32_t ts = lds_swz(tp) * (uint32_t)sizeof(T);
   if (active) {
 #pragma unroll
     for (int r = 0; r < (1 << EB); ++r) {
-      uint32_t t = tp;
+      uint32_t u = 0;
 #pragma unroll
       for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) t |= rp[j];
-      sm[lds_swz(t)] = v[r];
+        if ((r >> j) & 1) u ^= rs[j];
+      *reinterpret_cast<T*>(lds + (ts ^ u)) = v[r];
     }
   }
   __syncthreads();
@@ -622,15 +628,16 @@ __device__ __forceinline__ void reg_remap(T (&v)[1 << EB], T* sm, const RegOp<R>
 #pragma unroll
   for (int i = 0; i < 8; ++i) tp |= (uint32_t)((tid >> i) & 1) << op.map_new[i];
 #pragma unroll
-  for (int j = 0; j < EB; ++j) rp[j] = 1u << op.map_new[8 + j];
+  for (int j = 0; j < EB; ++j) rs[j] = lds_swz(1u << op.map_new[8 + j]) * (uint32_t)sizeof(T);
+  ts = lds_swz(tp) * (uint32_t)sizeof(T);
   if (active) {
 #pragma unroll
     for (int r = 0; r < (1 << EB); ++r) {
-      uint32_t t = tp;
+      uint32_t u = 0;
 #pragma unroll
       for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) t |= rp[j];
-      v[r] = sm[lds_swz(t)];
+        if ((r >> j) & 1) u ^= rs[j];
+      v[r] = *reinterpret_cast<const T*>(lds + (ts ^ u));
     }
   }
   __syncthreads();
@@ -662,11 +669,58 @@ __device__ __forceinline__ void reg_pair_op(T (&v)[1 << EB], const RegOp<R>& op,
 #undef NQPU_J2
 }
 
+typedef _Float16 nqpu_half8 __attribute__((ext_vector_type(8)));
+typedef float nqpu_f32x16 __attribute__((ext_vector_type(16)));
+
+// R_MFMA: a dense 16x16 complex block on the matrix cores.  Per register
+// half e (register bit 3), the wavefront's 32 columns (lane bits 0..4) are 32
+// vectors of 16 amplitudes: K-step s holds the 8 reals of a3 = s, a2 = lane
+// bit 5, (a0, a1) = register bits 0, 1.  Amplitudes (already scaled by
+// kMfmaScale) and the matrix are split into fp16 high and low parts; the
+// products hi*hi + hi*lo + lo*hi accumulate in FP32, which leaves a relative
+// error of about 2^-22 per element.  The 16 accumulator values of lane l are
+// rows 8j + 4 (l / 32) + i: exactly the amplitudes a0 = i / 2, a1 = lane bit
+// 5, a2 = j & 1, a3 = j >> 1 in register order (a0, a2, a3).
+template <typename T, int EB>
+__device__ __forceinline__ void reg_mfma(T (&v)[1 << EB], const unsigned char* blob, int s0, int lane) {
+  if constexpr (std::is_same<T, float2>::value && EB == 4) {
+    const nqpu_half8* ap = reinterpret_cast<const nqpu_half8*>(blob + (size_t)s0 * kMfmaBlobBytes + lane * 64);
+    const nqpu_half8 ah0 = ap[0], al0 = ap[1], ah1 = ap[2], al1 = ap[3];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      nqpu_half8 bh[2], bl[2];
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float2 x = v[8 * e + 4 * s + q];
+          const _Float16 hx = (_Float16)x.x, hy = (_Float16)x.y;
+          bh[s][2 * q] = hx;
+          bh[s][2 * q + 1] = hy;
+          bl[s][2 * q] = (_Float16)(x.x - (float)hx);
+          bl[s][2 * q + 1] = (_Float16)(x.y - (float)hy);
+        }
+      nqpu_f32x16 d = {};
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(al0, bh[0], d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bl[0], d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(al1, bh[1], d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bl[1], d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bh[0], d, 0, 0, 0);
+      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bh[1], d, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        v[8 * e + j].x = d[2 * j];
+        v[8 * e + j].y = d[2 * j + 1];
+      }
+    }
+  }
+}
+
 // Run a register-tile program on v (the standalone pass and the batched
 // kernel share it).
 template <typename R, typename T, int EB>
 __device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<R>* ops, int nops, int tid,
-                                            bool active, uint64_t base) {
+                                            bool active, uint64_t base, const unsigned char* blob) {
   for (int o = 0; o < nops; ++o) {
     const RegOp<R>& op = ops[o];
     // controls outside the tile: uniform over the workgroup (remaps have none)
@@ -674,6 +728,14 @@ __device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<
     const bool on = (tid & op.ct) == op.ct;
     switch (op.kind) {
       case R_REMAP: reg_remap<R, T, EB>(v, sm, op, tid, active); break;
+      case R_MFMA: {
+        // the block's matrix variant: bits of the tile's base index at the
+        // qubits outside the tile it depends on (uniform over the workgroup)
+        int var = 0;
+        for (int i = 0; i < op.pad; ++i) var |= (int)((base >> op.map_old[i]) & 1) << i;
+        reg_mfma<T, EB>(v, blob, op.s0 + var, tid & 63);
+        break;
+      }
       case R_U1: NQPU_SLOT_CASES(reg_u1, v, op, on) break;
       case R_FLIP: NQPU_SLOT_CASES(reg_flip, v, op, on) break;
       case R_D1:
@@ -687,7 +749,8 @@ __device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<
 
 template <typename R, typename T, int EB>
 __global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArgs pa, uint64_t ntiles,
-                                                  const RegOp<R>* __restrict__ prog) {
+                                                  const RegOp<R>* __restrict__ prog,
+                                                  const unsigned char* __restrict__ blob) {
   constexpr int E = 1 << EB, NT = 256;
   extern __shared__ unsigned char smraw[];
   T* sm = reinterpret_cast<T*>(smraw);
@@ -709,7 +772,21 @@ __global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArg
         if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
       v[r] = st[d];
     }
-    reg_program<R, T, EB>(v, sm, ops, pa.nops, tid, true, base);
+    if (pa.scaled) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        v[r].x *= (R)kMfmaScale;
+        v[r].y *= (R)kMfmaScale;
+      }
+    }
+    reg_program<R, T, EB>(v, sm, ops, pa.nops, tid, true, base, blob);
+    if (pa.scaled) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        v[r].x *= (R)(1.0f / kMfmaScale);
+        v[r].y *= (R)(1.0f / kMfmaScale);
+      }
+    }
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       uint64_t d = base | dlo;
@@ -817,7 +894,7 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
         v[r].x = (tid == 0 && r == 0) ? 1.f : 0.f;
         v[r].y = 0.f;
       }
-      reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, active, 0);
+      reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, active, 0, nullptr);
       if (active) {
 #pragma unroll
         for (int r = 0; r < E; ++r) sm[tid | (r << P.tb)] = v[r];
@@ -851,7 +928,7 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
               v[r] = st[d];
             }
           }
-          reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true, base);
+          reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true, base, nullptr);
 #pragma unroll
           for (int r = 0; r < E; ++r) {
             uint64_t d = base | dlo;
@@ -1254,7 +1331,7 @@ void Launch<R>::tile_pass(void* st, const PassArgs& pa, const void* dops, hipStr
 }
 
 template <typename R, int EB>
-static void launch_tile_reg(void* st, const PassArgs& pa, const void* prog, hipStream_t s) {
+static void launch_tile_reg(void* st, const PassArgs& pa, const void* prog, const void* blob, hipStream_t s) {
   const uint64_t ntiles = 1ull << (pa.n - pa.K);
   const uint64_t g = ntiles < (1ull << 20) ? ntiles : (1ull << 20);
   const size_t lds = sizeof(T_) << pa.K;
@@ -1264,15 +1341,17 @@ static void launch_tile_reg(void* st, const PassArgs& pa, const void* prog, hipS
     (void)hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_set = true;
   }
-  hipLaunchKernelGGL(fn, dim3((unsigned)g), dim3(kBlock), lds, s, (T_*)st, pa, ntiles, (const RegOp<R>*)prog);
+  hipLaunchKernelGGL(fn, dim3((unsigned)g), dim3(kBlock), lds, s, (T_*)st, pa, ntiles, (const RegOp<R>*)prog,
+                     (const unsigned char*)blob);
 }
 
 template <typename R>
-void Launch<R>::tile_pass_reg(void* st, const PassArgs& pa, int eb, const void* prog, hipStream_t s) {
+void Launch<R>::tile_pass_reg(void* st, const PassArgs& pa, int eb, const void* prog, const void* blob,
+                              hipStream_t s) {
   switch (eb) {
-    case 3: launch_tile_reg<R, 3>(st, pa, prog, s); return;
-    case 4: launch_tile_reg<R, 4>(st, pa, prog, s); return;
-    case 5: launch_tile_reg<R, 5>(st, pa, prog, s); return;
+    case 3: launch_tile_reg<R, 3>(st, pa, prog, blob, s); return;
+    case 4: launch_tile_reg<R, 4>(st, pa, prog, blob, s); return;
+    case 5: launch_tile_reg<R, 5>(st, pa, prog, blob, s); return;
     default: return;
   }
 }

@@ -56,6 +56,9 @@ struct PlanOptions {
   // Controls and diagonal factors on qubits outside a tile are resolved per
   // tile, so only the targets of non-diagonal ops need to be tile bits.
   bool global_ops = true;
+  // fuse runs of gates on four tile bits into dense blocks applied on the
+  // matrix cores (complex64 register tiles)
+  bool mfma = true;
 };
 
 // lowering and planning (pure host code; unit-testable without a device)
