@@ -23,6 +23,9 @@ FLAGS = [
     "-Wall",
     "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core results go straight to the VGPRs that hold the amplitudes
+    # (no accumulator-register copies after every fused block)
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

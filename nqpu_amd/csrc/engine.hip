@@ -797,7 +797,7 @@ struct RegProgram {
 
 // Fusion of dense runs into matrix-core blocks pays when the block replaces at
 // least this much vector-ALU work, in units of one dense 2x2 gate.
-constexpr double kMfmaMinWeight = 3.0;
+constexpr double kMfmaMinWeight = 1.0;
 
 template <typename R>
 RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false) {

@@ -670,17 +670,35 @@ __device__ __forceinline__ void reg_pair_op(T (&v)[1 << EB], const RegOp<R>& op,
 }
 
 typedef _Float16 nqpu_half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 nqpu_half2 __attribute__((ext_vector_type(2)));
+typedef float nqpu_float2v __attribute__((ext_vector_type(2)));
 typedef float nqpu_f32x16 __attribute__((ext_vector_type(16)));
+
+// x - (float)h for the low (HI = false) or high half of the packed fp16 pair
+// h, exact: one mixed-precision FMA, -h * 1 + x.  hipcc does not form
+// v_fma_mix_f32 from the plain expression (it emits a convert and a subtract).
+template <bool HI>
+__device__ __forceinline__ float sub_half(unsigned h, float x) {
+  float r;
+  if constexpr (HI)
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
+  else
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
+  return r;
+}
 
 // R_MFMA: a dense 16x16 complex block on the matrix cores.  Per register
 // half e (register bit 3), the wavefront's 32 columns (lane bits 0..4) are 32
 // vectors of 16 amplitudes: K-step s holds the 8 reals of a3 = s, a2 = lane
 // bit 5, (a0, a1) = register bits 0, 1.  Amplitudes (already scaled by
-// kMfmaScale) and the matrix are split into fp16 high and low parts; the
+// kMfmaScale) and the matrix are split into fp16 high and low parts (high =
+// round to nearest, low = the exact fp32 remainder rounded to nearest); the
 // products hi*hi + hi*lo + lo*hi accumulate in FP32, which leaves a relative
-// error of about 2^-22 per element.  The 16 accumulator values of lane l are
-// rows 8j + 4 (l / 32) + i: exactly the amplitudes a0 = i / 2, a1 = lane bit
-// 5, a2 = j & 1, a3 = j >> 1 in register order (a0, a2, a3).
+// error of about 2^-22 per element.  The split costs four vector instructions
+// per amplitude: two packed converts and two mixed-precision FMAs.  The 16
+// accumulator values of lane l are rows 8j + 4 (l / 32) + i: exactly the
+// amplitudes a0 = i / 2, a1 = lane bit 5, a2 = j & 1, a3 = j >> 1 in register
+// order (a0, a2, a3).
 template <typename T, int EB>
 __device__ __forceinline__ void reg_mfma(T (&v)[1 << EB], const unsigned char* blob, int s0, int lane) {
   if constexpr (std::is_same<T, float2>::value && EB == 4) {
@@ -694,11 +712,14 @@ __device__ __forceinline__ void reg_mfma(T (&v)[1 << EB], const unsigned char* b
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const float2 x = v[8 * e + 4 * s + q];
-          const _Float16 hx = (_Float16)x.x, hy = (_Float16)x.y;
-          bh[s][2 * q] = hx;
-          bh[s][2 * q + 1] = hy;
-          bl[s][2 * q] = (_Float16)(x.x - (float)hx);
-          bl[s][2 * q + 1] = (_Float16)(x.y - (float)hy);
+          const nqpu_half2 h = __builtin_convertvector((nqpu_float2v){x.x, x.y}, nqpu_half2);
+          const unsigned hb = __builtin_bit_cast(unsigned, h);
+          const nqpu_half2 l =
+              __builtin_convertvector((nqpu_float2v){sub_half<false>(hb, x.x), sub_half<true>(hb, x.y)}, nqpu_half2);
+          bh[s][2 * q] = h[0];
+          bh[s][2 * q + 1] = h[1];
+          bl[s][2 * q] = l[0];
+          bl[s][2 * q + 1] = l[1];
         }
       nqpu_f32x16 d = {};
       d = __builtin_amdgcn_mfma_f32_32x32x16_f16(al0, bh[0], d, 0, 0, 0);
