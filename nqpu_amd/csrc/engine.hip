@@ -942,56 +942,82 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         default: return 0.35;
       }
     };
-    std::vector<Blk> open;  // pairwise disjoint; each commutes with everything emitted after it opened
-    auto flush = [&](size_t bi) {
-      Blk b = std::move(open[bi]);
-      open.erase(open.begin() + (long)bi);
+    // Blocks are grown as light cones: starting from a seed op, every later op
+    // whose tile bits fit the (at most 4-bit) set, whose qubits outside the
+    // tile fit the selector set, and which no op left behind precedes on a
+    // shared bit joins.  Each round tries the first few ready ops as seeds and
+    // keeps the heaviest block; an op left behind blocks its bits, so every
+    // op taken commutes with all the ops it overtakes.
+    std::vector<int> rem(ops.size());
+    for (int k = 0; k < (int)ops.size(); ++k) rem[k] = k;
+    auto grow = [&](int seed, std::vector<int>* taken, std::vector<int>* left) {
+      uint32_t S = qmask(ops[seed]), blocked = 0;
+      uint64_t G = gmask(ops[seed]);
       double w = 0;
-      for (int k : b.ops) w += weight(ops[k]);
-      if (w >= kMfmaMinWeight) {
+      for (int k : rem) {
+        const L& o = ops[k];
+        const uint32_t qm = qmask(o);
+        bool ok = (qm & blocked) == 0 && fusable(o);
+        if (ok) {
+          const uint32_t S2 = S | qm;
+          const uint64_t G2 = G | gmask(o);
+          ok = __builtin_popcount(S2) <= 4 && __builtin_popcountll(G2) <= kMfmaMaxSel;
+          if (ok) {
+            S = S2;
+            G = G2;
+          }
+        }
+        if (ok) {
+          w += weight(o);
+          if (taken) taken->push_back(k);
+        } else {
+          blocked |= qm;
+          if (left) left->push_back(k);
+        }
+      }
+      return w;
+    };
+    constexpr int kSeeds = 8;
+    while (!rem.empty()) {
+      const L& first = ops[rem.front()];
+      if (!fusable(first) || qmask(first) == 0) {  // runs on its own
+        items.push_back(Item{rem.front(), -1, qmask(first), false});
+        rem.erase(rem.begin());
+        continue;
+      }
+      int best = -1, tried = 0;
+      double bw = -1;
+      uint32_t blocked = 0;
+      for (int k : rem) {
+        if (tried == kSeeds) break;
+        const L& o = ops[k];
+        const uint32_t qm = qmask(o);
+        if ((qm & blocked) == 0 && fusable(o) && qm != 0) {
+          ++tried;
+          const double w = grow(k, nullptr, nullptr);
+          if (w > bw + 1e-9) {
+            bw = w;
+            best = k;
+          }
+        }
+        blocked |= qm;
+      }
+      std::vector<int> taken, left;
+      grow(best, &taken, &left);
+      Blk b;
+      for (int k : taken) {
+        b.q |= qmask(ops[k]);
+        b.g |= gmask(ops[k]);
+      }
+      b.ops = taken;
+      if (bw >= kMfmaMinWeight) {
         blks.push_back(b);
         items.push_back(Item{-1, (int)blks.size() - 1, b.q, false});
       } else {
-        for (int k : b.ops) items.push_back(Item{k, -1, qmask(ops[k]), false});
+        for (int k : taken) items.push_back(Item{k, -1, qmask(ops[k]), false});
       }
-    };
-    for (int k = 0; k < (int)ops.size(); ++k) {
-      const uint32_t S = qmask(ops[k]);
-      const uint64_t G = gmask(ops[k]);
-      uint32_t U = S;
-      uint64_t GU = G;
-      for (const Blk& b : open)
-        if (b.q & S) {
-          U |= b.q;
-          GU |= b.g;
-        }
-      if (!fusable(ops[k]) || __builtin_popcount(U) > 4 || __builtin_popcountll(GU) > kMfmaMaxSel) {
-        for (size_t bi = 0; bi < open.size();)
-          if (open[bi].q & S) flush(bi);
-          else ++bi;
-        if (!fusable(ops[k]) || S == 0) {
-          items.push_back(Item{k, -1, S, false});
-          continue;
-        }
-        U = S;
-        GU = G;
-      }
-      Blk m;
-      m.q = U;
-      m.g = GU;
-      for (size_t bi = 0; bi < open.size();) {
-        if (open[bi].q & S) {
-          m.ops.insert(m.ops.end(), open[bi].ops.begin(), open[bi].ops.end());
-          open.erase(open.begin() + (long)bi);
-        } else {
-          ++bi;
-        }
-      }
-      std::sort(m.ops.begin(), m.ops.end());  // circuit order is a valid order for disjoint merged blocks
-      m.ops.push_back(k);
-      open.push_back(std::move(m));
+      rem.swap(left);
     }
-    while (!open.empty()) flush(0);
   } else {
     for (int k = 0; k < (int)ops.size(); ++k) items.push_back(Item{k, -1, qmask(ops[k]), false});
   }

@@ -47,6 +47,34 @@ __device__ __forceinline__ uint32_t ins0u(uint32_t v, int q) {
   return ((v >> q) << (q + 1)) | lo;
 }
 
+// Non-temporal state access for full-state passes: every amplitude is read
+// once and written once per pass, so the lines are not kept in the caches.
+template <typename T>
+struct NatVec2;
+template <>
+struct NatVec2<float2> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct NatVec2<double2> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <typename T>
+__device__ __forceinline__ T load_stream(const T* p) {
+  typedef typename NatVec2<T>::type V;
+  const V x = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+  T r;
+  r.x = x[0];
+  r.y = x[1];
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void store_stream(T* p, const T& v) {
+  typedef typename NatVec2<T>::type V;
+  const V x = {v.x, v.y};
+  __builtin_nontemporal_store(x, reinterpret_cast<V*>(p));
+}
+
 // Sorted bit positions with fixed values (targets and controls of an op).
 struct Fixed {
   int32_t nf;
@@ -791,7 +819,7 @@ __global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArg
 #pragma unroll
       for (int j = 0; j < EB; ++j)
         if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
-      v[r] = st[d];
+      v[r] = load_stream(st + d);
     }
     if (pa.scaled) {
 #pragma unroll
@@ -814,7 +842,7 @@ __global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArg
 #pragma unroll
       for (int j = 0; j < EB; ++j)
         if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
-      st[d] = v[r];
+      store_stream(st + d, v[r]);
     }
   }
 }
