@@ -177,7 +177,7 @@ def bench_qft_trotter(args):
     nc = circ.native()
     gates = len(circ.gates)
     run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds,
-                  mfma=not args.no_mfma)
+                  mfma=not args.no_mfma, compile=not (args.interpret or args.lds))
     log(f"workload n={n} gates={gates} plan={nc.plan('c64', **run_kw)}")
     zz = "ZZ" + "I" * (n - 2)
     sv = E.StateVector(n, "c64")
@@ -227,7 +227,8 @@ def bench_qft_trotter(args):
                    "passes_per_step": passes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_tile" if args.lds else "k_tile_reg", "avg_launch_ms": avg_pass_ms,
+                     "kernel": "k_tile" if args.lds else ("nqpu_pass_* (compiled)" if run_kw["compile"] else "k_tile_reg"),
+                     "avg_launch_ms": avg_pass_ms,
                      "bytes_per_launch": bytes_per_pass},
         "z0z1_c64": z64, "norm_c64": norm64, "norm_c64_before_renormalisation": norm_raw,
         "c128": {"qubits": n128, "seconds": dt128, "passes": st128["passes"], "z0z1": z128, "norm": norm128,
@@ -405,6 +406,9 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the complex128 accuracy run")
     ap.add_argument("--no-per-gate", action="store_true")
     ap.add_argument("--lds", action="store_true", help="LDS-resident tile kernel instead of the register one")
+    ap.add_argument("--interpret", action="store_true",
+                    help="run register tile passes with the interpreting kernel instead of per-pass kernels "
+                         "compiled at run time")
     ap.add_argument("--no-mfma", action="store_true",
                     help="keep every gate on the vector ALUs (no fused blocks on the matrix cores)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -432,7 +436,7 @@ def main():
     nc = circ.native()
     gates = len(circ.gates)
     run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds,
-                  mfma=not args.no_mfma)
+                  mfma=not args.no_mfma, compile=not (args.interpret or args.lds))
     log(f"workload n={n} depth={depth} gates={gates} plan={nc.plan('c64', **run_kw)}")
 
     sv = E.StateVector(n, "c64")
@@ -441,9 +445,12 @@ def main():
         tmp.copy_from(sv)
         tmp.close()
         log(f"copy probe: {2 * (8 << n)} bytes moved by k_copy")
+    warm_s = []
     for w in range(args.warmup):
+        tw = time.perf_counter()
         sv.run(nc, **run_kw)
-        log(f"warmup {w + 1}/{args.warmup} done")
+        warm_s.append(time.perf_counter() - tw)
+        log(f"warmup {w + 1}/{args.warmup} done ({warm_s[-1]:.2f} s)")
     pass_ms, timed_passes, passes = 0.0, 0, 0
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -479,9 +486,13 @@ def main():
                    "qubits": n, "depth": depth, "gates": gates, "passes_per_step": passes,
                    "parallelism": "single GPU"},
         "hbm_frac_per_step": hbm_frac_step,
+        "passes_compiled": bool(run_kw["compile"]),
+        # first execution of the circuit includes generating and compiling its pass kernels
+        "first_run_s": warm_s[0] if warm_s else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                     "kernel": "k_tile" if args.lds else "k_tile_reg", "avg_launch_ms": avg_pass_ms,
+                     "kernel": "k_tile" if args.lds else ("nqpu_pass_* (compiled)" if run_kw["compile"] else "k_tile_reg"),
+                     "avg_launch_ms": avg_pass_ms,
                      "bytes_per_launch": bytes_per_pass},
     }
     zz = None

@@ -216,6 +216,12 @@ typedef struct {
 /* complex64 register tiles: apply every gate on the vector ALUs instead of
  * fusing runs of gates on four qubits into dense blocks for the matrix cores */
 #define NQPU_RUN_NO_MFMA 8u
+/* complex64 register tiles: run each pass as its own straight-line kernel,
+ * generated from the pass's register program and compiled with hipRTC on
+ * first use (about a second per eight passes, on parallel threads); the
+ * kernels depend only on the program's structure, not on gate angles, and
+ * stay cached for the life of the process */
+#define NQPU_RUN_COMPILE 16u
 
 /* Statistics of the last run on a state. */
 typedef struct {
@@ -225,9 +231,10 @@ typedef struct {
   double bytes_moved;       /* algorithmic HBM bytes of those sweeps */
   double pass_ms;           /* with NQPU_RUN_TIME_PASSES: summed kernel time of the passes */
   uint64_t timed_passes;    /* passes that pass_ms covers */
-  uint64_t remaps;          /* LDS exchanges inside register-resident tile passes */
+  uint64_t remaps;          /* LDS remaps inside register-resident tile passes */
   uint64_t mfma_blocks;     /* fused 4-qubit blocks applied on the matrix cores */
   uint64_t mfma_gates;      /* device ops those blocks absorbed */
+  uint64_t exchanges;       /* register <-> lane-bit exchanges inside register tile passes (no LDS) */
 } NqpuRunStats;
 
 /* 1 <= n <= 34 (c64) or 33 (c128), bounded by free device memory. */
@@ -327,6 +334,13 @@ NqpuError nqpu_debug_batch_plan(const NqpuCircuit* const* circuits, uint64_t cou
  * without a device; slow. */
 NqpuError nqpu_debug_emulate(const NqpuCircuit* circuit, const NqpuRunOptions* opts, uint32_t precision,
                              NqpuComplex* amps, uint64_t len);
+
+/* Host-only check of NQPU_RUN_COMPILE for a complex64 state: generate the
+ * straight-line kernels of every register tile pass of `circuit` and compile
+ * them with hipRTC (nothing is loaded or cached; no device needed).
+ * NQPU_ERROR_DEVICE_ERROR with the compiler log in nqpu_last_error_detail()
+ * when compilation fails. */
+NqpuError nqpu_debug_compile_passes(const NqpuCircuit* circuit, const NqpuRunOptions* opts);
 
 /* Detail string of the last error on the calling thread. */
 const char* nqpu_last_error_detail(void);

@@ -58,7 +58,8 @@ RUN_LDS_TILES = 2
 class NqpuRunStats(ct.Structure):
     _fields_ = [("gates", ct.c_uint64), ("device_ops", ct.c_uint64), ("passes", ct.c_uint64),
                 ("bytes_moved", ct.c_double), ("pass_ms", ct.c_double), ("timed_passes", ct.c_uint64),
-                ("remaps", ct.c_uint64), ("mfma_blocks", ct.c_uint64), ("mfma_gates", ct.c_uint64)]
+                ("remaps", ct.c_uint64), ("mfma_blocks", ct.c_uint64), ("mfma_gates", ct.c_uint64),
+                ("exchanges", ct.c_uint64)]
 
 
 class NqpuPauliTerm(ct.Structure):
@@ -73,6 +74,7 @@ class NqpuBatchStats(ct.Structure):
 
 RUN_BATCH_LAYOUT = 4
 RUN_NO_MFMA = 8
+RUN_COMPILE = 16
 
 
 class NqpuFusionCounts(ct.Structure):
@@ -161,6 +163,7 @@ SIGNATURES = {
     "nqpu_circuit_plan": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, ct.POINTER(NqpuRunStats)]),
     "nqpu_circuit_fusion_counts": (_E, [_P, ct.POINTER(NqpuFusionCounts)]),
     "nqpu_debug_emulate": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, _P, ct.c_uint64]),
+    "nqpu_debug_compile_passes": (_E, [_P, ct.POINTER(NqpuRunOptions)]),
     "nqpu_batch_create": (_P, []),
     "nqpu_batch_destroy": (None, [_P]),
     "nqpu_batch_expectation": (_E, [_P, _P, ct.c_uint64, _P, _P, ct.POINTER(NqpuRunOptions), _P]),

@@ -13,7 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libnqpu_hip.so")
-SOURCES = ["kernels.hip", "engine.hip", "capi.hip", "server.hip"]
+SOURCES = ["kernels.hip", "engine.hip", "capi.hip", "server.hip", "jit.hip"]
+# device code that run-time compiled passes embed (jit.hip), in this order
+JIT_HEADERS = ["devtypes.h", "regtile.h"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",
@@ -29,16 +31,35 @@ FLAGS = [
 ]
 
 
+def write_jit_source(objdir: str) -> None:
+    """jit_src.inc: the text of JIT_HEADERS as one C++ raw string, after
+    fixed-width integer typedefs (hipRTC provides no <cstdint>)."""
+    parts = ["typedef signed char int8_t;\ntypedef unsigned char uint8_t;\ntypedef int int32_t;\n"
+             "typedef unsigned int uint32_t;\ntypedef long long int64_t;\ntypedef unsigned long long uint64_t;\n"]
+    for h in JIT_HEADERS:
+        with open(os.path.join(CSRC, h)) as f:
+            parts.append("".join(l for l in f if not l.startswith("#pragma once")))
+    text = "".join(parts)
+    assert ")NQPU_JIT\"" not in text
+    out = os.path.join(objdir, "jit_src.inc")
+    body = "static const char kJitPrelude[] = R\"NQPU_JIT(" + text + ")NQPU_JIT\";\n"
+    old = open(out).read() if os.path.exists(out) else None
+    if old != body:
+        with open(out, "w") as f:
+            f.write(body)
+
+
 def build(verbose: bool = False, jobs: int = 4) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    write_jit_source(objdir)
     procs = []
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *FLAGS, "-I", objdir, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -55,7 +76,7 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     if failed:
         raise RuntimeError(f"hipcc failed for {failed}")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-lhiprtc"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     return LIB

@@ -434,6 +434,14 @@ NqpuError nqpu_circuit_plan(const NqpuCircuit* c, const NqpuRunOptions* opts, ui
   return NQPU_OK;
 }
 
+NqpuError nqpu_debug_compile_passes(const NqpuCircuit* c, const NqpuRunOptions* opts) {
+  if (!c) return NQPU_NULL_POINTER;
+  NqpuRunOptions o{NQPU_FUSION_TILED, NQPU_SEMANTICS_STANDARD, 0, 0, 0};
+  if (opts) o = *opts;
+  if (o.semantics > NQPU_SEMANTICS_REFERENCE) return NQPU_INVALID_PARAMETER;
+  return jit_check_circuit(*Cc(c), o);
+}
+
 NqpuError nqpu_debug_emulate(const NqpuCircuit* c, const NqpuRunOptions* opts, uint32_t precision,
                              NqpuComplex* amps, uint64_t len) {
   if (!c || !amps) return NQPU_NULL_POINTER;

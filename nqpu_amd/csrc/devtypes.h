@@ -1,0 +1,80 @@
+// devtypes.h — register tile programs as the gfx950 kernels read them.
+// Shared by the host runtime, the precompiled kernels and the per-pass
+// kernels generated at run time (whose source embeds this file), so it uses
+// fixed-width integer types only and includes nothing.
+#pragma once
+
+namespace nqpu {
+
+// Register-resident tile program (k_tile_reg).  Each of the 256 threads of a
+// tile workgroup holds 2^EB amplitudes; tile bits are "thread slots" (0..7,
+// the thread index bits) or "register slots" (0..EB-1, the register index
+// bits).  map[0..7] / map[8..8+EB) give the tile-local bit held by each slot.
+enum RegOpKind : int32_t {
+  R_REMAP = 0,  // exchange through LDS: slots change from map_old to map_new
+  R_U1 = 1,     // 2x2 on register slot s0
+  R_FLIP = 2,   // X on register slot s0
+  R_D1 = 3,     // diag(m[0], m[1]) on position s0 (>= kGlobalBit: qubit s0-kGlobalBit outside the
+                //   tile; >= 16: register slot s0-16; else thread slot)
+  R_D2 = 4,     // diag(m[0..3]) on positions s0 (index bit 0), s1 (index bit 1)
+  R_U2 = 5,     // 4x4 on register slots s0 (index bit 0), s1 (index bit 1)
+  R_SWAP = 6,   // exchange register slots s0 and s1 (controlled SWAP)
+  // Dense 16x16 block on four tile bits, on the matrix cores (complex64, EB = 4
+  // only): block index bits a0, a1, a3 are register slots 0, 1, 2 and a2 is
+  // thread slot 5 (lane bit 5); register slot 3 is not in the block.  The
+  // result leaves a1 on thread slot 5 and a2 on register slot 1.  The block
+  // depends on pad (<= 2) qubits outside the tile, at positions s1 & 255 and
+  // s1 >> 8; their bits in the tile's base index select the variant, blob
+  // s0 + variant.  s0 indexes the op's operand blob (kMfmaBlobBytes each): per lane l and K-step s, the
+  // fp16 high and low parts of A[l % 32][16 s + 8 (l / 32) + i], i = 0..7,
+  // where A is the 32x32 real form of the block matrix (row 2a'+c', column
+  // 2a+c; c = 0 real, 1 imaginary part).
+  R_MFMA = 7,
+  // Exchange register slot s0 with thread slot s1 < 6 (a lane bit: DPP or
+  // lane-half swaps inside the wavefront, no LDS) or with register slot
+  // s1 - 16 (register moves).  Afterwards each slot holds the other's bit.
+  R_XCHG = 8
+};
+
+constexpr int kMfmaBlobBytes = 64 * 64;  // 64 lanes x (2 K-steps x (hi, lo) x 8 fp16)
+
+template <typename R>
+struct RegOp {
+  int32_t kind, s0, s1, pad;
+  uint32_t ct;  // controls on thread slots (bits of the thread index)
+  uint32_t cr;  // controls on register slots (bits of the register index)
+  uint8_t map_old[16];
+  uint8_t map_new[16];
+  uint64_t gc;  // controls outside the tile (global qubit mask): the op runs on tiles that satisfy them
+  R m[32];
+};
+
+// Diagonal-op positions at or above kGlobalBit name qubit (pos - kGlobalBit)
+// outside the tile; its value is the same for the whole tile.
+constexpr int kGlobalBit = 64;
+
+constexpr int kMaxTile = 14;  // maximum tile qubits (LDS: 2^13 * 16 B = 128 KiB)
+
+
+// One tile pass: tile qubits (global positions, ascending), the ops it
+// applies, and where they live in the uploaded op buffer.
+struct PassArgs {
+  int32_t K;
+  int32_t n;
+  int32_t nops;
+  int32_t op0;
+  int32_t q[kMaxTile];  // tile qubits, ascending (the tile's base index skips them)
+  // register tile pass: qubit held by each slot at load (ql) and at store
+  // (qf); slots 0..7 are thread-index bits, 8.. register-index bits
+  int32_t ql[16];
+  int32_t qf[16];
+  // nonzero: the pass holds R_MFMA ops, so amplitudes are kept multiplied by
+  // 2^15 in registers (loaded x 2^15, stored x 2^-15) and their fp16 parts
+  // stay clear of the subnormal range
+  int32_t scaled;
+  // register tile pass: bytes of LDS for R_REMAP (0 when the program has none)
+  int32_t lds_remap;
+};
+constexpr float kMfmaScale = 32768.0f;
+
+}  // namespace nqpu

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "devtypes.h"
+
 #include <complex>
 #include <cstdint>
 #include <string>
@@ -50,49 +52,6 @@ struct DevOp {
   R m[32];  // interleaved re/im, up to 4x4
 };
 
-// Register-resident tile program (k_tile_reg).  Each of the 256 threads of a
-// tile workgroup holds 2^EB amplitudes; tile bits are "thread slots" (0..7,
-// the thread index bits) or "register slots" (0..EB-1, the register index
-// bits).  map[0..7] / map[8..8+EB) give the tile-local bit held by each slot.
-enum RegOpKind : int32_t {
-  R_REMAP = 0,  // exchange through LDS: slots change from map_old to map_new
-  R_U1 = 1,     // 2x2 on register slot s0
-  R_FLIP = 2,   // X on register slot s0
-  R_D1 = 3,     // diag(m[0], m[1]) on position s0 (>= kGlobalBit: qubit s0-kGlobalBit outside the
-                //   tile; >= 16: register slot s0-16; else thread slot)
-  R_D2 = 4,     // diag(m[0..3]) on positions s0 (index bit 0), s1 (index bit 1)
-  R_U2 = 5,     // 4x4 on register slots s0 (index bit 0), s1 (index bit 1)
-  R_SWAP = 6,   // exchange register slots s0 and s1 (controlled SWAP)
-  // Dense 16x16 block on four tile bits, on the matrix cores (complex64, EB = 4
-  // only): block index bits a0, a1, a3 are register slots 0, 1, 2 and a2 is
-  // thread slot 5 (lane bit 5); register slot 3 is not in the block.  The
-  // result leaves a1 on thread slot 5 and a2 on register slot 1.  s0 indexes
-  // the op's operand blob (kMfmaBlobBytes each): per lane l and K-step s, the
-  // fp16 high and low parts of A[l % 32][16 s + 8 (l / 32) + i], i = 0..7,
-  // where A is the 32x32 real form of the block matrix (row 2a'+c', column
-  // 2a+c; c = 0 real, 1 imaginary part).
-  R_MFMA = 7
-};
-
-constexpr int kMfmaBlobBytes = 64 * 64;  // 64 lanes x (2 K-steps x (hi, lo) x 8 fp16)
-
-template <typename R>
-struct RegOp {
-  int32_t kind, s0, s1, pad;
-  uint32_t ct;  // controls on thread slots (bits of the thread index)
-  uint32_t cr;  // controls on register slots (bits of the register index)
-  uint8_t map_old[16];
-  uint8_t map_new[16];
-  uint64_t gc;  // controls outside the tile (global qubit mask): the op runs on tiles that satisfy them
-  R m[32];
-};
-
-// Diagonal-op positions at or above kGlobalBit name qubit (pos - kGlobalBit)
-// outside the tile; its value is the same for the whole tile.
-constexpr int kGlobalBit = 64;
-
-constexpr int kMaxTile = 14;  // maximum tile qubits (LDS: 2^13 * 16 B = 128 KiB)
-
 // ------------------------------------------------------ batched execution
 // k_batch: one persistent launch over many complex64 circuits.  Each
 // workgroup takes whole circuits from an atomic queue; a circuit of at most
@@ -130,20 +89,6 @@ void launch_batch(void* states, const BatchCircuit* circs, int ncirc, const Batc
                   const RegOp<float>* prog, const BatchTerm* terms, unsigned* counter, double* out,
                   int grid, hipStream_t s);
 
-// One tile pass: tile qubits (global positions, ascending), the ops it
-// applies, and where they live in the uploaded op buffer.
-struct PassArgs {
-  int32_t K;
-  int32_t n;
-  int32_t nops;
-  int32_t op0;
-  int32_t q[kMaxTile];
-  // nonzero: the pass holds R_MFMA ops, so amplitudes are kept multiplied by
-  // 2^15 in registers (loaded x 2^15, stored x 2^-15) and their fp16 parts
-  // stay clear of the subnormal range
-  int32_t scaled;
-};
-constexpr float kMfmaScale = 32768.0f;
 
 // ----------------------------------------------------------- kernel launches
 // All launches are asynchronous on `s`; R = float (c64) or double (c128).

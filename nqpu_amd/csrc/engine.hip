@@ -793,6 +793,10 @@ struct RegProgram {
   std::vector<RegOp<R>> ops;
   std::vector<std::array<cd, 256>> blocks;
   uint64_t fused_ops = 0;  // device ops inside those blocks
+  // tile bit held by each slot (0..7 thread, 8.. register) when the tile is
+  // loaded and when it is stored
+  uint8_t map0[16] = {};
+  uint8_t map1[16] = {};
 };
 
 // Fusion of dense runs into matrix-core blocks pays when the block replaces at
@@ -800,7 +804,14 @@ struct RegProgram {
 constexpr double kMfmaMinWeight = 1.0;
 
 template <typename R>
-RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false) {
+// xchg (full 256-thread tiles only): move bits between register slots and
+// lane bits with R_XCHG instead of LDS remaps wherever the thread slots that
+// change are lane bits, load the tile with the two least-used window bits on
+// the wave slots, and store it in whatever layout the program ends in, as
+// long as thread slots 0..4 hold the tile's five lowest bits again (so the
+// store is as coalesced as the load).
+RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false, bool xchg = false,
+                                int max_blobs = 1 << 20) {
   const int K = (int)p.qubits.size();  // == tb + eb
   int loc[64];
   for (int q = 0; q < 64; ++q) loc[q] = -1;
@@ -868,6 +879,64 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
     }
     prog.push_back(r);
     for (int i = 0; i < 16; ++i) map[i] = nm[i];
+  };
+  xchg = xchg && tb == 8;
+  // other: thread slot < 6, or 16 + register slot
+  auto emit_xchg = [&](int j, int other) {
+    RegOp<R> r{};
+    r.kind = R_XCHG;
+    r.s0 = j;
+    r.s1 = other;
+    prog.push_back(r);
+    std::swap(map[8 + j], map[other < 16 ? other : 8 + (other - 16)]);
+  };
+  // reach the slot assignment nm: by exchanges when the wave slots (thread
+  // slots 6, 7) keep their bits, else by one LDS remap
+  // dist[b]: how many pending items come before the next one that needs tile
+  // bit b on a register or lane-5 slot (large: never again)
+  auto remap_to = [&](const uint8_t* nm_in, const int* dist) {
+    uint8_t nm[16];
+    std::memcpy(nm, nm_in, 16);
+    if (xchg && (nm[6] != map[6] || nm[7] != map[7])) {
+      // an LDS remap is unavoidable: it may as well park on the wave slots the
+      // two bits whose next use is farthest away (Belady), so that later
+      // items reach their bits by exchanges
+      int order[16], cnt = 0;
+      for (int t = 0; t < 16; ++t)
+        if (t < tb || (t >= 8 && t < 8 + eb)) order[cnt++] = t;
+      std::stable_sort(order, order + cnt, [&](int x, int y) { return dist[nm[x]] > dist[nm[y]]; });
+      for (int w = 0; w < 2; ++w) {
+        const int want = order[w];  // slot in nm holding a bit to park
+        if (want == 6 || want == 7) continue;
+        const int ws = (order[1 - w] == 6) ? 7 : (order[1 - w] == 7 ? 6 : (w == 0 ? 6 : 7));
+        if (dist[nm[ws]] >= dist[nm[want]]) continue;
+        std::swap(nm[want], nm[ws]);
+      }
+    }
+    if (!xchg || nm[6] != map[6] || nm[7] != map[7]) {
+      emit_remap(nm);
+      return;
+    }
+    for (int i = 0; i < 6; ++i) {
+      if (nm[i] == map[i]) continue;
+      int where = -1;
+      for (int t = 0; t < 16; ++t)
+        if ((t < tb || (t >= 8 && t < 8 + eb)) && map[t] == nm[i]) where = t;
+      if (where >= 8) {
+        emit_xchg(where - 8, i);
+      } else {  // in a later lane slot: through register slot 0
+        emit_xchg(0, where);
+        emit_xchg(0, i);
+      }
+    }
+    for (int j = 0; j < eb; ++j) {
+      if (nm[8 + j] == map[8 + j]) continue;
+      for (int k = 0; k < eb; ++k)
+        if (map[8 + k] == nm[8 + j]) {
+          emit_xchg(j, 16 + k);
+          break;
+        }
+    }
   };
   auto emit = [&](const L& o) {
     if (o.kind == OP_SWAP && o.cm == 0 && o.gc == 0) {  // relabel
@@ -978,6 +1047,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       return w;
     };
     constexpr int kSeeds = 8;
+    int nvar = 0;  // operand blobs of the blocks so far (one per variant)
     while (!rem.empty()) {
       const L& first = ops[rem.front()];
       if (!fusable(first) || qmask(first) == 0) {  // runs on its own
@@ -1010,7 +1080,8 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         b.g |= gmask(ops[k]);
       }
       b.ops = taken;
-      if (bw >= kMfmaMinWeight) {
+      if (bw >= kMfmaMinWeight && nvar + (1 << __builtin_popcountll(b.g)) <= max_blobs) {
+        nvar += 1 << __builtin_popcountll(b.g);
         blks.push_back(b);
         items.push_back(Item{-1, (int)blks.size() - 1, b.q, false});
       } else {
@@ -1021,6 +1092,28 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
   } else {
     for (int k = 0; k < (int)ops.size(); ++k) items.push_back(Item{k, -1, qmask(ops[k]), false});
   }
+  if (xchg) {
+    // the two window bits first needed on register or lane-5 slots last go
+    // to the wave slots, where only an LDS remap can reach them
+    int first[32];
+    for (int b = 0; b < 32; ++b) first[b] = 1 << 20;
+    for (int c = (int)items.size() - 1; c >= 0; --c) {
+      const Item& it = items[c];
+      const uint32_t m = it.op >= 0 ? targets(ops[it.op]) : blks[it.blk].q;
+      for (int b = 0; b < K; ++b)
+        if ((m >> b) & 1) first[b] = c;
+    }
+    std::vector<int> win;
+    for (int b = 5; b < K; ++b) win.push_back(b);
+    std::stable_sort(win.begin(), win.end(), [&](int x, int y) { return first[x] > first[y]; });
+    std::vector<int> rest(win.begin() + 2, win.end());
+    std::sort(rest.begin(), rest.end());
+    map[6] = (uint8_t)win[0];
+    map[7] = (uint8_t)win[1];
+    map[5] = (uint8_t)rest[0];
+    for (int j = 0; j < eb; ++j) map[8 + j] = (uint8_t)rest[1 + j];
+  }
+  std::memcpy(out.map0, map, 16);
   // positions a block uses: block bits a0, a1, a3 on register slots 0, 1, 2 and a2 on thread slot 5
   auto block_slots = [&]() {
     return (1u << map[8]) | (1u << map[9]) | (1u << map[10]) | (1u << map[5]);
@@ -1039,7 +1132,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
     r.kind = R_MFMA;
     r.s0 = (int32_t)out.blocks.size();
     r.pad = ns;
-    for (int i = 0; i < ns; ++i) r.map_old[i] = (uint8_t)sel[i];
+    r.s1 = (ns > 0 ? sel[0] : 0) | (ns > 1 ? sel[1] << 8 : 0);
     for (int var = 0; var < (1 << ns); ++var) {
       uint64_t gset = 0;  // qubits outside the tile that are 1 in this variant
       for (int i = 0; i < ns; ++i)
@@ -1124,6 +1217,18 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       }
     }
     if (remaining == 0) break;
+    int dist[32];
+    for (int b = 0; b < 32; ++b) dist[b] = 1 << 20;
+    {
+      int c = 0;
+      for (const Item& it : items) {
+        if (it.done) continue;
+        const uint32_t req = it.op >= 0 ? targets(ops[it.op]) : blks[it.blk].q;
+        for (int b = 0; b < K; ++b)
+          if (((req >> b) & 1) && dist[b] > c) dist[b] = c;
+        ++c;
+      }
+    }
     uint8_t nm[16];
     for (int i = 0; i < 16; ++i) nm[i] = map[i];
     const Item* first = nullptr;
@@ -1146,7 +1251,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
           if ((i < tb || (i >= 8 && i < 8 + eb)) && nm[i] == lb) from = i;
         std::swap(nm[from], nm[P[pi]]);
       }
-      emit_remap(nm);
+      remap_to(nm, dist);
       continue;
     }
     uint32_t blocked = 0;
@@ -1175,9 +1280,33 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       nm[8 + j] = (uint8_t)lb;
       need &= ~(1u << lb);
     }
-    emit_remap(nm);
+    remap_to(nm, dist);
   }
-  if (std::memcmp(map, canon, 16) != 0) emit_remap(canon);
+  if (xchg) {
+    // thread slots 0..4 must hold the tile's five lowest bits (any order)
+    uint32_t low = 0;
+    for (int i = 0; i < 5; ++i) low |= 1u << out.map0[i];
+    for (int i = 0; i < 5; ++i) {
+      if ((low >> map[i]) & 1) continue;
+      int where = -1;  // prefer a register slot, then lane slot 5, then a wave slot
+      for (int t : {8, 9, 10, 11, 12, 5, 6, 7})
+        if (where < 0 && (t < tb || t < 8 + eb) && ((low >> map[t]) & 1)) where = t;
+      if (where >= 8) {
+        emit_xchg(where - 8, i);
+      } else if (where == 5) {
+        emit_xchg(0, 5);
+        emit_xchg(0, i);
+      } else {  // on a wave slot: one LDS remap back to the load layout
+        uint8_t nm[16];
+        std::memcpy(nm, out.map0, 16);
+        emit_remap(nm);
+        break;
+      }
+    }
+  } else if (std::memcmp(map, canon, 16) != 0) {
+    emit_remap(canon);
+  }
+  std::memcpy(out.map1, map, 16);
   return out;
 }
 
@@ -1351,8 +1480,27 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const Reg
       base = ((base >> q) << (q + 1)) | (base & ((1ull << q) - 1));
     }
     for (int tid = 0; tid < NTH; ++tid)
-      for (int r = 0; r < E; ++r) v[tid * E + r] = a[base | dep((uint32_t)tid | ((uint32_t)r << tb))];
+      for (int r = 0; r < E; ++r) v[tid * E + r] = a[base | dep(tmap(rp.map0, tid, r))];
     for (const RegOp<double>& op : prog) {
+      if (op.kind == R_XCHG) {
+        std::vector<cd> nv(v);
+        for (int tid = 0; tid < NTH; ++tid)
+          for (int r = 0; r < E; ++r) {
+            const int y = (r >> op.s0) & 1;
+            if (op.s1 >= 16) {  // register slots s0 and s1 - 16 trade bits
+              const int k = op.s1 - 16, z = (r >> k) & 1;
+              const int r2 = (r & ~(1 << op.s0) & ~(1 << k)) | (z << op.s0) | (y << k);
+              nv[tid * E + r] = v[tid * E + r2];
+            } else {  // register slot s0 and lane bit s1 trade bits
+              const int z = (tid >> op.s1) & 1;
+              const int tid2 = (tid & ~(1 << op.s1)) | (y << op.s1);
+              const int r2 = (r & ~(1 << op.s0)) | (z << op.s0);
+              nv[tid * E + r] = v[tid2 * E + r2];
+            }
+          }
+        v.swap(nv);
+        continue;
+      }
       if (op.kind == R_REMAP) {
         for (int tid = 0; tid < NTH; ++tid)
           for (int r = 0; r < E; ++r) lds[tmap(op.map_old, tid, r)] = v[tid * E + r];
@@ -1366,7 +1514,8 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const Reg
         // lane bit 5 on input; a0, a2, a3 = register bits 0, 1, 2 and a1 =
         // lane bit 5 on output
         int var = 0;  // variant chosen by the qubits outside the tile the block depends on
-        for (int i = 0; i < op.pad; ++i) var |= (int)((base >> op.map_old[i]) & 1) << i;
+        if (op.pad > 0) var |= (int)((base >> (op.s1 & 255)) & 1);
+        if (op.pad > 1) var |= (int)((base >> (op.s1 >> 8)) & 1) << 1;
         const std::array<cd, 256>& M = rp.blocks[op.s0 + var];
         std::vector<cd> nv(v);
         for (int w = 0; w < NTH; w += 64)
@@ -1439,7 +1588,7 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const Reg
       }
     }
     for (int tid = 0; tid < NTH; ++tid)
-      for (int r = 0; r < E; ++r) a[base | dep((uint32_t)tid | ((uint32_t)r << tb))] = v[tid * E + r];
+      for (int r = 0; r < E; ++r) a[base | dep(tmap(rp.map1, tid, r))] = v[tid * E + r];
   }
 }
 
@@ -1502,7 +1651,7 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
   const bool reg = po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13;
   for (const Pass& p : passes) {
     if (reg) {
-      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8, po.mfma), K - 8);
+      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8, po.mfma, true), K - 8);
     } else {
       std::vector<DevOp<double>> dops;
       append_dev_ops<double>(p, dops);
@@ -1513,22 +1662,30 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
 }
 
 template <typename R>
-static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb, bool mfma) {
+static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb, bool mfma, bool compile) {
   std::vector<RegOp<R>> prog;
   std::vector<uint16_t> blob;  // R_MFMA operand blobs of the whole run
   std::vector<PassArgs> args;
+  std::vector<size_t> blob_at;  // byte offset of each pass's operand blobs
   for (const Pass& p : passes) {
     PassArgs pa{};
     pa.K = (int)p.qubits.size();
     pa.n = s->n;
     pa.op0 = (int)prog.size();
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
-    RegProgram<R> pr = build_reg_program<R>(p, eb, 8, mfma);
-    const int b0 = (int)(blob.size() / (kMfmaBlobBytes / 2));
+    RegProgram<R> pr = build_reg_program<R>(p, eb, 8, mfma, true);
+    for (int slot = 0; slot < 8 + eb; ++slot) {  // thread slots 0..7, register slots 8..
+      pa.ql[slot] = p.qubits[pr.map0[slot]];
+      pa.qf[slot] = p.qubits[pr.map1[slot]];
+    }
+    blob_at.push_back(blob.size() * 2);
+    bool has_remap = false;
     for (RegOp<R>& r : pr.ops) {
       s->stats.remaps += r.kind == R_REMAP;
-      if (r.kind == R_MFMA) r.s0 += b0;
+      s->stats.exchanges += r.kind == R_XCHG;
+      has_remap = has_remap || r.kind == R_REMAP;
     }
+    pa.lds_remap = has_remap ? (int)(sizeof(R) * 2) << pa.K : 0;
     for (const auto& M : pr.blocks) {
       const size_t at = blob.size();
       blob.resize(at + kMfmaBlobBytes / 2);
@@ -1551,9 +1708,29 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
   if (!blob.empty())
     HIPTRY(copy_on(s->stream, (char*)s->opbuf.p + pb, blob.data(), blob.size() * 2, hipMemcpyHostToDevice),
            "block upload");
-  for (const PassArgs& pa : args) {
+  std::vector<hipFunction_t> fns;
+  if (compile && std::is_same<R, float>::value && eb == 4) {
+    std::vector<RegOp<float>> progf(prog.size());
+    std::memcpy(progf.data(), prog.data(), prog.size() * sizeof(RegOp<R>));
+    const NqpuError e = jit_pass_functions(progf, args, eb, &fns);
+    if (e != NQPU_OK) return e;
+  }
+  for (size_t i = 0; i < args.size(); ++i) {
+    const PassArgs& pa = args[i];
     s->timer.mark(s->stream);
-    Launch<R>::tile_pass_reg(s->amps, pa, eb, s->opbuf.p, (char*)s->opbuf.p + pb, s->stream);
+    if (!fns.empty()) {
+      void* st = s->amps;
+      const void* pg = s->opbuf.p;
+      const void* bl = (char*)s->opbuf.p + pb + blob_at[i];
+      uint64_t ntiles = 1ull << (pa.n - pa.K);
+      PassArgs a = pa;
+      void* kargs[] = {&st, &a, &ntiles, &pg, &bl};
+      const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 1ull << 20);
+      HIPTRY(hipModuleLaunchKernel(fns[i], grid, 1, 1, 256, 1, 1, (unsigned)pa.lds_remap, s->stream, kargs, nullptr),
+             "compiled register tile pass launch");
+    } else {
+      Launch<R>::tile_pass_reg(s->amps, pa, eb, s->opbuf.p, (char*)s->opbuf.p + pb + blob_at[i], s->stream);
+    }
     s->timer.mark(s->stream);
     HIPTRY(hipGetLastError(), "register tile pass launch");
   }
@@ -1568,7 +1745,7 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
   const int K = std::min(po.tile_qubits, s->n);
   if (po.registers && std::is_same<R, float>::value && K >= 11 && K <= 13) {
     s->stats.device_ops += ops.size();
-    return run_tiled_reg<R>(s, passes, K - 8, po.mfma);
+    return run_tiled_reg<R>(s, passes, K - 8, po.mfma, po.compile);
   }
   std::vector<DevOp<R>> dops;
   std::vector<PassArgs> args;
@@ -1599,6 +1776,27 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
   return NQPU_OK;
 }
 
+NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o) {
+  const PlanOptions po = plan_options(o, NQPU_PRECISION_C64);
+  const int K = std::min(po.tile_qubits, c.n);
+  if (!po.registers || K != 12) {
+    set_error("compiled passes need complex64 register tiles of 12 qubits");
+    return NQPU_INVALID_PARAMETER;
+  }
+  const std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, o.semantics == NQPU_SEMANTICS_REFERENCE));
+  std::vector<RegOp<float>> prog;
+  std::vector<PassArgs> args;
+  for (const Pass& p : plan_tiles(ops, c.n, po)) {
+    const RegProgram<float> pr = build_reg_program<float>(p, K - 8, 8, po.mfma, true);
+    PassArgs pa{};
+    pa.op0 = (int)prog.size();
+    pa.nops = (int)pr.ops.size();
+    prog.insert(prog.end(), pr.ops.begin(), pr.ops.end());
+    args.push_back(pa);
+  }
+  return jit_check_compile(prog, args, K - 8);
+}
+
 PlanOptions plan_options(const NqpuRunOptions& o, int prec) {
   PlanOptions po;
   const int kmax = (prec == NQPU_PRECISION_C64) ? 13 : 12;
@@ -1608,6 +1806,7 @@ PlanOptions plan_options(const NqpuRunOptions& o, int prec) {
   po.low_qubits = std::min(po.low_qubits, po.tile_qubits);
   po.registers = (o.flags & NQPU_RUN_LDS_TILES) == 0;
   po.mfma = (o.flags & NQPU_RUN_NO_MFMA) == 0;
+  po.compile = (o.flags & NQPU_RUN_COMPILE) != 0;
   return po;
 }
 
@@ -1646,8 +1845,11 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
     const int K = std::min(po.tile_qubits, c.n);
     if (po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13)
       for (const Pass& p : passes) {
-        const RegProgram<double> rp = build_reg_program<double>(p, K - 8, 8, po.mfma);
-        for (const RegOp<double>& r : rp.ops) st.remaps += r.kind == R_REMAP;
+        const RegProgram<double> rp = build_reg_program<double>(p, K - 8, 8, po.mfma, true);
+        for (const RegOp<double>& r : rp.ops) {
+          st.remaps += r.kind == R_REMAP;
+          st.exchanges += r.kind == R_XCHG;
+        }
         st.mfma_blocks += rp.blocks.size();
         st.mfma_gates += rp.fused_ops;
       }

@@ -1,0 +1,227 @@
+// jit.hip — straight-line register tile passes, compiled at run time.
+//
+// The interpreting kernel k_tile_reg dispatches every op of a pass's register
+// program through one loop over a switch; the amplitudes it carries from op to
+// op then pass through a merge point after every op, which costs about 64
+// register moves per op on gfx950, as much as the op's own arithmetic.  Here
+// each pass becomes its own kernel whose body calls the same device functions
+// (regtile.h) one after another with the slot numbers as template arguments,
+// so values stay in place.  Matrices, controls and remap tables are still
+// read from the uploaded program at run time: a module depends only on the
+// program's structure, and circuits that differ only in their angles reuse it.
+// Modules are compiled with hipRTC for the current device and kept for the
+// life of the process, keyed by their source text.
+#include <hip/hiprtc.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <thread>
+
+#include "runtime.h"
+#include "jit_src.inc"  // kJitPrelude: integer typedefs + devtypes.h + regtile.h (generated by build.py)
+
+namespace nqpu {
+
+namespace {
+
+struct JitModule {
+  hipModule_t mod = nullptr;
+  std::vector<hipFunction_t> fns;
+};
+
+std::mutex g_mu;
+std::map<std::string, std::unique_ptr<JitModule>> g_cache;  // kept until the process ends
+uint64_t g_compiled_passes = 0;
+
+void emit_op(std::ostringstream& o, const RegOp<float>& r, int k, int eb) {
+  switch (r.kind) {
+    case R_REMAP:
+      o << "    nqpu::reg_remap<float, float2, " << eb << ">(v, sm, ops[" << k << "], tid, true);\n";
+      return;
+    case R_MFMA:
+      o << "    { int var = 0;";
+      if (r.pad > 0) o << " var |= (int)((base >> " << (r.s1 & 255) << ") & 1);";
+      if (r.pad > 1) o << " var |= (int)((base >> " << (r.s1 >> 8) << ") & 1) << 1;";
+      o << " nqpu::reg_mfma<float2, " << eb << ">(v, blob, " << r.s0 << " + var, lane); }\n";
+      return;
+    case R_XCHG:
+      if (r.s1 < 16)
+        o << "    nqpu::reg_xchg_lane<float2, " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, lane);\n";
+      else
+        o << "    nqpu::reg_xchg_reg<float2, " << eb << ", " << r.s0 << ", " << (r.s1 - 16) << ">(v);\n";
+      return;
+    default:
+      break;
+  }
+  o << "    if ((base & " << r.gc << "ull) == " << r.gc << "ull) { const bool on = (tid & " << r.ct << "u) == "
+    << r.ct << "u; (void)on; ";
+  switch (r.kind) {
+    case R_U1: o << "nqpu::reg_u1<float, float2, " << eb << ", " << r.s0 << ">(v, ops[" << k << "], on);"; break;
+    case R_FLIP: o << "nqpu::reg_flip<float, float2, " << eb << ", " << r.s0 << ">(v, ops[" << k << "], on);"; break;
+    case R_D1:
+    case R_D2: o << "nqpu::reg_diag<float, float2, " << eb << ">(v, ops[" << k << "], tid, on, base);"; break;
+    case R_U2:
+      o << "nqpu::reg_u2<float, float2, " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, ops[" << k << "], on);";
+      break;
+    case R_SWAP:
+      o << "nqpu::reg_swap<float, float2, " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, ops[" << k << "], on);";
+      break;
+    default: break;
+  }
+  o << " }\n";
+}
+
+std::string kernel_source(int idx, const std::vector<RegOp<float>>& prog, int op0, int nops, int eb) {
+  std::ostringstream o;
+  o << "extern \"C\" __global__ __launch_bounds__(256, 1) void nqpu_pass_" << idx
+    << "(float2* __restrict__ st, nqpu::PassArgs pa, uint64_t ntiles,\n"
+       "    const nqpu::RegOp<float>* __restrict__ prog, const unsigned char* __restrict__ blob) {\n"
+       "  const nqpu::RegOp<float>* ops = prog + pa.op0;\n"
+       "  nqpu::reg_tile_pass<float, float2, "
+    << eb << ">(st, pa, ntiles, [&](float2 (&v)[" << (1 << eb)
+    << "], float2* sm, uint64_t base, int tid) __attribute__((always_inline)) {\n"
+       "    const int lane = tid & 63; (void)lane; (void)sm; (void)ops; (void)blob;\n";
+  for (int k = 0; k < nops; ++k) emit_op(o, prog[op0 + k], k, eb);
+  o << "  });\n}\n";
+  return o.str();
+}
+
+NqpuError compile_code(const std::string& src, std::vector<char>* code) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "nqpu_passes.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+    set_error("hipRTC: could not create the pass program");
+    return NQPU_DEVICE_ERROR;
+  }
+  // the same code generation settings as the precompiled kernels
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-mllvm", "-amdgpu-mfma-vgpr-form",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  if (rc != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    set_error(std::string("hipRTC compilation of register tile passes failed: ") + hiprtcGetErrorString(rc) +
+              "\n" + log.substr(0, 4000));
+    return NQPU_DEVICE_ERROR;
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  code->resize(cs);
+  hiprtcGetCode(prog, code->data());
+  hiprtcDestroyProgram(&prog);
+  return NQPU_OK;
+}
+
+NqpuError compile_module(const std::string& src, int nfn, JitModule* out) {
+  std::vector<char> code;
+  const NqpuError e = compile_code(src, &code);
+  if (e != NQPU_OK) return e;
+  if (hipModuleLoadData(&out->mod, code.data()) != hipSuccess) {
+    set_error("hipModuleLoadData failed for the compiled register tile passes");
+    return NQPU_DEVICE_ERROR;
+  }
+  out->fns.resize(nfn);
+  for (int i = 0; i < nfn; ++i) {
+    const std::string name = "nqpu_pass_" + std::to_string(i);
+    if (hipModuleGetFunction(&out->fns[i], out->mod, name.c_str()) != hipSuccess) {
+      set_error("hipModuleGetFunction failed for " + name);
+      return NQPU_DEVICE_ERROR;
+    }
+  }
+  return NQPU_OK;
+}
+
+constexpr int kPassesPerModule = 8;
+
+std::vector<std::string> module_sources(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args,
+                                        int eb) {
+  const int np = (int)args.size(), per = kPassesPerModule;
+  std::vector<std::string> srcs((np + per - 1) / per);
+  for (int m = 0; m < (int)srcs.size(); ++m) {
+    std::ostringstream o;
+    o << kJitPrelude;
+    for (int i = m * per; i < std::min(np, (m + 1) * per); ++i)
+      o << kernel_source(i - m * per, prog, args[i].op0, args[i].nops, eb);
+    srcs[m] = o.str();
+  }
+  return srcs;
+}
+
+}  // namespace
+
+NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb) {
+  for (const std::string& src : module_sources(prog, args, eb)) {
+    std::vector<char> code;
+    const NqpuError e = compile_code(src, &code);
+    if (e != NQPU_OK) return e;
+  }
+  return NQPU_OK;
+}
+
+NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb,
+                             std::vector<hipFunction_t>* fns) {
+  // passes are split over a few modules compiled on parallel threads
+  const int np = (int)args.size();
+  const int per = kPassesPerModule;
+  const std::vector<std::string> srcs = module_sources(prog, args, eb);
+  const int nmod = (int)srcs.size();
+  std::vector<JitModule*> mods(nmod, nullptr);
+  std::vector<int> todo;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (int m = 0; m < nmod; ++m) {
+      auto it = g_cache.find(srcs[m]);
+      if (it != g_cache.end()) mods[m] = it->second.get();
+      else todo.push_back(m);
+    }
+  }
+  if (!todo.empty()) {
+    std::vector<std::unique_ptr<JitModule>> built(todo.size());
+    std::vector<NqpuError> errs(todo.size(), NQPU_OK);
+    std::vector<std::string> msgs(todo.size());
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return NQPU_DEVICE_ERROR;
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < todo.size(); ++t) {
+      th.emplace_back([&, t] {
+        (void)hipSetDevice(dev);
+        built[t].reset(new JitModule);
+        const int m = todo[t];
+        const int nfn = std::min(np, (m + 1) * per) - m * per;
+        errs[t] = compile_module(srcs[m], nfn, built[t].get());
+        if (errs[t] != NQPU_OK) msgs[t] = last_error();
+      });
+    }
+    for (auto& x : th) x.join();
+    for (size_t t = 0; t < todo.size(); ++t)
+      if (errs[t] != NQPU_OK) {
+        set_error(msgs[t]);
+        return errs[t];
+      }
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (size_t t = 0; t < todo.size(); ++t) {
+      const int m = todo[t];
+      auto& slot = g_cache[srcs[m]];
+      if (!slot) {
+        slot = std::move(built[t]);
+        g_compiled_passes += slot->fns.size();
+      }
+      mods[m] = slot.get();
+    }
+  }
+  fns->resize(np);
+  for (int i = 0; i < np; ++i) (*fns)[i] = mods[i / per]->fns[i % per];
+  return NQPU_OK;
+}
+
+uint64_t jit_compiled_passes() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_compiled_passes;
+}
+
+}  // namespace nqpu

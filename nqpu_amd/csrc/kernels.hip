@@ -14,6 +14,7 @@
 //    back: one HBM read + write of the state for any number of gates.
 // Reductions accumulate in FP64 whatever the amplitude precision.
 #include "engine.h"
+#include "regtile.h"
 
 #include <algorithm>
 
@@ -23,59 +24,12 @@ template <typename R> struct V2;
 template <> struct V2<float> { using T = float2; };
 template <> struct V2<double> { using T = double2; };
 
-template <typename T> __device__ __forceinline__ T cmul_(T a, T b) {
-  T r;
-  r.x = a.x * b.x - a.y * b.y;
-  r.y = a.x * b.y + a.y * b.x;
-  return r;
-}
-// m0*x + m1*y for complex m0, m1 given as (re, im) pairs
-template <typename R, typename T>
-__device__ __forceinline__ T mac2(R m0r, R m0i, R m1r, R m1i, T x, T y) {
-  T r;
-  r.x = m0r * x.x - m0i * x.y + m1r * y.x - m1i * y.y;
-  r.y = m0r * x.y + m0i * x.x + m1r * y.y + m1i * y.x;
-  return r;
-}
 
-__device__ __forceinline__ uint64_t ins0(uint64_t v, int q) {
-  uint64_t lo = v & ((1ull << q) - 1);
-  return ((v >> q) << (q + 1)) | lo;
-}
 __device__ __forceinline__ uint32_t ins0u(uint32_t v, int q) {
   uint32_t lo = v & ((1u << q) - 1);
   return ((v >> q) << (q + 1)) | lo;
 }
 
-// Non-temporal state access for full-state passes: every amplitude is read
-// once and written once per pass, so the lines are not kept in the caches.
-template <typename T>
-struct NatVec2;
-template <>
-struct NatVec2<float2> {
-  typedef float type __attribute__((ext_vector_type(2)));
-};
-template <>
-struct NatVec2<double2> {
-  typedef double type __attribute__((ext_vector_type(2)));
-};
-template <typename T>
-__device__ __forceinline__ T load_stream(const T* p) {
-  typedef typename NatVec2<T>::type V;
-  const V x = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
-  T r;
-  r.x = x[0];
-  r.y = x[1];
-  return r;
-}
-template <typename T>
-__device__ __forceinline__ void store_stream(T* p, const T& v) {
-  typedef typename NatVec2<T>::type V;
-  const V x = {v.x, v.y};
-  __builtin_nontemporal_store(x, reinterpret_cast<V*>(p));
-}
-
-// Sorted bit positions with fixed values (targets and controls of an op).
 struct Fixed {
   int32_t nf;
   int32_t p[8];
@@ -447,404 +401,17 @@ __global__ __launch_bounds__(NT) void k_tile_small(T* __restrict__ st, PassArgs 
   }
 }
 
-// ---------------------------------------------------- register tile pass
-//
-// Each thread keeps E = 2^EB amplitudes of the tile in registers: amplitude
-// r of thread tid is tile element t = sum_i bit_i(tid) << map[i] +
-// sum_j bit_j(r) << map[8 + j].  A gate whose targets are register slots is
-// applied in registers; controls and diagonal factors may sit on any slot
-// (they only select or scale elements); everything else waits for an
-// R_REMAP, which moves the tile through LDS into a new slot assignment.
-// LDS addresses are XOR-swizzled so that lanes spread over banks whichever
-// tile bits they vary.
-
-__device__ __forceinline__ uint32_t lds_swz(uint32_t t) { return t ^ (((t >> 5) ^ (t >> 10)) & 31u); }
-
-template <typename R, typename T, int EB, int J>
-__device__ __forceinline__ void reg_u1(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
-  const R m0r = op.m[0], m0i = op.m[1], m1r = op.m[2], m1i = op.m[3];
-  const R m2r = op.m[4], m2i = op.m[5], m3r = op.m[6], m3i = op.m[7];
-  const uint32_t cr = op.cr;
-  if (op.ct == 0 && cr == 0) {  // uncontrolled: straight-line math
-#pragma unroll
-    for (int r0 = 0; r0 < (1 << EB); ++r0) {
-      if (r0 & (1 << J)) continue;
-      const int r1 = r0 | (1 << J);
-      T x = v[r0], y = v[r1];
-      v[r0] = mac2<R>(m0r, m0i, m1r, m1i, x, y);
-      v[r1] = mac2<R>(m2r, m2i, m3r, m3i, x, y);
-    }
-    return;
-  }
-#pragma unroll
-  for (int r0 = 0; r0 < (1 << EB); ++r0) {
-    if (r0 & (1 << J)) continue;
-    if ((r0 & cr) != cr) continue;
-    const int r1 = r0 | (1 << J);
-    T x = v[r0], y = v[r1];
-    T nx = mac2<R>(m0r, m0i, m1r, m1i, x, y);
-    T ny = mac2<R>(m2r, m2i, m3r, m3i, x, y);
-    v[r0] = on ? nx : x;
-    v[r1] = on ? ny : y;
-  }
-}
-
-template <typename R, typename T, int EB, int J>
-__device__ __forceinline__ void reg_flip(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
-  const uint32_t cr = op.cr;
-  if (op.ct == 0) {  // controls only on register slots: a uniform permutation
-#pragma unroll
-    for (int r0 = 0; r0 < (1 << EB); ++r0) {
-      if (r0 & (1 << J)) continue;
-      if ((r0 & cr) != cr) continue;
-      const int r1 = r0 | (1 << J);
-      T x = v[r0];
-      v[r0] = v[r1];
-      v[r1] = x;
-    }
-    return;
-  }
-#pragma unroll
-  for (int r0 = 0; r0 < (1 << EB); ++r0) {
-    if (r0 & (1 << J)) continue;
-    if ((r0 & cr) != cr) continue;
-    const int r1 = r0 | (1 << J);
-    T x = v[r0], y = v[r1];
-    v[r0] = on ? y : x;
-    v[r1] = on ? x : y;
-  }
-}
-
-template <typename R, typename T, int EB, int J1, int J2>
-__device__ __forceinline__ void reg_u2(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
-  const uint32_t cr = op.cr;
-#pragma unroll
-  for (int r = 0; r < (1 << EB); ++r) {
-    if ((r & (1 << J1)) || (r & (1 << J2))) continue;
-    if ((r & cr) != cr) continue;
-    const int idx[4] = {r, r | (1 << J1), r | (1 << J2), r | (1 << J1) | (1 << J2)};
-    T x[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) x[s] = v[idx[s]];
-#pragma unroll
-    for (int row = 0; row < 4; ++row) {
-      T acc;
-      acc.x = 0;
-      acc.y = 0;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const R mr = op.m[2 * (4 * row + c)], mi = op.m[2 * (4 * row + c) + 1];
-        acc.x += mr * x[c].x - mi * x[c].y;
-        acc.y += mr * x[c].y + mi * x[c].x;
-      }
-      v[idx[row]] = on ? acc : x[row];
-    }
-  }
-}
-
-template <typename R, typename T, int EB, int J1, int J2>
-__device__ __forceinline__ void reg_swap(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
-  const uint32_t cr = op.cr;
-#pragma unroll
-  for (int r = 0; r < (1 << EB); ++r) {
-    if (!(r & (1 << J1)) || (r & (1 << J2))) continue;
-    if ((r & cr) != cr) continue;
-    const int r2 = r ^ (1 << J1) ^ (1 << J2);
-    T x = v[r], y = v[r2];
-    v[r] = on ? y : x;
-    v[r2] = on ? x : y;
-  }
-}
-
-template <typename R, typename T, int EB>
-__device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, int tid, bool on, uint64_t base) {
-  T p0, p1, p2, p3;
-  p0.x = op.m[0]; p0.y = op.m[1];
-  p1.x = op.m[2]; p1.y = op.m[3];
-  p2.x = op.m[4]; p2.y = op.m[5];
-  p3.x = op.m[6]; p3.y = op.m[7];
-  const bool two = op.kind == R_D2;
-  int a = op.s0, b = two ? op.s1 : 16;
-  if (!two) {  // a single factor: the table does not depend on position b
-    p2 = p0;
-    p3 = p1;
-  }
-  // a factor on a qubit outside the tile is a fixed bit here: fold it into
-  // the phase table, leaving a diagonal on the other position (or a scalar,
-  // applied through register slot 0 with equal phases)
-  if (b >= kGlobalBit) {
-    if ((base >> (b - kGlobalBit)) & 1) {
-      p0 = p2;
-      p1 = p3;
-    }
-    b = 16;
-    p2 = p0;
-    p3 = p1;
-  }
-  if (a >= kGlobalBit) {
-    if ((base >> (a - kGlobalBit)) & 1) {
-      p0 = p1;
-      p2 = p3;
-    }
-    a = 16;
-    p1 = p0;
-    p3 = p2;
-  }
-  const bool areg = a >= 16, breg = b >= 16;
-  const int as = a & 15, bs = b & 15;
-  const uint32_t cr = op.cr;
-  if (op.ct == 0 && areg && breg) {
-    // every factor depends on register bits only: the phase of each element
-    // is chosen per register index, uniformly across the wavefront
-#pragma unroll
-    for (int r = 0; r < (1 << EB); ++r) {
-      if ((r & cr) != cr) continue;
-      const bool ba = (r >> as) & 1;
-      const bool bb = (r >> bs) & 1;
-      T ph;
-      ph.x = bb ? (ba ? p3.x : p2.x) : (ba ? p1.x : p0.x);
-      ph.y = bb ? (ba ? p3.y : p2.y) : (ba ? p1.y : p0.y);
-      v[r] = cmul_(ph, v[r]);
-    }
-    return;
-  }
-  const bool ta = (tid >> as) & 1, tb = (tid >> bs) & 1;
-#pragma unroll
-  for (int r = 0; r < (1 << EB); ++r) {
-    if ((r & cr) != cr) continue;
-    const bool ba = areg ? ((r >> as) & 1) : ta;
-    const bool bb = breg ? ((r >> bs) & 1) : tb;
-    T lo, hi, ph;
-    lo.x = ba ? p1.x : p0.x;
-    lo.y = ba ? p1.y : p0.y;
-    hi.x = ba ? p3.x : p2.x;
-    hi.y = ba ? p3.y : p2.y;
-    ph.x = bb ? hi.x : lo.x;
-    ph.y = bb ? hi.y : lo.y;
-    T nv = cmul_(ph, v[r]);
-    v[r] = on ? nv : v[r];
-  }
-}
-
-// `active` is false for threads beyond the tile's thread slots (small tiles
-// in batched execution); they only take part in the barriers.
-template <typename R, typename T, int EB>
-__device__ __forceinline__ void reg_remap(T (&v)[1 << EB], T* sm, const RegOp<R>& op, int tid, bool active) {
-  // lds_swz is linear over GF(2), and the thread and register parts of a tile
-  // index occupy different bits, so the swizzled byte offset of register r is
-  // the thread's part XOR a combination of per-register-slot parts that is
-  // the same for every lane: one vector XOR per register.
-  char* lds = reinterpret_cast<char*>(sm);
-  uint32_t tp = 0, rs[EB];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) tp |= (uint32_t)((tid >> i) & 1) << op.map_old[i];
-#pragma unroll
-  for (int j = 0; j < EB; ++j) rs[j] = lds_swz(1u << op.map_old[8 + j]) * (uint32_t)sizeof(T);
-  uint32_t ts = lds_swz(tp) * (uint32_t)sizeof(T);
-  if (active) {
-#pragma unroll
-    for (int r = 0; r < (1 << EB); ++r) {
-      uint32_t u = 0;
-#pragma unroll
-      for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) u ^= rs[j];
-      *reinterpret_cast<T*>(lds + (ts ^ u)) = v[r];
-    }
-  }
-  __syncthreads();
-  tp = 0;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) tp |= (uint32_t)((tid >> i) & 1) << op.map_new[i];
-#pragma unroll
-  for (int j = 0; j < EB; ++j) rs[j] = lds_swz(1u << op.map_new[8 + j]) * (uint32_t)sizeof(T);
-  ts = lds_swz(tp) * (uint32_t)sizeof(T);
-  if (active) {
-#pragma unroll
-    for (int r = 0; r < (1 << EB); ++r) {
-      uint32_t u = 0;
-#pragma unroll
-      for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) u ^= rs[j];
-      v[r] = *reinterpret_cast<const T*>(lds + (ts ^ u));
-    }
-  }
-  __syncthreads();
-}
-
-#define NQPU_SLOT_CASES(FN, ...)                                      \
-  switch (op.s0) {                                                    \
-    case 0: FN<R, T, EB, 0>(__VA_ARGS__); break;                      \
-    case 1: if constexpr (EB > 1) FN<R, T, EB, 1>(__VA_ARGS__); break; \
-    case 2: if constexpr (EB > 2) FN<R, T, EB, 2>(__VA_ARGS__); break; \
-    case 3: if constexpr (EB > 3) FN<R, T, EB, 3>(__VA_ARGS__); break; \
-    case 4: if constexpr (EB > 4) FN<R, T, EB, 4>(__VA_ARGS__); break; \
-    default: break;                                                   \
-  }
-
-template <typename R, typename T, int EB, int J1>
-__device__ __forceinline__ void reg_pair_op(T (&v)[1 << EB], const RegOp<R>& op, bool on, bool is_u2) {
-#define NQPU_J2(J2)                                                           \
-  case J2:                                                                    \
-    if constexpr (J2 < EB && J2 != J1) {                                      \
-      if (is_u2) reg_u2<R, T, EB, J1, J2>(v, op, on);                         \
-      else reg_swap<R, T, EB, J1, J2>(v, op, on);                             \
-    }                                                                         \
-    break;
-  switch (op.s1) {
-    NQPU_J2(0) NQPU_J2(1) NQPU_J2(2) NQPU_J2(3) NQPU_J2(4)
-    default: break;
-  }
-#undef NQPU_J2
-}
-
-typedef _Float16 nqpu_half8 __attribute__((ext_vector_type(8)));
-typedef _Float16 nqpu_half2 __attribute__((ext_vector_type(2)));
-typedef float nqpu_float2v __attribute__((ext_vector_type(2)));
-typedef float nqpu_f32x16 __attribute__((ext_vector_type(16)));
-
-// x - (float)h for the low (HI = false) or high half of the packed fp16 pair
-// h, exact: one mixed-precision FMA, -h * 1 + x.  hipcc does not form
-// v_fma_mix_f32 from the plain expression (it emits a convert and a subtract).
-template <bool HI>
-__device__ __forceinline__ float sub_half(unsigned h, float x) {
-  float r;
-  if constexpr (HI)
-    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
-  else
-    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(x));
-  return r;
-}
-
-// R_MFMA: a dense 16x16 complex block on the matrix cores.  Per register
-// half e (register bit 3), the wavefront's 32 columns (lane bits 0..4) are 32
-// vectors of 16 amplitudes: K-step s holds the 8 reals of a3 = s, a2 = lane
-// bit 5, (a0, a1) = register bits 0, 1.  Amplitudes (already scaled by
-// kMfmaScale) and the matrix are split into fp16 high and low parts (high =
-// round to nearest, low = the exact fp32 remainder rounded to nearest); the
-// products hi*hi + hi*lo + lo*hi accumulate in FP32, which leaves a relative
-// error of about 2^-22 per element.  The split costs four vector instructions
-// per amplitude: two packed converts and two mixed-precision FMAs.  The 16
-// accumulator values of lane l are rows 8j + 4 (l / 32) + i: exactly the
-// amplitudes a0 = i / 2, a1 = lane bit 5, a2 = j & 1, a3 = j >> 1 in register
-// order (a0, a2, a3).
-template <typename T, int EB>
-__device__ __forceinline__ void reg_mfma(T (&v)[1 << EB], const unsigned char* blob, int s0, int lane) {
-  if constexpr (std::is_same<T, float2>::value && EB == 4) {
-    const nqpu_half8* ap = reinterpret_cast<const nqpu_half8*>(blob + (size_t)s0 * kMfmaBlobBytes + lane * 64);
-    const nqpu_half8 ah0 = ap[0], al0 = ap[1], ah1 = ap[2], al1 = ap[3];
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      nqpu_half8 bh[2], bl[2];
-#pragma unroll
-      for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float2 x = v[8 * e + 4 * s + q];
-          const nqpu_half2 h = __builtin_convertvector((nqpu_float2v){x.x, x.y}, nqpu_half2);
-          const unsigned hb = __builtin_bit_cast(unsigned, h);
-          const nqpu_half2 l =
-              __builtin_convertvector((nqpu_float2v){sub_half<false>(hb, x.x), sub_half<true>(hb, x.y)}, nqpu_half2);
-          bh[s][2 * q] = h[0];
-          bh[s][2 * q + 1] = h[1];
-          bl[s][2 * q] = l[0];
-          bl[s][2 * q + 1] = l[1];
-        }
-      nqpu_f32x16 d = {};
-      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(al0, bh[0], d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bl[0], d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(al1, bh[1], d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bl[1], d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah0, bh[0], d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah1, bh[1], d, 0, 0, 0);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        v[8 * e + j].x = d[2 * j];
-        v[8 * e + j].y = d[2 * j + 1];
-      }
-    }
-  }
-}
-
-// Run a register-tile program on v (the standalone pass and the batched
-// kernel share it).
-template <typename R, typename T, int EB>
-__device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<R>* ops, int nops, int tid,
-                                            bool active, uint64_t base, const unsigned char* blob) {
-  for (int o = 0; o < nops; ++o) {
-    const RegOp<R>& op = ops[o];
-    // controls outside the tile: uniform over the workgroup (remaps have none)
-    if ((base & op.gc) != op.gc) continue;
-    const bool on = (tid & op.ct) == op.ct;
-    switch (op.kind) {
-      case R_REMAP: reg_remap<R, T, EB>(v, sm, op, tid, active); break;
-      case R_MFMA: {
-        // the block's matrix variant: bits of the tile's base index at the
-        // qubits outside the tile it depends on (uniform over the workgroup)
-        int var = 0;
-        for (int i = 0; i < op.pad; ++i) var |= (int)((base >> op.map_old[i]) & 1) << i;
-        reg_mfma<T, EB>(v, blob, op.s0 + var, tid & 63);
-        break;
-      }
-      case R_U1: NQPU_SLOT_CASES(reg_u1, v, op, on) break;
-      case R_FLIP: NQPU_SLOT_CASES(reg_flip, v, op, on) break;
-      case R_D1:
-      case R_D2: reg_diag<R, T, EB>(v, op, tid, on, base); break;
-      case R_U2: NQPU_SLOT_CASES(reg_pair_op, v, op, on, true) break;
-      case R_SWAP: NQPU_SLOT_CASES(reg_pair_op, v, op, on, false) break;
-      default: break;
-    }
-  }
-}
-
+// Register tile pass interpreting the pass's register program.
 template <typename R, typename T, int EB>
 __global__ __launch_bounds__(256, 1) void k_tile_reg(T* __restrict__ st, PassArgs pa, uint64_t ntiles,
                                                   const RegOp<R>* __restrict__ prog,
                                                   const unsigned char* __restrict__ blob) {
-  constexpr int E = 1 << EB, NT = 256;
-  extern __shared__ unsigned char smraw[];
-  T* sm = reinterpret_cast<T*>(smraw);
-  const int tid = threadIdx.x;
-  uint64_t dlo = 0;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) dlo |= (uint64_t)((tid >> j) & 1) << pa.q[j];
   const RegOp<R>* ops = prog + pa.op0;
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    uint64_t base = tile;
-#pragma unroll
-    for (int j = 0; j < 8 + EB; ++j) base = ins0(base, pa.q[j]);
-    T v[E];
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-      uint64_t d = base | dlo;
-#pragma unroll
-      for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
-      v[r] = load_stream(st + d);
-    }
-    if (pa.scaled) {
-#pragma unroll
-      for (int r = 0; r < E; ++r) {
-        v[r].x *= (R)kMfmaScale;
-        v[r].y *= (R)kMfmaScale;
-      }
-    }
+  // forced inline: a called body would keep the amplitudes in scratch memory
+  reg_tile_pass<R, T, EB>(st, pa, ntiles, [&](T (&v)[1 << EB], T* sm, uint64_t base, int tid)
+                                              __attribute__((always_inline)) {
     reg_program<R, T, EB>(v, sm, ops, pa.nops, tid, true, base, blob);
-    if (pa.scaled) {
-#pragma unroll
-      for (int r = 0; r < E; ++r) {
-        v[r].x *= (R)(1.0f / kMfmaScale);
-        v[r].y *= (R)(1.0f / kMfmaScale);
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-      uint64_t d = base | dlo;
-#pragma unroll
-      for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) d |= 1ull << pa.q[8 + j];
-      store_stream(st + d, v[r]);
-    }
-  }
+  });
 }
 
 // ------------------------------------------------------------- reductions
@@ -1383,7 +950,7 @@ template <typename R, int EB>
 static void launch_tile_reg(void* st, const PassArgs& pa, const void* prog, const void* blob, hipStream_t s) {
   const uint64_t ntiles = 1ull << (pa.n - pa.K);
   const uint64_t g = ntiles < (1ull << 20) ? ntiles : (1ull << 20);
-  const size_t lds = sizeof(T_) << pa.K;
+  const size_t lds = (size_t)pa.lds_remap;
   auto fn = k_tile_reg<R, T_, EB>;
   static bool attr_set = false;
   if (!attr_set) {

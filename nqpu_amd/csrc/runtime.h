@@ -59,6 +59,9 @@ struct PlanOptions {
   // fuse runs of gates on four tile bits into dense blocks applied on the
   // matrix cores (complex64 register tiles)
   bool mfma = true;
+  // run complex64 register tile passes as straight-line kernels compiled at
+  // run time (jit.hip) instead of the interpreting kernel
+  bool compile = false;
 };
 
 // lowering and planning (pure host code; unit-testable without a device)
@@ -73,6 +76,18 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec);
 // Host emulation of the tiled execution (plan, per-pass programs, kernel
 // semantics) on a complex128 vector.
 NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std::vector<cd>& a);
+
+// Kernels of the given register tile passes (EB = eb, complex64) as
+// straight-line code, compiled with hipRTC for the current device on first use
+// and cached for the life of the process by program structure (jit.hip).
+NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb,
+                             std::vector<hipFunction_t>* fns);
+// Register tile passes compiled so far in this process.
+uint64_t jit_compiled_passes();
+// Generate and compile (no device needed, nothing loaded or cached) the
+// straight-line kernels nqpu_run with NQPU_RUN_COMPILE would use for c.
+NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o);
+NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb);
 
 struct DeviceScratch {
   void* p = nullptr;

@@ -116,10 +116,11 @@ class NativeCircuit:
         L.check(lib.nqpu_circuit_add_gates(self.ptr, ops, len(circuit.gates)), "nqpu_circuit_add_gates")
 
     def plan(self, precision="c64", fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
-             lds_tiles=False, mfma=True):
+             lds_tiles=False, mfma=True, compile=False):
         lib = L.load()
         st = L.NqpuRunStats()
-        o = run_options(fusion, semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma)
+        o = run_options(fusion, semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma,
+                        compile=compile)
         L.check(lib.nqpu_circuit_plan(self.ptr, ct.byref(o), _prec(precision), ct.byref(st)), "plan")
         return _stats(st)
 
@@ -135,6 +136,13 @@ class NativeCircuit:
         L.check(L.load().nqpu_debug_emulate(self.ptr, ct.byref(o), _prec(precision), a.ctypes.data, len(a)),
                 "emulate")
         return a
+
+    def check_compile(self, semantics="standard", mfma=True):
+        """Generate and compile (hipRTC, no device) the straight-line pass
+        kernels a complex64 run with compile=True would use; raises with the
+        compiler log on failure."""
+        o = run_options("tiled", semantics, mfma=mfma, compile=True)
+        L.check(L.load().nqpu_debug_compile_passes(self.ptr, ct.byref(o)), "compile passes")
 
     def fusion_counts(self):
         lib = L.load()
@@ -160,23 +168,25 @@ _SEMANTICS = {"standard": L.SEMANTICS_STANDARD, "reference": L.SEMANTICS_REFEREN
 
 
 def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
-                time_passes=False, lds_tiles=False, mfma=True) -> L.NqpuRunOptions:
+                time_passes=False, lds_tiles=False, mfma=True, compile=False) -> L.NqpuRunOptions:
     """mfma=False keeps every gate of complex64 register tiles on the vector
-    ALUs (no fused blocks on the matrix cores)."""
+    ALUs (no fused blocks on the matrix cores); compile=True runs complex64
+    register tile passes as straight-line kernels compiled at run time."""
     o = L.NqpuRunOptions()
     o.fusion = _FUSION[fusion]
     o.semantics = _SEMANTICS[semantics]
     o.tile_qubits = tile_qubits
     o.low_qubits = low_qubits
     o.flags = ((L.RUN_TIME_PASSES if time_passes else 0) | (L.RUN_LDS_TILES if lds_tiles else 0)
-               | (0 if mfma else L.RUN_NO_MFMA))
+               | (0 if mfma else L.RUN_NO_MFMA) | (L.RUN_COMPILE if compile else 0))
     return o
 
 
 def _stats(st: L.NqpuRunStats) -> dict:
     return {"gates": st.gates, "device_ops": st.device_ops, "passes": st.passes, "bytes": st.bytes_moved,
             "pass_ms": st.pass_ms, "timed_passes": st.timed_passes, "remaps": st.remaps,
-            "mfma_blocks": st.mfma_blocks, "mfma_gates": st.mfma_gates}
+            "mfma_blocks": st.mfma_blocks, "mfma_gates": st.mfma_gates,
+            "exchanges": st.exchanges}
 
 
 def _prec(p) -> int:
@@ -248,10 +258,10 @@ class StateVector:
 
     # --- execution
     def run(self, circuit: Circuit | NativeCircuit, fusion="tiled", semantics="standard",
-            tile_qubits=0, low_qubits=0, time_passes=False, lds_tiles=False, mfma=True) -> dict:
+            tile_qubits=0, low_qubits=0, time_passes=False, lds_tiles=False, mfma=True, compile=False) -> dict:
         lib = L.load()
         nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
-        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles, mfma)
+        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles, mfma, compile)
         L.check(lib.nqpu_run(nc.ptr, self.ptr, ct.byref(o)), "nqpu_run")
         return self.last_stats()
 

@@ -101,6 +101,40 @@ def test_complex64_tile_kernels(oracle, tile, low, lds):
     assert infidelity(out, exp) < 1e-6
 
 
+@pytest.mark.parametrize("name", ["mixed16", "qft_trotter15", "brick14"])
+def test_compiled_passes_match_oracle(oracle, name):
+    """complex64 register tiles run as straight-line kernels compiled at run
+    time (compile=True): the same register programs as the interpreting
+    kernel, every op kind, controls and diagonal factors outside the tile."""
+    from tests.test_planner_emulation import CIRCUITS, random_state
+    c = CIRCUITS[name]()
+    n = c.num_qubits
+    a0 = random_state(n, 5)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    for compiled in (True, False):
+        with StateVector(n, "c64") as sv:
+            sv.set_amplitudes(a0)
+            st = sv.run(c, compile=compiled)
+            out = sv.amplitudes()
+        assert st["passes"] > 0
+        assert np.max(np.abs(out - exp)) < 2e-5, compiled
+        assert infidelity(out, exp) < 1e-6, compiled
+
+
+def test_compiled_passes_c1_bitwise_equal_interpreted(c1):
+    """The compiled and interpreted kernels execute the same operations in the
+    same order on the same registers: identical complex64 results."""
+    c, _ = c1
+    outs = []
+    for compiled in (False, True, True):  # the second compiled run reuses the cached kernels
+        with StateVector(20, "c64") as sv:
+            sv.run(c, compile=compiled)
+            outs.append(sv.amplitudes())
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[1], outs[2])
+
+
 @pytest.fixture(scope="module")
 def c1(oracle):
     """20 qubits, 20 U3 + CNOT-brickwork layers, seed 101; the oracle state."""
