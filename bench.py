@@ -36,9 +36,10 @@ def measured_traffic(args):
     kernel and workload shape."""
     if args.lds or args.tile not in (0, 12) or args.low not in (0, 5) or args.qubits != 30 or args.depth != 40:
         return None
+    key = "k_tile_reg" if args.interpret else "nqpu_pass_compiled"
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            return json.load(f)["k_tile_reg"]["bytes_per_launch"]
+            return json.load(f)[key]["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 

@@ -1228,6 +1228,11 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
           if (((req >> b) & 1) && dist[b] > c) dist[b] = c;
         ++c;
       }
+      // with exchanges, the bits loaded on lane slots 0..4 must be back on
+      // lane slots at the store: a use after the last item
+      if (xchg)
+        for (int i = 0; i < 5; ++i)
+          if (dist[out.map0[i]] > c) dist[out.map0[i]] = c;
     }
     uint8_t nm[16];
     for (int i = 0; i < 16; ++i) nm[i] = map[i];
@@ -1237,6 +1242,45 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         first = &it;
         break;
       }
+    if (first->op < 0 && xchg && eb == 4) {
+      // the block's bits must sit on register slots 0..2 and lane slot 5.
+      // Each bit elsewhere comes into a register slot whose bit is needed
+      // last (one exchange); lane 5 is refilled from a register only when all
+      // four block bits ended up in registers (one lane-half swap); register
+      // order is fixed last (register renaming).
+      const uint32_t B = blks[first->blk].q;
+      auto find = [&](int lb) {
+        for (int i = 0; i < 16; ++i)
+          if ((i < tb || (i >= 8 && i < 8 + eb)) && nm[i] == lb) return i;
+        return -1;
+      };
+      for (uint32_t miss = B; miss; miss &= miss - 1) {
+        const int lb = __builtin_ctz(miss);
+        const int from = find(lb);
+        if (from == 5 || from >= 8) continue;
+        int j = -1;
+        for (int r = 8; r < 12; ++r)
+          if (!((B >> nm[r]) & 1) && (j < 0 || dist[nm[r]] > dist[nm[j]])) j = r;
+        std::swap(nm[from], nm[j]);
+      }
+      if ((B >> nm[5]) & 1) {
+        // lane 5 already holds a block bit
+      } else if (__builtin_popcount(B) == 4) {
+        for (int r = 8; r < 12; ++r)
+          if ((B >> nm[r]) & 1) {
+            std::swap(nm[5], nm[r]);
+            break;
+          }
+      }
+      if ((B >> nm[11]) & 1)  // register slot 3 is not part of the block
+        for (int r = 8; r < 11; ++r)
+          if (!((B >> nm[r]) & 1)) {
+            std::swap(nm[11], nm[r]);
+            break;
+          }
+      remap_to(nm, dist);
+      continue;
+    }
     if (first->op < 0) {
       // move the block's bits onto register slots 0..2 and thread slot 5,
       // keeping those already there
@@ -1791,6 +1835,9 @@ NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o) {
     PassArgs pa{};
     pa.op0 = (int)prog.size();
     pa.nops = (int)pr.ops.size();
+    bool has_remap = false;
+    for (const RegOp<float>& r : pr.ops) has_remap = has_remap || r.kind == R_REMAP;
+    pa.lds_remap = has_remap ? 8 << K : 0;
     prog.insert(prog.end(), pr.ops.begin(), pr.ops.end());
     args.push_back(pa);
   }

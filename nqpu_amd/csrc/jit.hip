@@ -13,7 +13,10 @@
 // life of the process, keyed by their source text.
 #include <hip/hiprtc.h>
 
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -74,16 +77,17 @@ void emit_op(std::ostringstream& o, const RegOp<float>& r, int k, int eb) {
   o << " }\n";
 }
 
-std::string kernel_source(int idx, const std::vector<RegOp<float>>& prog, int op0, int nops, int eb) {
+std::string kernel_source(int idx, const std::vector<RegOp<float>>& prog, const PassArgs& pa, int eb) {
+  const int op0 = pa.op0, nops = pa.nops;
   std::ostringstream o;
   o << "extern \"C\" __global__ __launch_bounds__(256, 1) void nqpu_pass_" << idx
     << "(float2* __restrict__ st, nqpu::PassArgs pa, uint64_t ntiles,\n"
        "    const nqpu::RegOp<float>* __restrict__ prog, const unsigned char* __restrict__ blob) {\n"
-       "  const nqpu::RegOp<float>* ops = prog + pa.op0;\n"
-       "  nqpu::reg_tile_pass<float, float2, "
+       "  const nqpu::RegOp<float>* ops = prog + pa.op0;\n";
+  o << "  nqpu::reg_tile_pass<float, float2, "
     << eb << ">(st, pa, ntiles, [&](float2 (&v)[" << (1 << eb)
-    << "], float2* sm, uint64_t base, int tid) __attribute__((always_inline)) {\n"
-       "    const int lane = tid & 63; (void)lane; (void)sm; (void)ops; (void)blob;\n";
+    << "], float2* sm, uint64_t base, int tid) __attribute__((always_inline)) {\n";
+  o << "    const int lane = tid & 63; (void)lane; (void)sm; (void)ops; (void)blob;\n";
   for (int k = 0; k < nops; ++k) emit_op(o, prog[op0 + k], k, eb);
   o << "  });\n}\n";
   return o.str();
@@ -146,16 +150,34 @@ std::vector<std::string> module_sources(const std::vector<RegOp<float>>& prog, c
     std::ostringstream o;
     o << kJitPrelude;
     for (int i = m * per; i < std::min(np, (m + 1) * per); ++i)
-      o << kernel_source(i - m * per, prog, args[i].op0, args[i].nops, eb);
+      o << kernel_source(i - m * per, prog, args[i], eb);
     srcs[m] = o.str();
   }
   return srcs;
 }
 
+// With NQPU_JIT_DUMP=<directory>, every generated module source is also
+// written there (nqpu_passes_<hash>.hip), for inspection and profiling.
+void dump_sources(const std::vector<std::string>& srcs) {
+  const char* dir = std::getenv("NQPU_JIT_DUMP");
+  if (!dir || !*dir) return;
+  for (const std::string& src : srcs) {
+    char name[64];
+    std::snprintf(name, sizeof name, "/nqpu_passes_%016llx.hip", (unsigned long long)std::hash<std::string>()(src));
+    if (FILE* f = std::fopen((std::string(dir) + name).c_str(), "w")) {
+      std::fwrite(src.data(), 1, src.size(), f);
+      std::fclose(f);
+    }
+  }
+}
+
 }  // namespace
 
-NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb) {
-  for (const std::string& src : module_sources(prog, args, eb)) {
+NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args,
+                            int eb) {
+  const std::vector<std::string> srcs = module_sources(prog, args, eb);
+  dump_sources(srcs);
+  for (const std::string& src : srcs) {
     std::vector<char> code;
     const NqpuError e = compile_code(src, &code);
     if (e != NQPU_OK) return e;
@@ -163,12 +185,13 @@ NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::ve
   return NQPU_OK;
 }
 
-NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb,
-                             std::vector<hipFunction_t>* fns) {
+NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args,
+                             int eb, std::vector<hipFunction_t>* fns) {
   // passes are split over a few modules compiled on parallel threads
   const int np = (int)args.size();
   const int per = kPassesPerModule;
   const std::vector<std::string> srcs = module_sources(prog, args, eb);
+  dump_sources(srcs);
   const int nmod = (int)srcs.size();
   std::vector<JitModule*> mods(nmod, nullptr);
   std::vector<int> todo;

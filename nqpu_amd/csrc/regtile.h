@@ -406,10 +406,12 @@ __device__ __forceinline__ void xchg_component(float& lo, float& hi, bool y) {
     b = p[1];
   } else {
     // lanes with bit B set read lane - 2^B; lanes with it clear read lane + 2^B
+    // (a lane whose row has no such partner keeps an undefined value, which
+    // the select below never takes)
     constexpr int kDown = B == 0 ? 0xB1 : B == 1 ? 0x4E : B == 2 ? 0x114 : 0x128;
     constexpr int kUp = B == 0 ? 0xB1 : B == 1 ? 0x4E : B == 2 ? 0x104 : 0x128;
-    const unsigned from_hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)b, kDown, 0xF, 0xF, false);
-    const unsigned from_lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)a, kUp, 0xF, 0xF, false);
+    const unsigned from_hi = (unsigned)__builtin_amdgcn_mov_dpp((int)b, kDown, 0xF, 0xF, false);
+    const unsigned from_lo = (unsigned)__builtin_amdgcn_mov_dpp((int)a, kUp, 0xF, 0xF, false);
     const unsigned na = y ? from_hi : a;
     const unsigned nb = y ? b : from_lo;
     a = na;
