@@ -404,12 +404,23 @@ __device__ __forceinline__ void xchg_component(float& lo, float& hi, bool y) {
     const auto p = __builtin_amdgcn_permlane16_swap(a, b, false, false);
     a = p[0];
     b = p[1];
+  } else if constexpr (B >= 2) {
+    // the lanes with bit B set are whole DPP banks (4 lanes): each new value is
+    // one DPP move into a copy of the value it replaces, writing only the banks
+    // that take the partner's value (row_ror:N reads lane i - N of the row)
+    constexpr int kSetBanks = B == 2 ? 0xA : 0xC, kClearBanks = B == 2 ? 0x5 : 0x3;
+    constexpr int kDown = B == 2 ? 0x124 : 0x128, kUp = B == 2 ? 0x12C : 0x128;  // row_ror:4 / :12 / :8
+    (void)y;
+    const unsigned na = (unsigned)__builtin_amdgcn_update_dpp((int)a, (int)b, kDown, 0xF, kSetBanks, false);
+    const unsigned nb = (unsigned)__builtin_amdgcn_update_dpp((int)b, (int)a, kUp, 0xF, kClearBanks, false);
+    a = na;
+    b = nb;
   } else {
     // lanes with bit B set read lane - 2^B; lanes with it clear read lane + 2^B
     // (a lane whose row has no such partner keeps an undefined value, which
     // the select below never takes)
-    constexpr int kDown = B == 0 ? 0xB1 : B == 1 ? 0x4E : B == 2 ? 0x114 : 0x128;
-    constexpr int kUp = B == 0 ? 0xB1 : B == 1 ? 0x4E : B == 2 ? 0x104 : 0x128;
+    constexpr int kDown = B == 0 ? 0xB1 : 0x4E;  // quad_perm xor 1 / xor 2
+    constexpr int kUp = kDown;
     const unsigned from_hi = (unsigned)__builtin_amdgcn_mov_dpp((int)b, kDown, 0xF, 0xF, false);
     const unsigned from_lo = (unsigned)__builtin_amdgcn_mov_dpp((int)a, kUp, 0xF, 0xF, false);
     const unsigned na = y ? from_hi : a;
