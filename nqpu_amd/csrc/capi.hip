@@ -560,7 +560,11 @@ NqpuError nqpu_debug_batch_plan(const NqpuCircuit* const* circuits, uint64_t cou
   NqpuError e = batch_args(circuits, count, terms, offsets, opts, cs, ts, o);
   if (e != NQPU_OK) return e;
   BatchPlan plan;
-  e = plan_batch(cs, ts, o, plan);
+  std::vector<char> image;
+  e = plan_batch(cs, ts, o, plan, [&image](size_t bytes) {
+    image.resize(bytes);
+    return image.data();
+  });
   if (e != NQPU_OK) return e;
   *out = plan.st;
   if (program_ops) *program_ops = plan.program_ops;

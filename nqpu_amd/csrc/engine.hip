@@ -9,6 +9,9 @@
 #include <cstring>
 #include <chrono>
 #include <map>
+#include <functional>
+#include <atomic>
+#include <thread>
 
 namespace nqpu {
 
@@ -1219,7 +1222,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
     if (remaining == 0) break;
     int dist[32];
     for (int b = 0; b < 32; ++b) dist[b] = 1 << 20;
-    {
+    if (xchg) {
       int c = 0;
       for (const Item& it : items) {
         if (it.done) continue;
@@ -2260,6 +2263,7 @@ Batch::~Batch() {
     (void)hipStreamSynchronize(stream);
     (void)hipStreamDestroy(stream);
   }
+  if (pinned) (void)hipHostFree(pinned);
 }
 
 NqpuError create_batch(Batch** out) {
@@ -2294,144 +2298,222 @@ static BatchTerm tile_term(const PauliTerm& t, const std::vector<int>& q) {
   return bt;
 }
 
-NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vector<std::vector<PauliTerm>>& terms,
-                     const NqpuRunOptions& o, BatchPlan& plan) {
-  const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
-  constexpr int EB = kBatchEB, KH = 8 + kBatchEB;
-  std::vector<BatchCircuit> bc;
+namespace {
+// One circuit's part of a batch plan, with pass / program / term indices
+// relative to its own tables (plan_batch concatenates them).
+struct CircuitPlan {
+  NqpuError err = NQPU_OK;
+  std::string msg;
+  BatchCircuit C{};
   std::vector<BatchPass> bp;
   std::vector<RegOp<float>> prog;
   std::vector<BatchTerm> bt;
-  std::vector<double> cost;
+  double cost = 0;
+  uint64_t amps = 0;  // HBM amplitudes (0 when resident)
+};
+
+CircuitPlan plan_batch_circuit(const Circuit& c, const std::vector<PauliTerm>& terms, bool ref, int k) {
+  constexpr int EB = kBatchEB, KH = 8 + kBatchEB;
+  CircuitPlan cp;
+  const int n = c.n;
+  if (n < EB) {
+    cp.err = NQPU_INVALID_PARAMETER;
+    cp.msg = "batched circuits need at least 4 qubits";
+    return cp;
+  }
+  const uint64_t lim = n >= 64 ? ~0ull : (bit(n) - 1);
+  for (const PauliTerm& t : terms)
+    if ((t.x | t.z) & ~lim) {
+      cp.err = NQPU_INVALID_QUBIT;
+      cp.msg = "observable term acts on a qubit beyond the circuit";
+      return cp;
+    }
+  std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
+  BatchCircuit& C = cp.C;
+  C.n = n;
+  C.out = (int32_t)k;
+  if (n <= KH) {
+    C.resident = 1;
+    Pass p;
+    for (int q = 0; q < n; ++q) p.qubits.push_back(q);
+    p.ops = ops;
+    BatchPass P{};
+    P.K = n;
+    P.tb = n - EB;
+    for (int q = 0; q < n; ++q) P.q[q] = q;
+    cp.prog = build_reg_program<float>(p, EB, n - EB).ops;
+    P.op0 = 0;
+    P.nops = (int)cp.prog.size();
+    P.term0 = 0;
+    for (const PauliTerm& t : terms) cp.bt.push_back(tile_term(t, p.qubits));
+    P.nterm = (int)terms.size();
+    C.pass0 = C.red0 = 0;
+    C.npass = 1;
+    cp.bp.push_back(P);
+    cp.cost = (double)(ops.size() + 1) * (double)(1ull << n) / 64.0;
+    return cp;
+  }
+  C.resident = 0;
+  cp.amps = 1ull << n;
+  PlanOptions po;
+  po.tile_qubits = KH;
+  po.low_qubits = 5;
+  std::vector<Pass> passes = plan_tiles(ops, n, po);
+  C.pass0 = 0;
+  C.npass = (int)passes.size();
+  for (const Pass& p : passes) {
+    BatchPass P{};
+    P.K = KH;
+    P.tb = 8;
+    for (int j = 0; j < KH; ++j) P.q[j] = p.qubits[j];
+    std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, 8).ops;
+    P.op0 = (int)cp.prog.size();
+    P.nops = (int)pr.size();
+    cp.prog.insert(cp.prog.end(), pr.begin(), pr.end());
+    cp.bp.push_back(P);
+  }
+  // observable passes: each term on a tile set that holds its X bits
+  std::vector<std::vector<int>> sets(2);
+  for (int q = 0; q < KH; ++q) sets[0].push_back(q);
+  for (int q = n - KH; q < n; ++q) sets[1].push_back(q);
+  std::vector<std::vector<PauliTerm>> groups(2);
+  auto setmask = [](const std::vector<int>& s) {
+    uint64_t m = 0;
+    for (int q : s) m |= bit(q);
+    return m;
+  };
+  for (const PauliTerm& t : terms) {
+    size_t g = 0;
+    for (; g < sets.size(); ++g)
+      if ((t.x & ~setmask(sets[g])) == 0) break;
+    if (g == sets.size()) {
+      if (__builtin_popcountll(t.x) > KH) {
+        cp.err = NQPU_INVALID_PARAMETER;
+        cp.msg = "an observable term has X/Y on more qubits than a tile holds";
+        return cp;
+      }
+      uint64_t m = t.x;
+      for (int q = 0; q < n && __builtin_popcountll(m) < KH; ++q) m |= bit(q);
+      std::vector<int> st;
+      for (int q = 0; q < n; ++q)
+        if ((m >> q) & 1) st.push_back(q);
+      sets.push_back(st);
+      groups.emplace_back();
+    }
+    groups[g].push_back(t);
+  }
+  C.red0 = (int)cp.bp.size();
+  for (size_t g = 0; g < sets.size(); ++g) {
+    if (groups[g].empty()) continue;
+    BatchPass P{};
+    P.K = KH;
+    P.tb = 8;
+    for (int j = 0; j < KH; ++j) P.q[j] = sets[g][j];
+    P.term0 = (int)cp.bt.size();
+    for (const PauliTerm& t : groups[g]) cp.bt.push_back(tile_term(t, sets[g]));
+    P.nterm = (int)groups[g].size();
+    cp.bp.push_back(P);
+    C.nred++;
+  }
+  cp.cost = (double)(C.npass + C.nred) * (double)(1ull << n);
+  return cp;
+}
+}  // namespace
+
+// Circuits are planned independently on a pool of host threads, then their
+// tables are concatenated in circuit order (the result does not depend on the
+// number of threads).
+NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vector<std::vector<PauliTerm>>& terms,
+                     const NqpuRunOptions& o, BatchPlan& plan, const std::function<char*(size_t)>& buffer) {
+  const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
+  std::vector<CircuitPlan> cps(circuits.size());
+  {
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+      for (size_t k; (k = next.fetch_add(1)) < circuits.size();)
+        cps[k] = plan_batch_circuit(*circuits[k], terms[k], ref, (int)k);
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nth = std::min<size_t>(hw, std::max<size_t>(1, circuits.size() / 64));
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nth; ++t) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
+  }
+  const size_t nc = cps.size();
+  // offsets of each circuit's passes, program ops and terms in the tables
+  std::vector<size_t> bp_at(nc + 1, 0), pg_at(nc + 1, 0), bt_at(nc + 1, 0);
+  std::vector<BatchCircuit> bc(nc);
   uint64_t state_amps = 0;
   NqpuBatchStats st{};
-  for (size_t k = 0; k < circuits.size(); ++k) {
-    const Circuit& c = *circuits[k];
-    const int n = c.n;
-    if (n < EB) {
-      set_error("batched circuits need at least 4 qubits");
-      return NQPU_INVALID_PARAMETER;
+  for (size_t k = 0; k < nc; ++k) {
+    const CircuitPlan& cp = cps[k];
+    if (cp.err != NQPU_OK) {
+      set_error(cp.msg);
+      return cp.err;
     }
-    const uint64_t lim = n >= 64 ? ~0ull : (bit(n) - 1);
-    for (const PauliTerm& t : terms[k])
-      if ((t.x | t.z) & ~lim) {
-        set_error("observable term acts on a qubit beyond the circuit");
-        return NQPU_INVALID_QUBIT;
-      }
-    std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
-    BatchCircuit C{};
-    C.n = n;
-    C.out = (int32_t)k;
-    if (n <= KH) {
-      C.resident = 1;
-      Pass p;
-      for (int q = 0; q < n; ++q) p.qubits.push_back(q);
-      p.ops = ops;
-      BatchPass P{};
-      P.K = n;
-      P.tb = n - EB;
-      for (int q = 0; q < n; ++q) P.q[q] = q;
-      std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, n - EB).ops;
-      P.op0 = (int)prog.size();
-      P.nops = (int)pr.size();
-      prog.insert(prog.end(), pr.begin(), pr.end());
-      P.term0 = (int)bt.size();
-      for (const PauliTerm& t : terms[k]) bt.push_back(tile_term(t, p.qubits));
-      P.nterm = (int)terms[k].size();
-      C.pass0 = C.red0 = (int)bp.size();
-      C.npass = 1;
-      bp.push_back(P);
-      cost.push_back((double)(ops.size() + 1) * (double)(1ull << n) / 64.0);
-      st.resident++;
-    } else {
-      C.resident = 0;
+    bp_at[k + 1] = bp_at[k] + cp.bp.size();
+    pg_at[k + 1] = pg_at[k] + cp.prog.size();
+    bt_at[k + 1] = bt_at[k] + cp.bt.size();
+    BatchCircuit C = cp.C;
+    C.pass0 += (int32_t)bp_at[k];
+    C.red0 += (int32_t)bp_at[k];
+    if (!C.resident) {
       C.state_off = state_amps;
-      state_amps += 1ull << n;
-      PlanOptions po;
-      po.tile_qubits = KH;
-      po.low_qubits = 5;
-      std::vector<Pass> passes = plan_tiles(ops, n, po);
-      C.pass0 = (int)bp.size();
-      C.npass = (int)passes.size();
-      for (const Pass& p : passes) {
-        BatchPass P{};
-        P.K = KH;
-        P.tb = 8;
-        for (int j = 0; j < KH; ++j) P.q[j] = p.qubits[j];
-        std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, 8).ops;
-        P.op0 = (int)prog.size();
-        P.nops = (int)pr.size();
-        prog.insert(prog.end(), pr.begin(), pr.end());
-        bp.push_back(P);
-      }
-      // observable passes: each term on a tile set that holds its X bits
-      std::vector<std::vector<int>> sets(2);
-      for (int q = 0; q < KH; ++q) sets[0].push_back(q);
-      for (int q = n - KH; q < n; ++q) sets[1].push_back(q);
-      std::vector<std::vector<PauliTerm>> groups(2);
-      auto setmask = [](const std::vector<int>& s) {
-        uint64_t m = 0;
-        for (int q : s) m |= bit(q);
-        return m;
-      };
-      for (const PauliTerm& t : terms[k]) {
-        size_t g = 0;
-        for (; g < sets.size(); ++g)
-          if ((t.x & ~setmask(sets[g])) == 0) break;
-        if (g == sets.size()) {
-          if (__builtin_popcountll(t.x) > KH) {
-            set_error("an observable term has X/Y on more qubits than a tile holds");
-            return NQPU_INVALID_PARAMETER;
-          }
-          uint64_t m = t.x;
-          for (int q = 0; q < n && __builtin_popcountll(m) < KH; ++q) m |= bit(q);
-          std::vector<int> s;
-          for (int q = 0; q < n; ++q)
-            if ((m >> q) & 1) s.push_back(q);
-          sets.push_back(s);
-          groups.emplace_back();
-        }
-        groups[g].push_back(t);
-      }
-      C.red0 = (int)bp.size();
-      for (size_t g = 0; g < sets.size(); ++g) {
-        if (groups[g].empty()) continue;
-        BatchPass P{};
-        P.K = KH;
-        P.tb = 8;
-        for (int j = 0; j < KH; ++j) P.q[j] = sets[g][j];
-        P.term0 = (int)bt.size();
-        for (const PauliTerm& t : groups[g]) bt.push_back(tile_term(t, sets[g]));
-        P.nterm = (int)groups[g].size();
-        bp.push_back(P);
-        C.nred++;
-      }
+      state_amps += cp.amps;
       st.gate_passes += C.npass;
       st.observable_passes += C.nred;
-      cost.push_back((double)(C.npass + C.nred) * (double)(1ull << n));
+    } else {
+      st.resident++;
     }
-    bc.push_back(C);
+    bc[k] = C;
   }
   // longest circuits first: the queue hands them out before the short ones
-  std::vector<size_t> order(bc.size());
-  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
-  std::vector<BatchCircuit> sorted;
-  sorted.reserve(bc.size());
-  for (size_t i : order) sorted.push_back(bc[i]);
-  // one table upload: [circuits][passes][program][terms][counter]
+  std::vector<size_t> order(nc);
+  for (size_t i = 0; i < nc; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cps[x].cost > cps[y].cost; });
+  // one table image: [circuits][passes][program][terms][counter]
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-  const size_t oc = 0, op_ = up(oc + sorted.size() * sizeof(BatchCircuit));
-  const size_t og = up(op_ + bp.size() * sizeof(BatchPass));
-  const size_t ot = up(og + prog.size() * sizeof(RegOp<float>));
-  const size_t ocnt = up(ot + bt.size() * sizeof(BatchTerm));
+  const size_t oc = 0, op_ = up(oc + nc * sizeof(BatchCircuit));
+  const size_t og = up(op_ + bp_at[nc] * sizeof(BatchPass));
+  const size_t ot = up(og + pg_at[nc] * sizeof(RegOp<float>));
+  const size_t ocnt = up(ot + bt_at[nc] * sizeof(BatchTerm));
   const size_t total = ocnt + 256;
-  std::vector<char> host(total, 0);
-  std::memcpy(host.data() + oc, sorted.data(), sorted.size() * sizeof(BatchCircuit));
-  std::memcpy(host.data() + op_, bp.data(), bp.size() * sizeof(BatchPass));
-  if (!prog.empty()) std::memcpy(host.data() + og, prog.data(), prog.size() * sizeof(RegOp<float>));
-  if (!bt.empty()) std::memcpy(host.data() + ot, bt.data(), bt.size() * sizeof(BatchTerm));
-  plan.host = std::move(host);
+  char* host = buffer(total);
+  if (!host) {
+    set_error("no host memory for the batch tables");
+    return NQPU_ALLOCATION_FAILED;
+  }
+  for (size_t i = 0; i < nc; ++i)
+    std::memcpy(host + oc + i * sizeof(BatchCircuit), &bc[order[i]], sizeof(BatchCircuit));
+  std::memset(host + ocnt, 0, 256);  // the work-queue counter starts at zero
+  {
+    std::atomic<size_t> next{0};
+    auto fill = [&] {
+      for (size_t k; (k = next.fetch_add(1)) < nc;) {
+        const CircuitPlan& cp = cps[k];
+        BatchPass* dp = reinterpret_cast<BatchPass*>(host + op_) + bp_at[k];
+        for (size_t i = 0; i < cp.bp.size(); ++i) {
+          BatchPass P = cp.bp[i];
+          P.op0 += (int32_t)pg_at[k];
+          P.term0 += (int32_t)bt_at[k];
+          std::memcpy(dp + i, &P, sizeof(BatchPass));
+        }
+        if (!cp.prog.empty())
+          std::memcpy(reinterpret_cast<RegOp<float>*>(host + og) + pg_at[k], cp.prog.data(),
+                      cp.prog.size() * sizeof(RegOp<float>));
+        if (!cp.bt.empty())
+          std::memcpy(reinterpret_cast<BatchTerm*>(host + ot) + bt_at[k], cp.bt.data(), cp.bt.size() * sizeof(BatchTerm));
+      }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nth = std::min<size_t>(hw, std::max<size_t>(1, nc / 64));
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nth; ++t) th.emplace_back(fill);
+    fill();
+    for (auto& x : th) x.join();
+  }
+  plan.host = host;
   plan.oc = oc;
   plan.op = op_;
   plan.og = og;
@@ -2439,8 +2521,8 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   plan.ocnt = ocnt;
   plan.total = total;
   plan.state_amps = state_amps;
-  plan.ncirc = sorted.size();
-  plan.program_ops = prog.size();
+  plan.ncirc = nc;
+  plan.program_ops = pg_at[nc];
   st.circuits = circuits.size();
   st.state_bytes = (double)state_amps * 8.0;
   plan.st = st;
@@ -2453,7 +2535,21 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   using clock = std::chrono::steady_clock;
   const auto t0 = clock::now();
   BatchPlan plan;
-  NqpuError pe = plan_batch(circuits, terms, o, plan);
+  // the tables are written straight into a pinned host buffer the batch keeps
+  auto pinned = [b](size_t bytes) -> char* {
+    if (bytes > b->pinned_bytes) {
+      if (b->pinned) (void)hipHostFree(b->pinned);
+      b->pinned = nullptr;
+      b->pinned_bytes = 0;
+      void* p = nullptr;
+      if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+      b->pinned = (char*)p;
+      b->pinned_bytes = bytes;
+    }
+    return b->pinned;
+  };
+  HIPTRY(hipStreamSynchronize(b->stream), "sync before planning");  // the previous upload may still read the buffer
+  NqpuError pe = plan_batch(circuits, terms, o, plan, pinned);
   if (pe != NQPU_OK) return pe;
   const size_t oc = plan.oc, op_ = plan.op, og = plan.og, ot = plan.ot, ocnt = plan.ocnt, total = plan.total;
   const uint64_t state_amps = plan.state_amps;
@@ -2464,7 +2560,7 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
     set_error("batch states do not fit in device memory");
     return NQPU_OUT_OF_DEVICE_MEMORY;
   }
-  HIPTRY(copy_on(b->stream, b->tables.p, plan.host.data(), total, hipMemcpyHostToDevice), "batch table upload");
+  HIPTRY(copy_on(b->stream, b->tables.p, plan.host, total, hipMemcpyHostToDevice), "batch table upload");
   char* base = (char*)b->tables.p;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);

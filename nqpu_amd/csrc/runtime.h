@@ -5,6 +5,7 @@
 #include "../../include/nqpu_hip.h"
 #include "engine.h"
 
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -157,19 +158,22 @@ struct PauliTerm {
 struct Batch {
   hipStream_t stream = nullptr;
   DeviceScratch states, tables, out;
+  char* pinned = nullptr;  // pinned host image of the tables, reused across batches
+  size_t pinned_bytes = 0;
   NqpuBatchStats stats{};
   ~Batch();
 };
 // Host half of a batch: the tables k_batch reads, in one upload image.
 struct BatchPlan {
-  std::vector<char> host;
+  char* host = nullptr;  // from the buffer callback of plan_batch (not owned)
   size_t oc = 0, op = 0, og = 0, ot = 0, ocnt = 0, total = 0;
   uint64_t state_amps = 0;
   size_t ncirc = 0, program_ops = 0;
   NqpuBatchStats st{};
 };
+// buffer(bytes) returns at least `bytes` of host memory for the table image
 NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vector<std::vector<PauliTerm>>& terms,
-                     const NqpuRunOptions& o, BatchPlan& plan);
+                     const NqpuRunOptions& o, BatchPlan& plan, const std::function<char*(size_t)>& buffer);
 NqpuError create_batch(Batch** out);
 NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuits,
                             const std::vector<std::vector<PauliTerm>>& terms, const NqpuRunOptions& o,

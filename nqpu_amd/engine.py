@@ -8,6 +8,7 @@ Every call goes through libnqpu_hip.so; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as ct
+import functools
 from dataclasses import dataclass, field
 from typing import Iterable, Sequence
 
@@ -199,6 +200,18 @@ def _prec(p) -> int:
 
 def pauli_masks(pauli: str | Sequence[str]) -> tuple[int, int]:
     """Pauli string (character q acts on qubit q, as lib.rs:465) -> (xmask, zmask)."""
+    if isinstance(pauli, str):
+        return _pauli_masks_str(pauli)
+    return _pauli_masks(pauli)
+
+
+@functools.lru_cache(maxsize=1 << 16)
+def _pauli_masks_str(pauli: str) -> tuple[int, int]:
+    # strings repeat across the circuits of a batch (the same Hamiltonian terms)
+    return _pauli_masks(pauli)
+
+
+def _pauli_masks(pauli) -> tuple[int, int]:
     x = z = 0
     for q, c in enumerate(pauli):
         c = c.upper()
@@ -371,6 +384,20 @@ def _terms(obs):
     return out
 
 
+_TERM_DTYPE = np.dtype([("xmask", np.uint64), ("zmask", np.uint64), ("coeff", np.float64)])
+
+
+def _term_table(observables):
+    """All circuits' terms as one NqpuPauliTerm array (numpy, same layout) and
+    the per-circuit offsets."""
+    terms, offsets = [], [0]
+    for obs in observables:
+        terms.extend(_terms(obs))
+        offsets.append(len(terms))
+    table = np.array(terms if terms else [(0, 0, 0.0)], dtype=_TERM_DTYPE)
+    return table, np.array(offsets, dtype=np.uint64)
+
+
 class Batch:
     """Many complex64 circuits in one persistent GPU launch, each returning
     the FP64 expectation of its own Pauli sum (the batched counterpart of
@@ -386,24 +413,34 @@ class Batch:
         lib = L.load()
         natives = [c if isinstance(c, NativeCircuit) else c.native() for c in circuits]
         arr = (ct.c_void_p * max(1, len(natives)))(*[nc.ptr for nc in natives])
-        terms, offsets = [], [0]
-        for obs in observables:
-            terms.extend(_terms(obs))
-            offsets.append(len(terms))
-        tarr = (L.NqpuPauliTerm * max(1, len(terms)))()
-        for i, (x, z, c) in enumerate(terms):
-            tarr[i].xmask, tarr[i].zmask, tarr[i].coeff = x, z, c
-        off = np.array(offsets, dtype=np.uint64)
+        table, off = _term_table(observables)
         out = np.empty(max(1, len(natives)), dtype=np.float64)
         o = run_options("tiled", semantics)
-        L.check(lib.nqpu_batch_expectation(self.ptr, arr, len(natives), tarr, off.ctypes.data, ct.byref(o),
-                                           out.ctypes.data), "nqpu_batch_expectation")
+        L.check(lib.nqpu_batch_expectation(self.ptr, arr, len(natives), table.ctypes.data, off.ctypes.data,
+                                           ct.byref(o), out.ctypes.data), "nqpu_batch_expectation")
         return out[: len(natives)]
 
     def last_stats(self) -> dict:
         st = L.NqpuBatchStats()
         L.check(L.load().nqpu_batch_last_stats(self.ptr, ct.byref(st)), "batch stats")
         return {k: getattr(st, k) for k, _ in L.NqpuBatchStats._fields_}
+
+    @staticmethod
+    def plan(circuits, observables, semantics="standard") -> dict:
+        """The host half of expectation() only (no device): the batch's pass
+        counts, planning time and the number of register-program ops."""
+        lib = L.load()
+        natives = [c if isinstance(c, NativeCircuit) else c.native() for c in circuits]
+        arr = (ct.c_void_p * max(1, len(natives)))(*[nc.ptr for nc in natives])
+        table, off = _term_table(observables)
+        st = L.NqpuBatchStats()
+        nops = ct.c_uint64()
+        o = run_options("tiled", semantics)
+        L.check(lib.nqpu_debug_batch_plan(arr, len(natives), table.ctypes.data, off.ctypes.data, ct.byref(o),
+                                          ct.byref(st), ct.byref(nops)), "nqpu_debug_batch_plan")
+        out = {k: getattr(st, k) for k, _ in L.NqpuBatchStats._fields_}
+        out["program_ops"] = nops.value
+        return out
 
     def close(self):
         if getattr(self, "ptr", None):

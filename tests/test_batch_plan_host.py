@@ -1,0 +1,36 @@
+"""Host half of batched execution (no device): circuits are planned on a pool
+of host threads and their tables concatenated in circuit order, so the plan
+is the same on every call; errors of any circuit reach the caller."""
+import pytest
+
+from nqpu_amd import _lib as L
+from nqpu_amd.engine import Batch, Circuit
+from nqpu_amd.workloads import variational_batch
+
+
+def test_plan_is_deterministic_and_counts_passes():
+    vcs = variational_batch(300, seed=103)
+    circs = [v.circuit.native() for v in vcs]
+    obs = [v.hamiltonian() for v in vcs]
+    a = Batch.plan(circs, obs)
+    b = Batch.plan(circs, obs)
+    for k in ("circuits", "resident", "gate_passes", "observable_passes", "state_bytes", "program_ops"):
+        assert a[k] == b[k], k
+    assert a["circuits"] == 300
+    small = sum(1 for v in vcs if v.circuit.num_qubits <= 12)
+    assert a["resident"] == small
+    assert a["gate_passes"] > 0 and a["observable_passes"] > 0
+    assert a["state_bytes"] == 8.0 * sum(1 << v.circuit.num_qubits for v in vcs if v.circuit.num_qubits > 12)
+
+
+def test_error_of_one_circuit_is_reported():
+    vcs = variational_batch(100, seed=5)
+    circs = [v.circuit.native() for v in vcs]
+    obs = [v.hamiltonian() for v in vcs]
+    tiny = Circuit(3)
+    tiny.h(0)
+    circs[57] = tiny.native()
+    obs[57] = [(1.0, "ZII")]
+    with pytest.raises(L.NqpuError) as e:
+        Batch.plan(circs, obs)
+    assert e.value.code == 2  # INVALID_PARAMETER: batched circuits need at least 4 qubits
