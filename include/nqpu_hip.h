@@ -222,6 +222,9 @@ typedef struct {
  * kernels depend only on the program's structure, not on gate angles, and
  * stay cached for the life of the process */
 #define NQPU_RUN_COMPILE 16u
+/* complex64 register tiles: apply runs of diagonal gates one by one instead
+ * of merging each run into one quadratic phase evaluated per amplitude */
+#define NQPU_RUN_NO_QPHASE 32u
 
 /* Statistics of the last run on a state. */
 typedef struct {
@@ -235,6 +238,7 @@ typedef struct {
   uint64_t mfma_blocks;     /* fused 4-qubit blocks applied on the matrix cores */
   uint64_t mfma_gates;      /* device ops those blocks absorbed */
   uint64_t exchanges;       /* register <-> lane-bit exchanges inside register tile passes (no LDS) */
+  uint64_t qphase_gates;    /* diagonal device ops merged into quadratic phase ops */
 } NqpuRunStats;
 
 /* 1 <= n <= 34 (c64) or 33 (c128), bounded by free device memory. */

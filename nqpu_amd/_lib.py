@@ -59,7 +59,7 @@ class NqpuRunStats(ct.Structure):
     _fields_ = [("gates", ct.c_uint64), ("device_ops", ct.c_uint64), ("passes", ct.c_uint64),
                 ("bytes_moved", ct.c_double), ("pass_ms", ct.c_double), ("timed_passes", ct.c_uint64),
                 ("remaps", ct.c_uint64), ("mfma_blocks", ct.c_uint64), ("mfma_gates", ct.c_uint64),
-                ("exchanges", ct.c_uint64)]
+                ("exchanges", ct.c_uint64), ("qphase_gates", ct.c_uint64)]
 
 
 class NqpuPauliTerm(ct.Structure):
@@ -75,6 +75,7 @@ class NqpuBatchStats(ct.Structure):
 RUN_BATCH_LAYOUT = 4
 RUN_NO_MFMA = 8
 RUN_COMPILE = 16
+RUN_NO_QPHASE = 32
 
 
 class NqpuFusionCounts(ct.Structure):

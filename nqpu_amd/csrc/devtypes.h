@@ -33,10 +33,24 @@ enum RegOpKind : int32_t {
   // Exchange register slot s0 with thread slot s1 < 6 (a lane bit: DPP or
   // lane-half swaps inside the wavefront, no LDS) or with register slot
   // s1 - 16 (register moves).  Afterwards each slot holds the other's bit.
-  R_XCHG = 8
+  R_XCHG = 8,
+  // Phase exp(2 pi i Phi(x)) with Phi a quadratic form in the index bits, in
+  // revolutions (a run of one- and two-qubit diagonal gates).  Per thread
+  // (tid) and register index r:
+  //   Phi = C[tid] + sum_c K0_c + sum_t tid_t Kt_c[t] + sum_j r_j (L_j[tid] + sum_c Kr_c[j])
+  //         + sum_{j<k} r_j r_k m[pair(j, k)]
+  // Tables start at operand unit s0 (kMfmaBlobBytes each) of the pass: C[256]
+  // then L_j[256] for j < EB (floats), then one table per chunk c < pad of
+  // qubits outside the tile, 256 x 16 floats indexed by the chunk's bits of
+  // the tile's base index (lowest qubit = bit 0): [K0, Kt[0..7], Kr[0..EB-1]].
+  // Chunk c's qubits are the set bits of the 64-bit mask qphase_chunk(op, c)
+  // (bytes 8c.. of map_old ++ map_new).  m[pair(j, k)]: register-slot pair
+  // coefficients, pair(j, k) = k (k - 1) / 2 + j.
+  R_QPHASE = 9
 };
 
 constexpr int kMfmaBlobBytes = 64 * 64;  // 64 lanes x (2 K-steps x (hi, lo) x 8 fp16)
+constexpr int kQPhaseChunkFloats = 256 * 16;  // one chunk table
 
 template <typename R>
 struct RegOp {
@@ -48,6 +62,19 @@ struct RegOp {
   uint64_t gc;  // controls outside the tile (global qubit mask): the op runs on tiles that satisfy them
   R m[32];
 };
+
+// R_QPHASE: mask of the qubits outside the tile in chunk c < 4 (read as two
+// 32-bit words: the kernels load it with scalar loads)
+template <typename R>
+__host__ __device__ inline uint64_t qphase_chunk(const RegOp<R>& op, int c) {
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(c < 2 ? op.map_old : op.map_new) + 2 * (c & 1);
+  return (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+}
+template <typename R>
+inline void set_qphase_chunk(RegOp<R>& op, int c, uint64_t m) {
+  uint8_t* b = (c < 2 ? op.map_old : op.map_new) + 8 * (c & 1);
+  for (int i = 0; i < 8; ++i) b[i] = (uint8_t)(m >> (8 * i));
+}
 
 // Diagonal-op positions at or above kGlobalBit name qubit (pos - kGlobalBit)
 // outside the tile; its value is the same for the whole tile.

@@ -796,6 +796,11 @@ struct RegProgram {
   std::vector<RegOp<R>> ops;
   std::vector<std::array<cd, 256>> blocks;
   uint64_t fused_ops = 0;  // device ops inside those blocks
+  // R_QPHASE tables, in operand units of kMfmaBlobBytes (an op's s0 counts
+  // units from the start of qtab; the device program places them after the
+  // blocks' operand blobs)
+  std::vector<R> qtab;  // float on the device; the FP64 emulation keeps FP64 entries
+  uint64_t qphase_gates = 0;  // diagonal gates merged into R_QPHASE ops
   // tile bit held by each slot (0..7 thread, 8.. register) when the tile is
   // loaded and when it is stored
   uint8_t map0[16] = {};
@@ -814,7 +819,7 @@ template <typename R>
 // long as thread slots 0..4 hold the tile's five lowest bits again (so the
 // store is as coalesced as the load).
 RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false, bool xchg = false,
-                                int max_blobs = 1 << 20) {
+                                int max_blobs = 1 << 20, bool qphase = false) {
   const int K = (int)p.qubits.size();  // == tb + eb
   int loc[64];
   for (int q = 0; q < 64; ++q) loc[q] = -1;
@@ -827,6 +832,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
     uint64_t gc;
     const HostOp* h;
     bool done;
+    int grp = -1;  // kQPhaseOp: index into qgroups
   };
   auto local = [&](int q) { return loc[q] >= 0 ? loc[q] : kGlobalBit + q; };
   std::vector<L> ops;
@@ -844,7 +850,28 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
   auto two = [](int k) { return k == OP_U2 || k == OP_SWAP || k == OP_D2; };
   // tile bits an op touches (bits outside the tile are constant over it)
   auto lbit = [](int x) { return x < kGlobalBit ? 1u << x : 0u; };
-  auto qmask = [&](const L& o) { return o.cm | lbit(o.a) | (two(o.kind) ? lbit(o.b) : 0u); };
+  // ---- runs of one- and two-qubit diagonal gates become one R_QPHASE each.
+  // Every such gate with unit-modulus phases is exp(2 pi i phi) with phi a
+  // quadratic polynomial in at most two index bits (its positions and
+  // controls), so a run of them is one quadratic form over the bits it
+  // touches: Phi = c0 + sum lin[x] b_x + sum quad[x, y] b_x b_y (revolutions;
+  // positions x are tile-local bits or kGlobalBit + qubit).
+  struct QGroup {
+    double c0 = 0;
+    std::map<int, double> lin;
+    std::map<std::pair<int, int>, double> quad;  // x < y
+    uint32_t tile = 0;                           // tile bits it touches
+    uint64_t gmask = 0;                          // qubits outside the tile it touches
+    std::vector<std::vector<int>> chunks;        // those qubits, <= 8 per chunk, <= 4 chunks
+    std::vector<int> members;
+  };
+  std::vector<QGroup> qgroups;
+  constexpr int kQPhaseOp = 100;     // host-side kind of a merged run
+  constexpr size_t kQPhaseMinOps = 3;  // shorter runs stay single ops (or join blocks)
+  auto qmask = [&](const L& o) {
+    if (o.kind == kQPhaseOp) return qgroups[o.grp].tile;
+    return o.cm | lbit(o.a) | (two(o.kind) ? lbit(o.b) : 0u);
+  };
   auto targets = [&](const L& o) -> uint32_t {
     switch (o.kind) {
       case OP_U1: case OP_FLIP: return 1u << o.a;
@@ -853,6 +880,180 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       default: return 0u;
     }
   };
+  if (qphase && tb == 8) {
+    // phase terms of one diagonal op: constant, up to two positions with
+    // linear coefficients, and their product coefficient
+    struct QTerms {
+      double c0 = 0, lin[2] = {0, 0}, quad = 0;
+      int np = 0, pos[2] = {0, 0};
+    };
+    auto rev = [](cd z) { return std::arg(z) / (2 * M_PI); };
+    auto unit = [](cd z) { return std::abs(std::abs(z) - 1.0) < 1e-12; };
+    auto qterms = [&](const L& o, QTerms* t) -> bool {
+      if (o.kind != OP_D1 && o.kind != OP_D2) return false;
+      int ctl[2], nc = 0;
+      for (int q = 0; q < K; ++q)
+        if ((o.cm >> q) & 1) {
+          if (nc == 2) return false;
+          ctl[nc++] = q;
+        }
+      for (int q = 0; q < 64; ++q)
+        if ((o.gc >> q) & 1) {
+          if (nc == 2) return false;
+          ctl[nc++] = kGlobalBit + q;
+        }
+      const HostOp& h = *o.h;
+      if (o.kind == OP_D1) {  // diag(m0, m1) on a, when the control (if any) is 1
+        if (nc > 1 || !unit(h.m[0]) || !unit(h.m[1])) return false;
+        const double t0 = rev(h.m[0]), t1 = rev(h.m[1]);
+        t->pos[0] = o.a;
+        if (nc == 0) {
+          t->np = 1;
+          t->c0 = t0;
+          t->lin[0] = t1 - t0;
+        } else {
+          t->np = 2;
+          t->pos[1] = ctl[0];
+          t->lin[1] = t0;
+          t->quad = t1 - t0;
+        }
+        return true;
+      }
+      if (nc > 0) return false;
+      for (int i = 0; i < 4; ++i)
+        if (!unit(h.m[i])) return false;
+      // index bit_a | bit_b << 1
+      const double t00 = rev(h.m[0]), t10 = rev(h.m[1]), t01 = rev(h.m[2]), t11 = rev(h.m[3]);
+      t->np = 2;
+      t->pos[0] = o.a;
+      t->pos[1] = o.b;
+      t->c0 = t00;
+      t->lin[0] = t10 - t00;
+      t->lin[1] = t01 - t00;
+      t->quad = t11 - t10 - t01 + t00;
+      return true;
+    };
+    // qubits outside the tile, in chunks of at most 8 (at most 4 chunks); two
+    // of them with a product term must share a chunk
+    auto pack = [](const std::vector<int>& gq, const std::vector<std::pair<int, int>>& gg,
+                   std::vector<std::vector<int>>* out) -> bool {
+      std::map<int, int> par;
+      for (int q : gq) par[q] = q;
+      std::function<int(int)> find = [&](int x) { return par[x] == x ? x : par[x] = find(par[x]); };
+      for (const auto& e : gg) par[find(e.first)] = find(e.second);
+      std::map<int, std::vector<int>> comp;
+      for (int q : gq) comp[find(q)].push_back(q);
+      std::vector<std::vector<int>> cs;
+      for (auto& kv : comp) cs.push_back(kv.second);
+      std::stable_sort(cs.begin(), cs.end(), [](const std::vector<int>& x, const std::vector<int>& y) {
+        return x.size() > y.size();
+      });
+      std::vector<std::vector<int>> bins;
+      for (const auto& c : cs) {
+        if (c.size() > 8) return false;
+        bool placed = false;
+        for (auto& b : bins)
+          if (b.size() + c.size() <= 8) {
+            b.insert(b.end(), c.begin(), c.end());
+            placed = true;
+            break;
+          }
+        if (!placed) {
+          if (bins.size() == 4) return false;
+          bins.push_back(c);
+        }
+      }
+      if (out) *out = bins;
+      return true;
+    };
+    // A run grows in circuit order: a diagonal op joins the open run when no
+    // non-diagonal op since the run's first member targets one of its tile
+    // bits (so it commutes with every op it overtakes); otherwise it opens a
+    // new run.  Diagonal ops that do not fit the form commute with the run.
+    auto ntargets = [&](const L& o) -> uint32_t {
+      switch (o.kind) {
+        case OP_U1: case OP_FLIP: return lbit(o.a);
+        case OP_U2: case OP_SWAP: return lbit(o.a) | lbit(o.b);
+        default: return 0u;
+      }
+    };
+    struct Run {
+      std::vector<int> members, gq;
+      std::vector<std::pair<int, int>> gg;
+    };
+    std::vector<Run> runs;
+    std::vector<int> run_of(ops.size(), -1);
+    int cur = -1;
+    uint32_t blocked = 0;
+    for (int k = 0; k < (int)ops.size(); ++k) {
+      const L& o = ops[k];
+      QTerms t;
+      if (!qterms(o, &t)) {
+        if (o.kind != OP_D1 && o.kind != OP_D2) blocked |= ntargets(o);
+        continue;
+      }
+      uint32_t bits = 0;
+      for (int i = 0; i < t.np; ++i) bits |= lbit(t.pos[i]);
+      auto with = [&](Run r) {
+        for (int i = 0; i < t.np; ++i)
+          if (t.pos[i] >= kGlobalBit && std::find(r.gq.begin(), r.gq.end(), t.pos[i]) == r.gq.end())
+            r.gq.push_back(t.pos[i]);
+        if (t.np == 2 && t.pos[0] >= kGlobalBit && t.pos[1] >= kGlobalBit) r.gg.push_back({t.pos[0], t.pos[1]});
+        r.members.push_back(k);
+        return r;
+      };
+      bool joined = false;
+      if (cur >= 0 && (bits & blocked) == 0) {
+        Run r = with(runs[cur]);
+        if (pack(r.gq, r.gg, nullptr)) {
+          runs[cur] = std::move(r);
+          joined = true;
+        }
+      }
+      if (!joined) {
+        runs.push_back(with(Run{}));
+        cur = (int)runs.size() - 1;
+        blocked = 0;
+      }
+      run_of[k] = cur;
+    }
+    std::vector<L> merged;
+    merged.reserve(ops.size());
+    for (int k = 0; k < (int)ops.size(); ++k) {
+      const int ri = run_of[k];
+      if (ri < 0 || runs[ri].members.size() < kQPhaseMinOps) {
+        merged.push_back(ops[k]);
+        continue;
+      }
+      const Run& run = runs[ri];
+      if (run.members[0] != k) continue;  // merged into the run's first position
+      QGroup g;
+      for (int m : run.members) {
+        QTerms t;
+        qterms(ops[m], &t);
+        g.c0 += t.c0;
+        for (int i = 0; i < t.np; ++i) {
+          g.lin[t.pos[i]] += t.lin[i];
+          if (t.pos[i] < kGlobalBit) g.tile |= 1u << t.pos[i];
+          else g.gmask |= bit(t.pos[i] - kGlobalBit);
+        }
+        if (t.np == 2) g.quad[{std::min(t.pos[0], t.pos[1]), std::max(t.pos[0], t.pos[1])}] += t.quad;
+      }
+      std::vector<std::vector<int>> ch;
+      pack(run.gq, run.gg, &ch);
+      for (auto& c : ch) {
+        for (int& x : c) x -= kGlobalBit;  // positions -> qubits
+        std::sort(c.begin(), c.end());
+      }
+      g.chunks = ch;
+      g.members = run.members;
+      qgroups.push_back(std::move(g));
+      L l{kQPhaseOp, -1, -1, 0, 0, nullptr, false};
+      l.grp = (int)qgroups.size() - 1;
+      merged.push_back(l);
+    }
+    ops.swap(merged);
+  }
   // canonical assignment: thread slot i holds tile bit i (i < tb), register
   // slot j holds tile bit tb + j; thread slots >= tb do not exist
   uint8_t map[16], canon[16];
@@ -941,7 +1142,101 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         }
     }
   };
+  // R_QPHASE of a merged run, with its coefficients sorted by the slots that
+  // hold their bits now
+  auto emit_qphase = [&](const QGroup& g) {
+    // code: thread slot (0..7), 16 + register slot, or kGlobalBit + qubit
+    auto code = [&](int x) { return x >= kGlobalBit ? x : slot(x); };
+    double lt[8] = {}, lr[8] = {}, att[8][8] = {}, atr[8][8] = {}, arr[8][8] = {};
+    std::map<int, double> lg;
+    std::map<int, std::array<double, 8>> agt, agr;
+    std::map<std::pair<int, int>, double> agg;
+    for (const auto& kv : g.lin) {
+      const int c = code(kv.first);
+      if (c >= kGlobalBit) lg[c - kGlobalBit] += kv.second;
+      else if (c >= 16) lr[c - 16] += kv.second;
+      else lt[c] += kv.second;
+    }
+    for (const auto& kv : g.quad) {
+      int x = code(kv.first.first), y = code(kv.first.second);
+      if (x > y) std::swap(x, y);  // thread < register < outside the tile
+      const double a = kv.second;
+      if (x >= kGlobalBit) agg[{x - kGlobalBit, y - kGlobalBit}] += a;
+      else if (y >= kGlobalBit) {
+        auto& row = x >= 16 ? agr[y - kGlobalBit] : agt[y - kGlobalBit];
+        row[x >= 16 ? x - 16 : x] += a;
+      } else if (x >= 16) arr[x - 16][y - 16] += a;
+      else if (y >= 16) atr[x][y - 16] += a;
+      else att[x][y] += a;
+    }
+    auto fr = [](double x) { return (R)(x - std::nearbyint(x)); };
+    const int nch = (int)g.chunks.size();
+    const size_t len = 256 * (size_t)(1 + eb) + (size_t)nch * kQPhaseChunkFloats;
+    const size_t unit = kMfmaBlobBytes / sizeof(float);
+    const size_t at = out.qtab.size();
+    out.qtab.resize(at + (len + unit - 1) / unit * unit, (R)0);
+    R* T = out.qtab.data() + at;
+    for (int tid = 0; tid < 256; ++tid) {
+      double c = g.c0;
+      for (int t = 0; t < 8; ++t) {
+        if (!((tid >> t) & 1)) continue;
+        c += lt[t];
+        for (int u = t + 1; u < 8; ++u)
+          if ((tid >> u) & 1) c += att[t][u];
+      }
+      T[tid] = fr(c);
+      for (int j = 0; j < eb; ++j) {
+        double l = lr[j];
+        for (int t = 0; t < 8; ++t)
+          if ((tid >> t) & 1) l += atr[t][j];
+        T[256 * (1 + j) + tid] = fr(l);
+      }
+    }
+    RegOp<R> r{};
+    r.kind = R_QPHASE;
+    r.s0 = (int32_t)(at / unit);
+    r.pad = nch;
+    for (int k = 1; k < eb; ++k)
+      for (int j = 0; j < k; ++j) r.m[k * (k - 1) / 2 + j] = (R)fr(arr[j][k]);
+    R* C = T + 256 * (1 + eb);
+    for (int c = 0; c < nch; ++c) {
+      const std::vector<int>& qs = g.chunks[c];  // ascending
+      const int cnt = (int)qs.size();
+      uint64_t cm = 0;
+      for (int q : qs) cm |= bit(q);
+      set_qphase_chunk(r, c, cm);
+      for (int pat = 0; pat < (1 << cnt); ++pat) {
+        double k0 = 0, kt[8] = {}, kr[8] = {};
+        for (int i = 0; i < cnt; ++i) {
+          if (!((pat >> i) & 1)) continue;
+          auto li = lg.find(qs[i]);
+          if (li != lg.end()) k0 += li->second;
+          for (int i2 = i + 1; i2 < cnt; ++i2) {
+            if (!((pat >> i2) & 1)) continue;
+            auto gi = agg.find({std::min(qs[i], qs[i2]), std::max(qs[i], qs[i2])});
+            if (gi != agg.end()) k0 += gi->second;
+          }
+          auto ti = agt.find(qs[i]);
+          if (ti != agt.end())
+            for (int t = 0; t < 8; ++t) kt[t] += ti->second[t];
+          auto ri = agr.find(qs[i]);
+          if (ri != agr.end())
+            for (int j = 0; j < eb; ++j) kr[j] += ri->second[j];
+        }
+        R* e = C + (size_t)c * kQPhaseChunkFloats + 16 * pat;
+        e[0] = fr(k0);
+        for (int t = 0; t < 8; ++t) e[1 + t] = fr(kt[t]);
+        for (int j = 0; j < eb; ++j) e[9 + j] = fr(kr[j]);
+      }
+    }
+    out.qphase_gates += g.members.size();
+    prog.push_back(r);
+  };
   auto emit = [&](const L& o) {
+    if (o.kind == kQPhaseOp) {
+      emit_qphase(qgroups[o.grp]);
+      return;
+    }
     if (o.kind == OP_SWAP && o.cm == 0 && o.gc == 0) {  // relabel
       int sa = slot(o.a), sb = slot(o.b);
       int ia = sa >= 16 ? 8 + (sa - 16) : sa, ib = sb >= 16 ? 8 + (sb - 16) : sb;
@@ -991,6 +1286,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
   };
   constexpr int kMfmaMaxSel = 2;
   auto gmask = [&](const L& o) {
+    if (o.kind == kQPhaseOp) return qgroups[o.grp].gmask;
     uint64_t m = o.gc;
     if (o.a >= kGlobalBit) m |= bit(o.a - kGlobalBit);
     if (two(o.kind) && o.b >= kGlobalBit) m |= bit(o.b - kGlobalBit);
@@ -1585,6 +1881,36 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const Reg
         v.swap(nv);
         continue;
       }
+      if (op.kind == R_QPHASE) {
+        const double* tab = rp.qtab.data() + (size_t)op.s0 * (kMfmaBlobBytes / sizeof(float));
+        const double* ch = tab + 256 * (1 + eb);
+        for (int tid = 0; tid < NTH; ++tid) {
+          double c = tab[tid], l[8];
+          for (int j = 0; j < eb; ++j) l[j] = tab[256 * (1 + j) + tid];
+          for (int k = 0; k < op.pad; ++k) {
+            uint32_t pat = 0;
+            int i = 0;
+            for (uint64_t m = qphase_chunk(op, k); m; m &= m - 1, ++i)
+              pat |= (uint32_t)((base >> __builtin_ctzll(m)) & 1) << i;
+            const double* e = ch + (size_t)k * kQPhaseChunkFloats + 16 * pat;
+            c += e[0];
+            for (int t = 0; t < 8; ++t)
+              if ((tid >> t) & 1) c += e[1 + t];
+            for (int j = 0; j < eb; ++j) l[j] += e[9 + j];
+          }
+          for (int r = 0; r < E; ++r) {
+            double phi = c;
+            for (int j = 0; j < eb; ++j) {
+              if (!((r >> j) & 1)) continue;
+              phi += l[j];
+              for (int k = j + 1; k < eb; ++k)
+                if ((r >> k) & 1) phi += op.m[k * (k - 1) / 2 + j];
+            }
+            v[tid * E + r] *= std::polar(1.0, 2 * M_PI * phi);
+          }
+        }
+        continue;
+      }
       if ((base & op.gc) != op.gc) continue;  // a control outside the tile is 0 here
       cd m[16];
       for (int i = 0; i < 16; ++i) m[i] = cd(op.m[2 * i], op.m[2 * i + 1]);
@@ -1698,7 +2024,7 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
   const bool reg = po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13;
   for (const Pass& p : passes) {
     if (reg) {
-      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8, po.mfma, true), K - 8);
+      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8, po.mfma, true, 1 << 20, po.qphase), K - 8);
     } else {
       std::vector<DevOp<double>> dops;
       append_dev_ops<double>(p, dops);
@@ -1709,7 +2035,8 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
 }
 
 template <typename R>
-static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb, bool mfma, bool compile) {
+static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb, bool mfma, bool compile,
+                               bool qphase) {
   std::vector<RegOp<R>> prog;
   std::vector<uint16_t> blob;  // R_MFMA operand blobs of the whole run
   std::vector<PassArgs> args;
@@ -1720,7 +2047,10 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     pa.n = s->n;
     pa.op0 = (int)prog.size();
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
-    RegProgram<R> pr = build_reg_program<R>(p, eb, 8, mfma, true);
+    // the device reads R_QPHASE tables as float
+    RegProgram<R> pr = build_reg_program<R>(p, eb, 8, mfma, true, 1 << 20, qphase && std::is_same<R, float>::value);
+    for (RegOp<R>& r : pr.ops)  // R_QPHASE tables follow the blocks' operand blobs
+      if (r.kind == R_QPHASE) r.s0 += (int32_t)pr.blocks.size();
     for (int slot = 0; slot < 8 + eb; ++slot) {  // thread slots 0..7, register slots 8..
       pa.ql[slot] = p.qubits[pr.map0[slot]];
       pa.qf[slot] = p.qubits[pr.map1[slot]];
@@ -1738,6 +2068,12 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
       blob.resize(at + kMfmaBlobBytes / 2);
       mfma_blob(M, blob.data() + at);
     }
+    if (!pr.qtab.empty()) {
+      const size_t at = blob.size();
+      blob.resize(at + pr.qtab.size() * sizeof(R) / 2);
+      std::memcpy(blob.data() + at, pr.qtab.data(), pr.qtab.size() * sizeof(R));
+    }
+    s->stats.qphase_gates += pr.qphase_gates;
     s->stats.mfma_blocks += pr.blocks.size();
     s->stats.mfma_gates += pr.fused_ops;
     pa.scaled = pr.blocks.empty() ? 0 : 1;
@@ -1792,7 +2128,7 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
   const int K = std::min(po.tile_qubits, s->n);
   if (po.registers && std::is_same<R, float>::value && K >= 11 && K <= 13) {
     s->stats.device_ops += ops.size();
-    return run_tiled_reg<R>(s, passes, K - 8, po.mfma, po.compile);
+    return run_tiled_reg<R>(s, passes, K - 8, po.mfma, po.compile, po.qphase);
   }
   std::vector<DevOp<R>> dops;
   std::vector<PassArgs> args;
@@ -1834,7 +2170,7 @@ NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o) {
   std::vector<RegOp<float>> prog;
   std::vector<PassArgs> args;
   for (const Pass& p : plan_tiles(ops, c.n, po)) {
-    const RegProgram<float> pr = build_reg_program<float>(p, K - 8, 8, po.mfma, true);
+    const RegProgram<float> pr = build_reg_program<float>(p, K - 8, 8, po.mfma, true, 1 << 20, po.qphase);
     PassArgs pa{};
     pa.op0 = (int)prog.size();
     pa.nops = (int)pr.ops.size();
@@ -1857,6 +2193,7 @@ PlanOptions plan_options(const NqpuRunOptions& o, int prec) {
   po.registers = (o.flags & NQPU_RUN_LDS_TILES) == 0;
   po.mfma = (o.flags & NQPU_RUN_NO_MFMA) == 0;
   po.compile = (o.flags & NQPU_RUN_COMPILE) != 0;
+  po.qphase = (o.flags & NQPU_RUN_NO_QPHASE) == 0;
   return po;
 }
 
@@ -1895,13 +2232,14 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
     const int K = std::min(po.tile_qubits, c.n);
     if (po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13)
       for (const Pass& p : passes) {
-        const RegProgram<double> rp = build_reg_program<double>(p, K - 8, 8, po.mfma, true);
+        const RegProgram<double> rp = build_reg_program<double>(p, K - 8, 8, po.mfma, true, 1 << 20, po.qphase);
         for (const RegOp<double>& r : rp.ops) {
           st.remaps += r.kind == R_REMAP;
           st.exchanges += r.kind == R_XCHG;
         }
         st.mfma_blocks += rp.blocks.size();
         st.mfma_gates += rp.fused_ops;
+        st.qphase_gates += rp.qphase_gates;
       }
   }
   return st;

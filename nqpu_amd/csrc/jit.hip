@@ -56,6 +56,9 @@ void emit_op(std::ostringstream& o, const RegOp<float>& r, int k, int eb) {
       else
         o << "    nqpu::reg_xchg_reg<float2, " << eb << ", " << r.s0 << ", " << (r.s1 - 16) << ">(v);\n";
       return;
+    case R_QPHASE:
+      o << "    nqpu::reg_qphase<float, float2, " << eb << ">(v, ops[" << k << "], blob, tid, base);\n";
+      return;
     default:
       break;
   }

@@ -337,6 +337,68 @@ __device__ __forceinline__ float sub_half(unsigned h, float x) {
   return r;
 }
 
+// R_QPHASE: the phase of a run of diagonal gates as one quadratic form Phi of
+// the index bits (layout in devtypes.h).  Table entries and coefficients are
+// reduced to [-1/2, 1/2) revolutions on the host; the sums per thread are
+// reduced again before they are combined, so Phi(r) stays within a few
+// revolutions and keeps an absolute error of a few 1e-7.  v_sin_f32 and
+// v_cos_f32 take their argument in revolutions.
+template <typename R, typename T, int EB>
+__device__ __forceinline__ void reg_qphase(T (&v)[1 << EB], const RegOp<R>& op, const unsigned char* blob, int tid,
+                                           uint64_t base) {
+  const float* tab = reinterpret_cast<const float*>(blob + (size_t)op.s0 * kMfmaBlobBytes);
+  float c = tab[tid];
+  float l[EB];
+#pragma unroll
+  for (int j = 0; j < EB; ++j) l[j] = tab[256 * (1 + j) + tid];
+  // qubits outside the tile: one table row per chunk, chosen by the tile's
+  // base index (uniform over the workgroup)
+  const float* ch = tab + 256 * (1 + EB);
+  if (op.pad > 0) {
+    float ks[16];  // sum of the chunk rows (uniform)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) ks[i] = 0.0f;
+    for (int k = 0; k < op.pad; ++k) {
+      uint32_t p = 0;
+      int i = 0;
+      for (uint64_t m = qphase_chunk(op, k); m; m &= m - 1, ++i)
+        p |= (uint32_t)((base >> __builtin_ctzll(m)) & 1) << i;
+      const float* e = ch + (size_t)k * kQPhaseChunkFloats + 16 * p;
+#pragma unroll
+      for (int x = 0; x < 16; ++x) ks[x] += e[x];
+    }
+    float kt = ks[0];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) kt += ((tid >> t) & 1) ? ks[1 + t] : 0.0f;
+    c += kt;
+#pragma unroll
+    for (int j = 0; j < EB; ++j) l[j] += ks[9 + j];
+  }
+  c -= rintf(c);
+#pragma unroll
+  for (int j = 0; j < EB; ++j) l[j] -= rintf(l[j]);
+  // Phi over the register index: r = rest + 2^j with j its lowest set bit
+  float ph[1 << EB];
+  ph[0] = c;
+#pragma unroll
+  for (int r = 1; r < (1 << EB); ++r) {
+    const int j = __builtin_ctz(r), rest = r & (r - 1);
+    float x = ph[rest] + l[j];
+#pragma unroll
+    for (int k = j + 1; k < EB; ++k)
+      if ((rest >> k) & 1) x += (float)op.m[k * (k - 1) / 2 + j];
+    ph[r] = x;
+  }
+#pragma unroll
+  for (int r = 0; r < (1 << EB); ++r) {
+    const float x = ph[r] - rintf(ph[r]);
+    T z;
+    z.x = __builtin_amdgcn_cosf(x);
+    z.y = __builtin_amdgcn_sinf(x);
+    v[r] = cmul_(v[r], z);
+  }
+}
+
 // R_MFMA: a dense 16x16 complex block on the matrix cores.  Per register
 // half e (register bit 3), the wavefront's 32 columns (lane bits 0..4) are 32
 // vectors of 16 amplitudes: K-step s holds the 8 reals of a3 = s, a2 = lane
@@ -500,6 +562,7 @@ __device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<
         break;
       }
       case R_XCHG: NQPU_SLOT_CASES(reg_xchg, v, op, tid & 63) break;
+      case R_QPHASE: reg_qphase<R, T, EB>(v, op, blob, tid, base); break;
       case R_U1: NQPU_SLOT_CASES(reg_u1, v, op, on) break;
       case R_FLIP: NQPU_SLOT_CASES(reg_flip, v, op, on) break;
       case R_D1:

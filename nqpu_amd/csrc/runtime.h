@@ -63,6 +63,9 @@ struct PlanOptions {
   // run complex64 register tile passes as straight-line kernels compiled at
   // run time (jit.hip) instead of the interpreting kernel
   bool compile = false;
+  // merge runs of diagonal gates in a register tile pass into one quadratic
+  // phase op (R_QPHASE)
+  bool qphase = true;
 };
 
 // lowering and planning (pure host code; unit-testable without a device)

@@ -117,32 +117,33 @@ class NativeCircuit:
         L.check(lib.nqpu_circuit_add_gates(self.ptr, ops, len(circuit.gates)), "nqpu_circuit_add_gates")
 
     def plan(self, precision="c64", fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
-             lds_tiles=False, mfma=True, compile=False):
+             lds_tiles=False, mfma=True, compile=False, qphase=True):
         lib = L.load()
         st = L.NqpuRunStats()
         o = run_options(fusion, semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma,
-                        compile=compile)
+                        compile=compile, qphase=qphase)
         L.check(lib.nqpu_circuit_plan(self.ptr, ct.byref(o), _prec(precision), ct.byref(st)), "plan")
         return _stats(st)
 
     def emulate(self, amps, precision="c64", semantics="standard", tile_qubits=0, low_qubits=0,
-                lds_tiles=False, batch_layout=False, mfma=True):
+                lds_tiles=False, batch_layout=False, mfma=True, qphase=True):
         """Host emulation of the tiled GPU execution (plan, per-pass programs
         and kernel semantics) in FP64; returns the new amplitudes.  With
         batch_layout, emulates the layout the batched kernel uses."""
         a = np.ascontiguousarray(amps, dtype=np.complex128).copy()
-        o = run_options("tiled", semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma)
+        o = run_options("tiled", semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma,
+                        qphase=qphase)
         if batch_layout:
             o.flags |= L.RUN_BATCH_LAYOUT
         L.check(L.load().nqpu_debug_emulate(self.ptr, ct.byref(o), _prec(precision), a.ctypes.data, len(a)),
                 "emulate")
         return a
 
-    def check_compile(self, semantics="standard", mfma=True):
+    def check_compile(self, semantics="standard", mfma=True, qphase=True):
         """Generate and compile (hipRTC, no device) the straight-line pass
         kernels a complex64 run with compile=True would use; raises with the
         compiler log on failure."""
-        o = run_options("tiled", semantics, mfma=mfma, compile=True)
+        o = run_options("tiled", semantics, mfma=mfma, compile=True, qphase=qphase)
         L.check(L.load().nqpu_debug_compile_passes(self.ptr, ct.byref(o)), "compile passes")
 
     def fusion_counts(self):
@@ -169,17 +170,20 @@ _SEMANTICS = {"standard": L.SEMANTICS_STANDARD, "reference": L.SEMANTICS_REFEREN
 
 
 def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
-                time_passes=False, lds_tiles=False, mfma=True, compile=False) -> L.NqpuRunOptions:
+                time_passes=False, lds_tiles=False, mfma=True, compile=False, qphase=True) -> L.NqpuRunOptions:
     """mfma=False keeps every gate of complex64 register tiles on the vector
     ALUs (no fused blocks on the matrix cores); compile=True runs complex64
-    register tile passes as straight-line kernels compiled at run time."""
+    register tile passes as straight-line kernels compiled at run time;
+    qphase=False applies runs of diagonal gates one by one instead of as one
+    quadratic phase per run."""
     o = L.NqpuRunOptions()
     o.fusion = _FUSION[fusion]
     o.semantics = _SEMANTICS[semantics]
     o.tile_qubits = tile_qubits
     o.low_qubits = low_qubits
     o.flags = ((L.RUN_TIME_PASSES if time_passes else 0) | (L.RUN_LDS_TILES if lds_tiles else 0)
-               | (0 if mfma else L.RUN_NO_MFMA) | (L.RUN_COMPILE if compile else 0))
+               | (0 if mfma else L.RUN_NO_MFMA) | (L.RUN_COMPILE if compile else 0)
+               | (0 if qphase else L.RUN_NO_QPHASE))
     return o
 
 
@@ -187,7 +191,7 @@ def _stats(st: L.NqpuRunStats) -> dict:
     return {"gates": st.gates, "device_ops": st.device_ops, "passes": st.passes, "bytes": st.bytes_moved,
             "pass_ms": st.pass_ms, "timed_passes": st.timed_passes, "remaps": st.remaps,
             "mfma_blocks": st.mfma_blocks, "mfma_gates": st.mfma_gates,
-            "exchanges": st.exchanges}
+            "exchanges": st.exchanges, "qphase_gates": st.qphase_gates}
 
 
 def _prec(p) -> int:
@@ -271,10 +275,11 @@ class StateVector:
 
     # --- execution
     def run(self, circuit: Circuit | NativeCircuit, fusion="tiled", semantics="standard",
-            tile_qubits=0, low_qubits=0, time_passes=False, lds_tiles=False, mfma=True, compile=False) -> dict:
+            tile_qubits=0, low_qubits=0, time_passes=False, lds_tiles=False, mfma=True, compile=False,
+            qphase=True) -> dict:
         lib = L.load()
         nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
-        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles, mfma, compile)
+        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles, mfma, compile, qphase)
         L.check(lib.nqpu_run(nc.ptr, self.ptr, ct.byref(o)), "nqpu_run")
         return self.last_stats()
 

@@ -122,6 +122,29 @@ def test_compiled_passes_match_oracle(oracle, name):
         assert infidelity(out, exp) < 1e-6, compiled
 
 
+@pytest.mark.parametrize("n,tile,low,count", [(17, 12, 5, 200), (20, 11, 3, 90), (20, 13, 6, 150)])
+def test_quadratic_phase_runs_match_oracle(oracle, n, tile, low, count):
+    """Runs of diagonal gates merged into one quadratic phase per run, with
+    its angle tables over thread slots, register slots and chunks of qubits
+    outside the tile (hardware sin/cos in revolutions): interpreted and, for
+    12-qubit tiles, compiled; the same circuit with the gates applied one by
+    one agrees too."""
+    from tests.test_planner_emulation import diagonal_heavy_circuit, random_state
+    c = diagonal_heavy_circuit(n, count, 40 + n + tile)
+    a0 = random_state(n, 3)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    for compiled in ((False, True) if tile == 12 else (False,)):
+        for qphase in (True, False):
+            with StateVector(n, "c64") as sv:
+                sv.set_amplitudes(a0)
+                st = sv.run(c, tile_qubits=tile, low_qubits=low, compile=compiled, qphase=qphase)
+                out = sv.amplitudes()
+            assert (st["qphase_gates"] > 0) == qphase
+            assert np.max(np.abs(out - exp)) < 2e-5, (compiled, qphase)
+            assert infidelity(out, exp) < 1e-6, (compiled, qphase)
+
+
 def test_compiled_passes_c1_bitwise_equal_interpreted(c1):
     """The compiled and interpreted kernels execute the same operations in the
     same order on the same registers: identical complex64 results."""

@@ -82,6 +82,51 @@ def test_emulated_reference_semantics(oracle):
     assert np.max(np.abs(got - exp)) < 1e-12
 
 
+DIAGONAL_KINDS = ["z", "s", "t", "rz", "phase", "cz", "crz", "cr", "rzz", "ccz"]
+MIXING_KINDS = ["h", "rx", "u", "cnot", "swap", "cswap"]
+
+
+def diagonal_heavy_circuit(n, count, seed):
+    """Long runs of one- and two-qubit diagonal gates (merged into quadratic
+    phase ops), some with three qubits (CCZ: left as single ops), broken up by
+    a few mixing gates; many touch only qubits outside the tile."""
+    rng = np.random.default_rng(seed)
+    c = Circuit(n)
+    for _ in range(count):
+        pool = DIAGONAL_KINDS if rng.random() < 0.8 else MIXING_KINDS
+        k, t, cq, p, cust = mg.random_gate(rng, n, pool[rng.integers(len(pool))])
+        c.gates.append(Gate(k, tuple(t), tuple(cq), tuple(p), cust))
+    return c
+
+
+@pytest.mark.parametrize("n,tile,low,count", [(14, 12, 5, 300), (17, 12, 5, 200), (17, 13, 6, 200),
+                                               (20, 11, 3, 90)])
+@pytest.mark.parametrize("qphase", [True, False])
+def test_emulated_quadratic_phase_runs(oracle, n, tile, low, count, qphase):
+    """Runs of diagonal gates merged into one quadratic phase per run (tables
+    over thread slots, register slots and chunks of qubits outside the tile,
+    up to 9 such qubits at n = 20) give the oracle's state; with qphase=False
+    the same gates run one by one."""
+    c = diagonal_heavy_circuit(n, count, 40 + n + tile)
+    a0 = random_state(n, 3)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    nc = c.native()
+    got = nc.emulate(a0, "c64", tile_qubits=tile, low_qubits=low, qphase=qphase)
+    assert np.max(np.abs(got - exp)) < 1e-12
+    merged = nc.plan("c64", tile_qubits=tile, low_qubits=low, qphase=qphase)["qphase_gates"]
+    assert (merged > count // 4) if qphase else merged == 0
+
+
+def test_qft_controlled_phases_merge():
+    """A 30-qubit QFT's controlled phases merge into one quadratic phase per
+    Hadamard, so far fewer matrix-core blocks remain."""
+    nc = qft(30).native()
+    on, off = nc.plan("c64", compile=True), nc.plan("c64", compile=True, qphase=False)
+    assert on["qphase_gates"] >= 400 and off["qphase_gates"] == 0
+    assert on["mfma_blocks"] * 4 < off["mfma_blocks"]
+
+
 @pytest.mark.parametrize("n", [4, 5, 8, 11, 12, 13, 15])
 def test_emulated_batch_layout_matches_oracle(oracle, n):
     """The register programs the batched kernel runs: whole-state tiles with
