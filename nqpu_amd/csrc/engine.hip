@@ -1310,30 +1310,29 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         default: return 0.35;
       }
     };
-    // Blocks are grown as light cones: starting from a seed op, every later op
-    // whose tile bits fit the (at most 4-bit) set, whose qubits outside the
-    // tile fit the selector set, and which no op left behind precedes on a
-    // shared bit joins.  Each round tries the first few ready ops as seeds and
-    // keeps the heaviest block; an op left behind blocks its bits, so every
-    // op taken commutes with all the ops it overtakes.
+    // A block is the light cone of a set S of at most four tile bits: every
+    // pending op whose tile bits lie in S, whose qubits outside the tile fit
+    // the selector set, and which no op left behind precedes on a shared bit.
+    // An op left behind blocks its bits, so every op taken commutes with all
+    // the ops it overtakes.  Each round evaluates the cone of every 4-subset
+    // of the tile bits the next pending ops touch (at most C(12, 4) = 495
+    // scans of the pending ops) and keeps the heaviest: growing one set from a
+    // seed op in circuit order lets the first ops met claim the four bits, and
+    // gave 27 % more blocks on a 30-qubit U3 + CNOT brickwork.
     std::vector<int> rem(ops.size());
     for (int k = 0; k < (int)ops.size(); ++k) rem[k] = k;
-    auto grow = [&](int seed, std::vector<int>* taken, std::vector<int>* left) {
-      uint32_t S = qmask(ops[seed]), blocked = 0;
-      uint64_t G = gmask(ops[seed]);
+    auto cone = [&](uint32_t S, std::vector<int>* taken, std::vector<int>* left) {
+      uint32_t blocked = 0;
+      uint64_t G = 0;
       double w = 0;
       for (int k : rem) {
         const L& o = ops[k];
         const uint32_t qm = qmask(o);
-        bool ok = (qm & blocked) == 0 && fusable(o);
+        bool ok = (qm & blocked) == 0 && (qm & ~S) == 0 && fusable(o);
         if (ok) {
-          const uint32_t S2 = S | qm;
           const uint64_t G2 = G | gmask(o);
-          ok = __builtin_popcount(S2) <= 4 && __builtin_popcountll(G2) <= kMfmaMaxSel;
-          if (ok) {
-            S = S2;
-            G = G2;
-          }
+          ok = __builtin_popcountll(G2) <= kMfmaMaxSel;
+          if (ok) G = G2;
         }
         if (ok) {
           w += weight(o);
@@ -1345,7 +1344,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       }
       return w;
     };
-    constexpr int kSeeds = 8;
+    constexpr int kWindow = 24;  // pending ops whose bits are candidates
     int nvar = 0;  // operand blobs of the blocks so far (one per variant)
     while (!rem.empty()) {
       const L& first = ops[rem.front()];
@@ -1354,25 +1353,41 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         rem.erase(rem.begin());
         continue;
       }
-      int best = -1, tried = 0;
-      double bw = -1;
-      uint32_t blocked = 0;
+      uint32_t touched = 0;
+      int seen = 0;
       for (int k : rem) {
-        if (tried == kSeeds) break;
-        const L& o = ops[k];
-        const uint32_t qm = qmask(o);
-        if ((qm & blocked) == 0 && fusable(o) && qm != 0) {
-          ++tried;
-          const double w = grow(k, nullptr, nullptr);
-          if (w > bw + 1e-9) {
-            bw = w;
-            best = k;
-          }
-        }
-        blocked |= qm;
+        if (seen++ == kWindow) break;
+        if (fusable(ops[k])) touched |= qmask(ops[k]);
+      }
+      uint32_t bestS = touched;
+      double bw = -1;
+      if (__builtin_popcount(touched) <= 4) {
+        bw = cone(touched, nullptr, nullptr);
+      } else {
+        int bits[32], nb = 0;
+        for (int q = 0; q < K; ++q)
+          if ((touched >> q) & 1) bits[nb++] = q;
+        const uint32_t f = qmask(first);
+        for (int i0 = 0; i0 < nb; ++i0)
+          for (int i1 = i0 + 1; i1 < nb; ++i1)
+            for (int i2 = i1 + 1; i2 < nb; ++i2)
+              for (int i3 = i2 + 1; i3 < nb; ++i3) {
+                const uint32_t S = (1u << bits[i0]) | (1u << bits[i1]) | (1u << bits[i2]) | (1u << bits[i3]);
+                // ties go to a set holding the first pending op's bits
+                const double w = cone(S, nullptr, nullptr) + ((S & f) == f ? 1e-6 : 0.0);
+                if (w > bw + 1e-9) {
+                  bw = w;
+                  bestS = S;
+                }
+              }
       }
       std::vector<int> taken, left;
-      grow(best, &taken, &left);
+      cone(bestS, &taken, &left);
+      if (taken.empty()) {  // nothing weighs anything (relabelling SWAPs only): the first op alone
+        items.push_back(Item{rem.front(), -1, qmask(first), false});
+        rem.erase(rem.begin());
+        continue;
+      }
       Blk b;
       for (int k : taken) {
         b.q |= qmask(ops[k]);
