@@ -2036,10 +2036,12 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
   PlanOptions po = plan_options(o, prec);
   std::vector<Pass> passes = plan_tiles(ops, c.n, po);
   const int K = std::min(po.tile_qubits, c.n);
-  const bool reg = po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13;
+  const bool reg = po.registers && K >= 11 && K <= 13;
+  const bool c64 = prec == NQPU_PRECISION_C64;
   for (const Pass& p : passes) {
     if (reg) {
-      emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8, po.mfma, true, 1 << 20, po.qphase), K - 8);
+      emulate_reg_pass(a, c.n, p,
+                       build_reg_program<double>(p, K - 8, 8, po.mfma && c64, c64, 1 << 20, po.qphase && c64), K - 8);
     } else {
       std::vector<DevOp<double>> dops;
       append_dev_ops<double>(p, dops);
@@ -2062,8 +2064,11 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     pa.n = s->n;
     pa.op0 = (int)prog.size();
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
-    // the device reads R_QPHASE tables as float
-    RegProgram<R> pr = build_reg_program<R>(p, eb, 8, mfma, true, 1 << 20, qphase && std::is_same<R, float>::value);
+    // matrix-core blocks, lane exchanges and quadratic phases (float tables)
+    // exist for complex64 only; complex128 programs use LDS remaps and
+    // vector-ALU gates
+    constexpr bool c64 = std::is_same<R, float>::value;
+    RegProgram<R> pr = build_reg_program<R>(p, eb, 8, mfma && c64, c64, 1 << 20, qphase && c64);
     for (RegOp<R>& r : pr.ops)  // R_QPHASE tables follow the blocks' operand blobs
       if (r.kind == R_QPHASE) r.s0 += (int32_t)pr.blocks.size();
     for (int slot = 0; slot < 8 + eb; ++slot) {  // thread slots 0..7, register slots 8..
@@ -2141,7 +2146,7 @@ template <typename R>
 static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanOptions& po) {
   std::vector<Pass> passes = plan_tiles(ops, s->n, po);
   const int K = std::min(po.tile_qubits, s->n);
-  if (po.registers && std::is_same<R, float>::value && K >= 11 && K <= 13) {
+  if (po.registers && K >= 11 && K <= 13) {
     s->stats.device_ops += ops.size();
     return run_tiled_reg<R>(s, passes, K - 8, po.mfma, po.compile, po.qphase);
   }
@@ -2245,9 +2250,11 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
     st.device_ops = ops.size();
     st.bytes_moved = (double)passes.size() * 2.0 * (double)(asz << c.n);
     const int K = std::min(po.tile_qubits, c.n);
-    if (po.registers && prec == NQPU_PRECISION_C64 && K >= 11 && K <= 13)
+    const bool c64 = prec == NQPU_PRECISION_C64;
+    if (po.registers && K >= 11 && K <= 13)
       for (const Pass& p : passes) {
-        const RegProgram<double> rp = build_reg_program<double>(p, K - 8, 8, po.mfma, true, 1 << 20, po.qphase);
+        const RegProgram<double> rp =
+            build_reg_program<double>(p, K - 8, 8, po.mfma && c64, c64, 1 << 20, po.qphase && c64);
         for (const RegOp<double>& r : rp.ops) {
           st.remaps += r.kind == R_REMAP;
           st.exchanges += r.kind == R_XCHG;

@@ -17,6 +17,14 @@ struct nqpu_is_float2<float2> {
   static constexpr bool value = true;
 };
 
+// on ? a : b, per component: a select of whole HIP double2 values makes the
+// compiler keep the amplitude array in scratch memory
+template <typename T> __device__ __forceinline__ T sel_(bool on, T a, T b) {
+  T r;
+  r.x = on ? a.x : b.x;
+  r.y = on ? a.y : b.y;
+  return r;
+}
 template <typename T> __device__ __forceinline__ T cmul_(T a, T b) {
   T r;
   r.x = a.x * b.x - a.y * b.y;
@@ -102,8 +110,8 @@ __device__ __forceinline__ void reg_u1(T (&v)[1 << EB], const RegOp<R>& op, bool
     T x = v[r0], y = v[r1];
     T nx = mac2<R>(m0r, m0i, m1r, m1i, x, y);
     T ny = mac2<R>(m2r, m2i, m3r, m3i, x, y);
-    v[r0] = on ? nx : x;
-    v[r1] = on ? ny : y;
+    v[r0] = sel_(on, nx, x);
+    v[r1] = sel_(on, ny, y);
   }
 }
 
@@ -128,8 +136,8 @@ __device__ __forceinline__ void reg_flip(T (&v)[1 << EB], const RegOp<R>& op, bo
     if ((r0 & cr) != cr) continue;
     const int r1 = r0 | (1 << J);
     T x = v[r0], y = v[r1];
-    v[r0] = on ? y : x;
-    v[r1] = on ? x : y;
+    v[r0] = sel_(on, y, x);
+    v[r1] = sel_(on, x, y);
   }
 }
 
@@ -155,7 +163,7 @@ __device__ __forceinline__ void reg_u2(T (&v)[1 << EB], const RegOp<R>& op, bool
         acc.x += mr * x[c].x - mi * x[c].y;
         acc.y += mr * x[c].y + mi * x[c].x;
       }
-      v[idx[row]] = on ? acc : x[row];
+      v[idx[row]] = sel_(on, acc, x[row]);
     }
   }
 }
@@ -169,44 +177,41 @@ __device__ __forceinline__ void reg_swap(T (&v)[1 << EB], const RegOp<R>& op, bo
     if ((r & cr) != cr) continue;
     const int r2 = r ^ (1 << J1) ^ (1 << J2);
     T x = v[r], y = v[r2];
-    v[r] = on ? y : x;
-    v[r2] = on ? x : y;
+    v[r] = sel_(on, y, x);
+    v[r2] = sel_(on, x, y);
   }
 }
 
 template <typename R, typename T, int EB>
 __device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, int tid, bool on, uint64_t base) {
-  T p0, p1, p2, p3;
-  p0.x = op.m[0]; p0.y = op.m[1];
-  p1.x = op.m[2]; p1.y = op.m[3];
-  p2.x = op.m[4]; p2.y = op.m[5];
-  p3.x = op.m[6]; p3.y = op.m[7];
+  R p0r = op.m[0], p0i = op.m[1], p1r = op.m[2], p1i = op.m[3];
+  R p2r = op.m[4], p2i = op.m[5], p3r = op.m[6], p3i = op.m[7];
   const bool two = op.kind == R_D2;
   int a = op.s0, b = two ? op.s1 : 16;
   if (!two) {  // a single factor: the table does not depend on position b
-    p2 = p0;
-    p3 = p1;
+    p2r = p0r; p2i = p0i;
+    p3r = p1r; p3i = p1i;
   }
   // a factor on a qubit outside the tile is a fixed bit here: fold it into
   // the phase table, leaving a diagonal on the other position (or a scalar,
   // applied through register slot 0 with equal phases)
   if (b >= kGlobalBit) {
     if ((base >> (b - kGlobalBit)) & 1) {
-      p0 = p2;
-      p1 = p3;
+      p0r = p2r; p0i = p2i;
+      p1r = p3r; p1i = p3i;
     }
     b = 16;
-    p2 = p0;
-    p3 = p1;
+    p2r = p0r; p2i = p0i;
+    p3r = p1r; p3i = p1i;
   }
   if (a >= kGlobalBit) {
     if ((base >> (a - kGlobalBit)) & 1) {
-      p0 = p1;
-      p2 = p3;
+      p0r = p1r; p0i = p1i;
+      p2r = p3r; p2i = p3i;
     }
     a = 16;
-    p1 = p0;
-    p3 = p2;
+    p1r = p0r; p1i = p0i;
+    p3r = p2r; p3i = p2i;
   }
   const bool areg = a >= 16, breg = b >= 16;
   const int as = a & 15, bs = b & 15;
@@ -220,8 +225,8 @@ __device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, in
       const bool ba = (r >> as) & 1;
       const bool bb = (r >> bs) & 1;
       T ph;
-      ph.x = bb ? (ba ? p3.x : p2.x) : (ba ? p1.x : p0.x);
-      ph.y = bb ? (ba ? p3.y : p2.y) : (ba ? p1.y : p0.y);
+      ph.x = bb ? (ba ? p3r : p2r) : (ba ? p1r : p0r);
+      ph.y = bb ? (ba ? p3i : p2i) : (ba ? p1i : p0i);
       v[r] = cmul_(ph, v[r]);
     }
     return;
@@ -232,15 +237,11 @@ __device__ __forceinline__ void reg_diag(T (&v)[1 << EB], const RegOp<R>& op, in
     if ((r & cr) != cr) continue;
     const bool ba = areg ? ((r >> as) & 1) : ta;
     const bool bb = breg ? ((r >> bs) & 1) : tb;
-    T lo, hi, ph;
-    lo.x = ba ? p1.x : p0.x;
-    lo.y = ba ? p1.y : p0.y;
-    hi.x = ba ? p3.x : p2.x;
-    hi.y = ba ? p3.y : p2.y;
-    ph.x = bb ? hi.x : lo.x;
-    ph.y = bb ? hi.y : lo.y;
+    T ph;
+    ph.x = bb ? (ba ? p3r : p2r) : (ba ? p1r : p0r);
+    ph.y = bb ? (ba ? p3i : p2i) : (ba ? p1i : p0i);
     T nv = cmul_(ph, v[r]);
-    v[r] = on ? nv : v[r];
+    v[r] = sel_(on, nv, v[r]);
   }
 }
 
@@ -561,7 +562,11 @@ __device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<
         reg_mfma<T, EB>(v, blob, op.s0 + var, tid & 63);
         break;
       }
-      case R_XCHG: NQPU_SLOT_CASES(reg_xchg, v, op, tid & 63) break;
+      case R_XCHG:  // complex64 programs only (the complex128 build keeps it out: its double2 swaps went to scratch)
+        if constexpr (nqpu_is_float2<T>::value) {
+          NQPU_SLOT_CASES(reg_xchg, v, op, tid & 63)
+        }
+        break;
       case R_QPHASE: reg_qphase<R, T, EB>(v, op, blob, tid, base); break;
       case R_U1: NQPU_SLOT_CASES(reg_u1, v, op, on) break;
       case R_FLIP: NQPU_SLOT_CASES(reg_flip, v, op, on) break;
