@@ -225,6 +225,10 @@ typedef struct {
 /* complex64 register tiles: apply runs of diagonal gates one by one instead
  * of merging each run into one quadratic phase evaluated per amplitude */
 #define NQPU_RUN_NO_QPHASE 32u
+/* complex64 register tiles: keep the lowest qubits of the state in every
+ * tile instead of handing the tile's low positions to the qubits the next
+ * pass works on (which needs passes that only permute at the end of a run) */
+#define NQPU_RUN_FIXED_LOW 64u
 
 /* Statistics of the last run on a state. */
 typedef struct {

@@ -76,6 +76,7 @@ RUN_BATCH_LAYOUT = 4
 RUN_NO_MFMA = 8
 RUN_COMPILE = 16
 RUN_NO_QPHASE = 32
+RUN_FIXED_LOW = 64
 
 
 class NqpuFusionCounts(ct.Structure):

@@ -101,6 +101,11 @@ struct PassArgs {
   int32_t scaled;
   // register tile pass: bytes of LDS for R_REMAP (0 when the program has none)
   int32_t lds_remap;
+  // register tile pass: nonzero when a wavefront stores amplitudes that
+  // another wavefront of the workgroup loaded (the pass permutes the
+  // positions held by the wave slots) and no R_REMAP barrier separates the
+  // loads from the stores: all loads of the workgroup complete first
+  int32_t sync_store;
 };
 constexpr float kMfmaScale = 32768.0f;
 

@@ -298,7 +298,7 @@ std::vector<HostOp> merge_single_qubit(const std::vector<HostOp>& ops) {
 // its qubits fit in S; otherwise its qubits become blocked.  Ops that join
 // commute with every op they overtake (disjoint qubits), so the order of
 // application is preserved.
-std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po) {
+static std::vector<Pass> plan_tiles_once(const std::vector<HostOp>& ops, int n, const PlanOptions& po, bool move) {
   std::vector<Pass> passes;
   const int K = std::min(po.tile_qubits, n);
   const int L = std::min(po.low_qubits, K);
@@ -350,16 +350,104 @@ std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOp
       for (; i < rem.size(); ++i) next->push_back(rem[i]);
     return w;
   };
+  // Moving low set (po.move_low, complex64 register passes of 11..13 qubits):
+  // a pass stores its tile in any order of the tile's physical positions, so
+  // the five physical low positions, which every tile must hold for coalesced
+  // loads and stores, can carry any five qubits of the previous tile.  The
+  // planner keeps a map pi from circuit qubits to physical positions; a pass
+  // is planned on circuit qubits (its tile S must contain the qubits now at
+  // the physical low positions) and emitted in physical positions.  It then
+  // hands five qubits of S to the low positions (those shared with the best
+  // next window of K consecutive qubits), puts the rest of S back at their
+  // own positions where it can, and the run ends with passes that only
+  // permute, until pi is the identity again.
+
+  std::vector<int> pi(n), inv(n);
+  for (int q = 0; q < n; ++q) pi[q] = inv[q] = q;
+  auto low_set = [&]() {
+    uint64_t m = 0;
+    for (int p = 0; p < L; ++p) m |= bit(inv[p]);
+    return m;
+  };
+  auto to_phys = [&](uint64_t m) {
+    uint64_t r = 0;
+    for (; m; m &= m - 1) r |= bit(pi[__builtin_ctzll(m)]);
+    return r;
+  };
+  // emit a pass of circuit-qubit tile S (K qubits) with the given ops, which
+  // stores circuit qubit q at newpi[q] (a permutation of pi over S)
+  auto emit_pass = [&](uint64_t S, const std::vector<size_t>& chosen, const std::vector<int>& newpi) {
+    Pass p;
+    const uint64_t P = to_phys(S);
+    for (int x = 0; x < n; ++x)
+      if ((P >> x) & 1) p.qubits.push_back(x);
+    for (size_t k : chosen) {
+      HostOp o = ops[k];
+      if (move) {
+        o.a = pi[o.a];
+        if (o.kind == OP_U2 || o.kind == OP_SWAP || o.kind == OP_D2) o.b = pi[o.b];
+        o.cmask = to_phys(o.cmask);
+      }
+      p.ops.push_back(std::move(o));
+    }
+    if (move) {
+      bool ident = true;
+      for (int x : p.qubits) {
+        p.out.push_back(newpi[inv[x]]);
+        ident = ident && p.out.back() == x;
+      }
+      if (ident) p.out.clear();
+    }
+    passes.push_back(std::move(p));
+  };
+  // new placement of the circuit qubits of S: Lnew on the low positions (those
+  // already there keep their slot), the others at home where possible
+  auto place = [&](uint64_t S, uint64_t Lnew) {
+    std::vector<int> np(pi);
+    const uint64_t P = to_phys(S);
+    uint64_t used = 0;
+    for (uint64_t m = Lnew; m; m &= m - 1) {
+      const int q = __builtin_ctzll(m);
+      if (pi[q] < L) used |= bit(np[q] = pi[q]);
+    }
+    for (uint64_t m = Lnew; m; m &= m - 1) {
+      const int q = __builtin_ctzll(m);
+      if (pi[q] < L) continue;
+      int x = 0;
+      while ((used >> x) & 1) ++x;
+      used |= bit(np[q] = x);
+    }
+    const uint64_t rest = S & ~Lnew;
+    for (uint64_t m = rest; m; m &= m - 1) {
+      const int q = __builtin_ctzll(m);
+      np[q] = -1;
+      if (((P >> q) & 1) && !((used >> q) & 1) && q >= L) used |= bit(np[q] = q);
+    }
+    for (uint64_t m = rest; m; m &= m - 1) {
+      const int q = __builtin_ctzll(m);
+      if (np[q] >= 0) continue;
+      const uint64_t fr = P & ~used & ~(bit(L) - 1);
+      used |= bit(np[q] = __builtin_ctzll(fr));
+    }
+    return np;
+  };
+  auto commit = [&](const std::vector<int>& np) {
+    pi = np;
+    for (int q = 0; q < n; ++q) inv[pi[q]] = q;
+  };
   while (!rem.empty()) {
     // candidates: the greedy set, and the low qubits plus every window of
-    // K - L consecutive qubits (circuits with 1D structure sweep well with these)
-    uint64_t best = low;
+    // consecutive qubits that fills the tile (circuits with 1D structure
+    // sweep well with these)
+    const uint64_t lowm = move ? low_set() : low;
+    uint64_t best = lowm;
     double bw = take(best, true, nullptr, nullptr);
     for (int q = 0; q < n && __builtin_popcountll(best) < K; ++q) best |= bit(q);
-    const int w = K - L;
-    if (w > 0 && n > K)
-      for (int s = L; s + w <= n; ++s) {
-        uint64_t S = low | (((bit(w) - 1)) << s);
+    if (K > L && n > K)
+      for (int s0 = 0; s0 < n; ++s0) {
+        uint64_t S = lowm;
+        for (int q = s0; q < n && __builtin_popcountll(S) < K; ++q) S |= bit(q);
+        if (__builtin_popcountll(S) < K) break;
         const double cw = take(S, false, nullptr, nullptr);
         if (cw > bw + 1e-9) {
           bw = cw;
@@ -376,14 +464,99 @@ std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOp
       next.erase(next.begin());
     }
     for (int q = 0; q < n && __builtin_popcountll(S) < K; ++q) S |= bit(q);
-    Pass p;
-    for (int q = 0; q < n; ++q)
-      if ((S >> q) & 1) p.qubits.push_back(q);
-    for (size_t k : chosen) p.ops.push_back(ops[k]);
-    passes.push_back(std::move(p));
+    std::vector<int> np(pi);
+    if (move) {
+      // the low set for the next pass: the qubits of S in the best next
+      // window of K consecutive qubits (ties: already low, then lowest)
+      uint64_t share = 0;
+      double nw = -1;
+      rem.swap(next);
+      for (int s0 = 0; s0 + K <= n && !rem.empty(); ++s0) {
+        uint64_t W = (bit(K) - 1) << s0;
+        if (__builtin_popcountll(W & S) < L) continue;
+        const double w2 = take(W, false, nullptr, nullptr) + 1e-3 * __builtin_popcountll(W & S & lowm);
+        if (w2 > nw + 1e-9) {
+          nw = w2;
+          share = W & S;
+        }
+      }
+      rem.swap(next);
+      if (next.empty()) share = S & (bit(L) - 1);  // the last pass sends the low qubits home
+      std::vector<int> cand;
+      for (uint64_t m = share; m; m &= m - 1) cand.push_back(__builtin_ctzll(m));
+      std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) {
+        return std::make_pair(pi[x] >= L, x >= L) < std::make_pair(pi[y] >= L, y >= L);
+      });
+      uint64_t Lnew = 0;
+      for (int q : cand)
+        if (__builtin_popcountll(Lnew) < L) Lnew |= bit(q);
+      for (int q = 0; q < n && __builtin_popcountll(Lnew) < L; ++q)  // fewer than L shared: keep low ones
+        if (((S >> q) & 1) && pi[q] < L) Lnew |= bit(q);
+      for (int q = 0; q < n && __builtin_popcountll(Lnew) < L; ++q)
+        if ((S >> q) & 1) Lnew |= bit(q);
+      np = place(S, Lnew);
+    }
+    emit_pass(S, chosen, np);
+    if (move) commit(np);
     rem.swap(next);
   }
+  // back to the identity: each pass holds the low positions and, cycle by
+  // cycle of the remaining permutation (shortest first, those through the
+  // low positions first), as many further positions as fit; every qubit whose
+  // position is in the pass goes home if its home is too, the others fill
+  // the positions left
+  for (int guard = 0; move && guard < 4 * n; ++guard) {
+    bool done = true;
+    for (int q = 0; q < n; ++q) done = done && pi[q] == q;
+    if (done) break;
+    std::vector<std::vector<int>> cycles;
+    std::vector<char> seen(n, 0);
+    for (int x = 0; x < n; ++x) {
+      if (seen[x] || inv[x] == x) continue;
+      std::vector<int> cyc;
+      for (int y = x; !seen[y]; y = inv[y]) {  // the qubit at y belongs at position inv[y]
+        seen[y] = 1;
+        cyc.push_back(y);
+      }
+      cycles.push_back(cyc);
+    }
+    std::stable_sort(cycles.begin(), cycles.end(), [&](const std::vector<int>& a, const std::vector<int>& b) {
+      const bool la = *std::min_element(a.begin(), a.end()) < L, lb = *std::min_element(b.begin(), b.end()) < L;
+      return la != lb ? la : a.size() < b.size();
+    });
+    uint64_t P = bit(L) - 1;
+    for (const auto& cyc : cycles)
+      for (int x : cyc)
+        if (__builtin_popcountll(P) < K) P |= bit(x);
+    for (int x = 0; x < n && __builtin_popcountll(P) < K; ++x) P |= bit(x);
+    uint64_t S = 0;
+    for (uint64_t m = P; m; m &= m - 1) S |= bit(inv[__builtin_ctzll(m)]);
+    std::vector<int> np(pi);
+    uint64_t used = 0;
+    for (uint64_t m = S; m; m &= m - 1) {
+      const int q = __builtin_ctzll(m);
+      np[q] = -1;
+      if ((P >> q) & 1) used |= bit(np[q] = q);
+    }
+    for (uint64_t m = S; m; m &= m - 1) {
+      const int q = __builtin_ctzll(m);
+      if (np[q] < 0) used |= bit(np[q] = __builtin_ctzll(P & ~used));
+    }
+    emit_pass(S, {}, np);
+    commit(np);
+  }
   return passes;
+}
+
+// With a moving low set (complex64 register passes), both plans are made and
+// the one with fewer passes kept: a QFT's final qubit reversal, for one, is
+// free inside fixed-low tiles but leaves a permutation to undo otherwise.
+std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po) {
+  const int K = std::min(po.tile_qubits, n), L = std::min(po.low_qubits, K);
+  std::vector<Pass> fixed = plan_tiles_once(ops, n, po, false);
+  if (!(po.move_low && n > K && K >= 11 && K <= 13 && L == 5)) return fixed;
+  std::vector<Pass> moving = plan_tiles_once(ops, n, po, true);
+  return moving.size() < fixed.size() ? moving : fixed;
 }
 
 // ------------------------------------------------------ reference fusion
@@ -821,6 +994,11 @@ template <typename R>
 RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false, bool xchg = false,
                                 int max_blobs = 1 << 20, bool qphase = false) {
   const int K = (int)p.qubits.size();  // == tb + eb
+  // tile bits stored to the five lowest positions of the tile (the bits
+  // loaded from there unless the pass permutes its positions)
+  uint32_t store_low = 0;
+  for (int j = 0; j < K && j < 32; ++j)
+    if ((p.out.empty() ? p.qubits[j] : p.out[j]) < 5) store_low |= 1u << j;
   int loc[64];
   for (int q = 0; q < 64; ++q) loc[q] = -1;
   for (int j = 0; j < K; ++j) loc[p.qubits[j]] = j;
@@ -1542,11 +1720,11 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
           if (((req >> b) & 1) && dist[b] > c) dist[b] = c;
         ++c;
       }
-      // with exchanges, the bits loaded on lane slots 0..4 must be back on
-      // lane slots at the store: a use after the last item
+      // with exchanges, the bits stored to the five lowest positions must be
+      // on lane slots 0..4 at the store: a use after the last item
       if (xchg)
-        for (int i = 0; i < 5; ++i)
-          if (dist[out.map0[i]] > c) dist[out.map0[i]] = c;
+        for (int b = 0; b < K; ++b)
+          if (((store_low >> b) & 1) && dist[b] > c) dist[b] = c;
     }
     uint8_t nm[16];
     for (int i = 0; i < 16; ++i) nm[i] = map[i];
@@ -1641,9 +1819,9 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
     remap_to(nm, dist);
   }
   if (xchg) {
-    // thread slots 0..4 must hold the tile's five lowest bits (any order)
-    uint32_t low = 0;
-    for (int i = 0; i < 5; ++i) low |= 1u << out.map0[i];
+    // thread slots 0..4 must hold the bits stored to the five lowest
+    // positions (any order)
+    const uint32_t low = store_low;
     for (int i = 0; i < 5; ++i) {
       if ((low >> map[i]) & 1) continue;
       int where = -1;  // prefer a register slot, then lane slot 5, then a wave slot
@@ -1654,9 +1832,17 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       } else if (where == 5) {
         emit_xchg(0, 5);
         emit_xchg(0, i);
-      } else {  // on a wave slot: one LDS remap back to the load layout
+      } else {  // on a wave slot: one LDS remap that brings all of them to lane slots 0..4
         uint8_t nm[16];
-        std::memcpy(nm, out.map0, 16);
+        std::memcpy(nm, map, 16);
+        for (int k = 0; k < 5; ++k) {
+          if ((low >> nm[k]) & 1) continue;
+          for (int t = 5; t < 16; ++t)
+            if ((t < tb || (t >= 8 && t < 8 + eb)) && ((low >> nm[t]) & 1)) {
+              std::swap(nm[k], nm[t]);
+              break;
+            }
+        }
         emit_remap(nm);
         break;
       }
@@ -1825,6 +2011,12 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const Reg
       if ((t >> j) & 1) d |= 1ull << p.qubits[j];
     return d;
   };
+  auto dep_out = [&](uint32_t t) {  // store positions (the pass may permute them)
+    uint64_t d = 0;
+    for (int j = 0; j < K; ++j)
+      if ((t >> j) & 1) d |= 1ull << (p.out.empty() ? p.qubits[j] : p.out[j]);
+    return d;
+  };
   auto tmap = [&](const uint8_t* m, int tid, int r) {
     uint32_t t = 0;
     for (int i = 0; i < 8; ++i) t |= (uint32_t)((tid >> i) & 1) << m[i];
@@ -1976,7 +2168,7 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const Reg
       }
     }
     for (int tid = 0; tid < NTH; ++tid)
-      for (int r = 0; r < E; ++r) a[base | dep(tmap(rp.map1, tid, r))] = v[tid * E + r];
+      for (int r = 0; r < E; ++r) a[base | dep_out(tmap(rp.map1, tid, r))] = v[tid * E + r];
   }
 }
 
@@ -2073,7 +2265,7 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
       if (r.kind == R_QPHASE) r.s0 += (int32_t)pr.blocks.size();
     for (int slot = 0; slot < 8 + eb; ++slot) {  // thread slots 0..7, register slots 8..
       pa.ql[slot] = p.qubits[pr.map0[slot]];
-      pa.qf[slot] = p.qubits[pr.map1[slot]];
+      pa.qf[slot] = (p.out.empty() ? p.qubits : p.out)[pr.map1[slot]];
     }
     blob_at.push_back(blob.size() * 2);
     bool has_remap = false;
@@ -2083,6 +2275,10 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
       has_remap = has_remap || r.kind == R_REMAP;
     }
     pa.lds_remap = has_remap ? (int)(sizeof(R) * 2) << pa.K : 0;
+    // a store permutation that moves the wave slots' positions makes
+    // wavefronts overwrite amplitudes other wavefronts load; a remap's
+    // barrier already orders all loads before all stores
+    pa.sync_store = !has_remap && (pa.qf[6] != pa.ql[6] || pa.qf[7] != pa.ql[7]);
     for (const auto& M : pr.blocks) {
       const size_t at = blob.size();
       blob.resize(at + kMfmaBlobBytes / 2);
@@ -2214,6 +2410,7 @@ PlanOptions plan_options(const NqpuRunOptions& o, int prec) {
   po.mfma = (o.flags & NQPU_RUN_NO_MFMA) == 0;
   po.compile = (o.flags & NQPU_RUN_COMPILE) != 0;
   po.qphase = (o.flags & NQPU_RUN_NO_QPHASE) == 0;
+  po.move_low = po.registers && prec == NQPU_PRECISION_C64 && (o.flags & NQPU_RUN_FIXED_LOW) == 0;
   return po;
 }
 

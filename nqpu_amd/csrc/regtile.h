@@ -616,6 +616,10 @@ __device__ __forceinline__ void reg_tile_pass(T* __restrict__ st, const PassArgs
       }
     }
     body(v, sm, base, tid);
+    if (pa.sync_store) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
     if (pa.scaled) {
 #pragma unroll
       for (int r = 0; r < E; ++r) {

@@ -46,8 +46,11 @@ struct FusionResult {
 
 // One planned tile pass.
 struct Pass {
-  std::vector<int> qubits;  // ascending global positions, size K
-  std::vector<HostOp> ops;  // global numbering
+  std::vector<int> qubits;  // ascending physical positions, size K
+  std::vector<HostOp> ops;  // physical numbering
+  // the store position of the amplitude bit loaded from qubits[j] (a
+  // permutation of qubits); empty = the same positions
+  std::vector<int> out;
 };
 
 struct PlanOptions {
@@ -66,6 +69,9 @@ struct PlanOptions {
   // merge runs of diagonal gates in a register tile pass into one quadratic
   // phase op (R_QPHASE)
   bool qphase = true;
+  // complex64 register passes: the physical low positions may hold any
+  // qubits of the previous tile (plan_tiles)
+  bool move_low = false;
 };
 
 // lowering and planning (pure host code; unit-testable without a device)

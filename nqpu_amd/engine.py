@@ -117,22 +117,22 @@ class NativeCircuit:
         L.check(lib.nqpu_circuit_add_gates(self.ptr, ops, len(circuit.gates)), "nqpu_circuit_add_gates")
 
     def plan(self, precision="c64", fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
-             lds_tiles=False, mfma=True, compile=False, qphase=True):
+             lds_tiles=False, mfma=True, compile=False, qphase=True, fixed_low=False):
         lib = L.load()
         st = L.NqpuRunStats()
         o = run_options(fusion, semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma,
-                        compile=compile, qphase=qphase)
+                        compile=compile, qphase=qphase, fixed_low=fixed_low)
         L.check(lib.nqpu_circuit_plan(self.ptr, ct.byref(o), _prec(precision), ct.byref(st)), "plan")
         return _stats(st)
 
     def emulate(self, amps, precision="c64", semantics="standard", tile_qubits=0, low_qubits=0,
-                lds_tiles=False, batch_layout=False, mfma=True, qphase=True):
+                lds_tiles=False, batch_layout=False, mfma=True, qphase=True, fixed_low=False):
         """Host emulation of the tiled GPU execution (plan, per-pass programs
         and kernel semantics) in FP64; returns the new amplitudes.  With
         batch_layout, emulates the layout the batched kernel uses."""
         a = np.ascontiguousarray(amps, dtype=np.complex128).copy()
         o = run_options("tiled", semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma,
-                        qphase=qphase)
+                        qphase=qphase, fixed_low=fixed_low)
         if batch_layout:
             o.flags |= L.RUN_BATCH_LAYOUT
         L.check(L.load().nqpu_debug_emulate(self.ptr, ct.byref(o), _prec(precision), a.ctypes.data, len(a)),
@@ -170,12 +170,14 @@ _SEMANTICS = {"standard": L.SEMANTICS_STANDARD, "reference": L.SEMANTICS_REFEREN
 
 
 def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
-                time_passes=False, lds_tiles=False, mfma=True, compile=False, qphase=True) -> L.NqpuRunOptions:
+                time_passes=False, lds_tiles=False, mfma=True, compile=False, qphase=True,
+                fixed_low=False) -> L.NqpuRunOptions:
     """mfma=False keeps every gate of complex64 register tiles on the vector
     ALUs (no fused blocks on the matrix cores); compile=True runs complex64
     register tile passes as straight-line kernels compiled at run time;
     qphase=False applies runs of diagonal gates one by one instead of as one
-    quadratic phase per run."""
+    quadratic phase per run; fixed_low=True keeps the state's lowest qubits
+    in every complex64 register tile."""
     o = L.NqpuRunOptions()
     o.fusion = _FUSION[fusion]
     o.semantics = _SEMANTICS[semantics]
@@ -183,7 +185,7 @@ def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=
     o.low_qubits = low_qubits
     o.flags = ((L.RUN_TIME_PASSES if time_passes else 0) | (L.RUN_LDS_TILES if lds_tiles else 0)
                | (0 if mfma else L.RUN_NO_MFMA) | (L.RUN_COMPILE if compile else 0)
-               | (0 if qphase else L.RUN_NO_QPHASE))
+               | (0 if qphase else L.RUN_NO_QPHASE) | (L.RUN_FIXED_LOW if fixed_low else 0))
     return o
 
 
@@ -276,10 +278,11 @@ class StateVector:
     # --- execution
     def run(self, circuit: Circuit | NativeCircuit, fusion="tiled", semantics="standard",
             tile_qubits=0, low_qubits=0, time_passes=False, lds_tiles=False, mfma=True, compile=False,
-            qphase=True) -> dict:
+            qphase=True, fixed_low=False) -> dict:
         lib = L.load()
         nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
-        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles, mfma, compile, qphase)
+        o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles, mfma, compile, qphase,
+                        fixed_low)
         L.check(lib.nqpu_run(nc.ptr, self.ptr, ct.byref(o)), "nqpu_run")
         return self.last_stats()
 

@@ -145,6 +145,26 @@ def test_quadratic_phase_runs_match_oracle(oracle, n, tile, low, count):
             assert infidelity(out, exp) < 1e-6, (compiled, qphase)
 
 
+@pytest.mark.parametrize("n,tile,depth", [(20, 12, 16), (18, 11, 12), (19, 13, 14)])
+def test_moving_low_positions_match_oracle(oracle, n, tile, depth):
+    """Passes that hand the physical low positions to the next window's qubits
+    (permuted stores, closing permutation-only passes): interpreted and, for
+    12-qubit tiles, compiled, against the oracle and the fixed-low plan."""
+    from tests.test_planner_emulation import random_state
+    c = random_u3_brickwork(n, depth, 50 + n)
+    a0 = random_state(n, 4)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    for compiled in ((False, True) if tile == 12 else (False,)):
+        for fixed in (False, True):
+            with StateVector(n, "c64") as sv:
+                sv.set_amplitudes(a0)
+                sv.run(c, tile_qubits=tile, low_qubits=5, compile=compiled, fixed_low=fixed)
+                out = sv.amplitudes()
+            assert np.max(np.abs(out - exp)) < 2e-5, (compiled, fixed)
+            assert infidelity(out, exp) < 1e-6, (compiled, fixed)
+
+
 def test_compiled_passes_c1_bitwise_equal_interpreted(c1):
     """The compiled and interpreted kernels execute the same operations in the
     same order on the same registers: identical complex64 results."""

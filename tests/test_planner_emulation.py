@@ -118,6 +118,25 @@ def test_emulated_quadratic_phase_runs(oracle, n, tile, low, count, qphase):
     assert (merged > count // 4) if qphase else merged == 0
 
 
+@pytest.mark.parametrize("n,tile,depth", [(20, 12, 16), (18, 11, 12), (19, 13, 14)])
+def test_emulated_moving_low_positions(oracle, n, tile, depth):
+    """A brickwork needs far fewer passes when each pass hands the physical
+    low positions to qubits of the next window (the planner tracks the
+    qubit-to-position map, permutes inside each tile's store and ends with
+    passes that only permute); the emulated state equals the oracle's."""
+    c = random_u3_brickwork(n, depth, 50 + n)
+    nc = c.native()
+    moving = nc.plan("c64", tile_qubits=tile, low_qubits=5)["passes"]
+    fixed = nc.plan("c64", tile_qubits=tile, low_qubits=5, fixed_low=True)["passes"]
+    assert moving < fixed
+    a0 = random_state(n, 4)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    for fl in (False, True):
+        got = nc.emulate(a0, "c64", tile_qubits=tile, low_qubits=5, fixed_low=fl)
+        assert np.max(np.abs(got - exp)) < 1e-12, fl
+
+
 def test_qft_controlled_phases_merge():
     """A 30-qubit QFT's controlled phases merge into one quadratic phase per
     Hadamard, so far fewer matrix-core blocks remain."""
