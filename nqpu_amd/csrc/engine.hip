@@ -553,9 +553,11 @@ static std::vector<Pass> plan_tiles_once(const std::vector<HostOp>& ops, int n, 
 // free inside fixed-low tiles but leaves a permutation to undo otherwise.
 std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po) {
   const int K = std::min(po.tile_qubits, n), L = std::min(po.low_qubits, K);
-  std::vector<Pass> fixed = plan_tiles_once(ops, n, po, false);
-  if (!(po.move_low && n > K && K >= 11 && K <= 13 && L == 5)) return fixed;
+  if (!(po.move_low && n > K && K >= 11 && K <= 13 && L == 5)) return plan_tiles_once(ops, n, po, false);
+  std::vector<Pass> fixed;
+  std::thread th([&] { fixed = plan_tiles_once(ops, n, po, false); });
   std::vector<Pass> moving = plan_tiles_once(ops, n, po, true);
+  th.join();
   return moving.size() < fixed.size() ? moving : fixed;
 }
 
@@ -791,6 +793,8 @@ bool DeviceScratch::ensure(size_t b) {
 
 State::~State() {
   if (stream) (void)hipStreamSynchronize(stream);
+  if (stage) (void)hipHostFree(stage);
+  if (stage_ev) (void)hipEventDestroy(stage_ev);
   if (amps && owns_amps) (void)hipFree(amps);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -2250,17 +2254,31 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
   std::vector<uint16_t> blob;  // R_MFMA operand blobs of the whole run
   std::vector<PassArgs> args;
   std::vector<size_t> blob_at;  // byte offset of each pass's operand blobs
+  // the passes' programs are independent: built on parallel host threads.
+  // Matrix-core blocks, lane exchanges and quadratic phases (float tables)
+  // exist for complex64 only; complex128 programs use LDS remaps and
+  // vector-ALU gates.
+  constexpr bool c64 = std::is_same<R, float>::value;
+  std::vector<RegProgram<R>> progs(passes.size());
+  {
+    const size_t nt = std::min<size_t>(passes.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+      for (size_t i; (i = next.fetch_add(1)) < passes.size();)
+        progs[i] = build_reg_program<R>(passes[i], eb, 8, mfma && c64, c64, 1 << 20, qphase && c64);
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
+  }
   for (const Pass& p : passes) {
     PassArgs pa{};
     pa.K = (int)p.qubits.size();
     pa.n = s->n;
     pa.op0 = (int)prog.size();
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
-    // matrix-core blocks, lane exchanges and quadratic phases (float tables)
-    // exist for complex64 only; complex128 programs use LDS remaps and
-    // vector-ALU gates
-    constexpr bool c64 = std::is_same<R, float>::value;
-    RegProgram<R> pr = build_reg_program<R>(p, eb, 8, mfma && c64, c64, 1 << 20, qphase && c64);
+    RegProgram<R>& pr = progs[&p - passes.data()];
     for (RegOp<R>& r : pr.ops)  // R_QPHASE tables follow the blocks' operand blobs
       if (r.kind == R_QPHASE) r.s0 += (int32_t)pr.blocks.size();
     for (int slot = 0; slot < 8 + eb; ++slot) {  // thread slots 0..7, register slots 8..
@@ -2299,14 +2317,6 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
   }
   const size_t pb = (std::max<size_t>(prog.size(), 1) * sizeof(RegOp<R>) + 255) & ~size_t(255);
   const size_t ob = pb + blob.size() * 2;
-  HIPTRY(hipStreamSynchronize(s->stream), "sync before program upload");
-  if (!s->opbuf.ensure(ob)) return NQPU_OUT_OF_DEVICE_MEMORY;
-  if (!prog.empty())
-    HIPTRY(copy_on(s->stream, s->opbuf.p, prog.data(), prog.size() * sizeof(RegOp<R>), hipMemcpyHostToDevice),
-           "program upload");
-  if (!blob.empty())
-    HIPTRY(copy_on(s->stream, (char*)s->opbuf.p + pb, blob.data(), blob.size() * 2, hipMemcpyHostToDevice),
-           "block upload");
   std::vector<hipFunction_t> fns;
   if (compile && std::is_same<R, float>::value && eb == 4) {
     std::vector<RegOp<float>> progf(prog.size());
@@ -2314,6 +2324,31 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     const NqpuError e = jit_pass_functions(progf, args, eb, &fns);
     if (e != NQPU_OK) return e;
   }
+  // Upload: program and operand blobs go through the pinned stage with one
+  // asynchronous copy in stream order (the previous run's kernels have read
+  // the device buffer before it lands); the stage is reused once the
+  // previous copy out of it is done.  Growing either buffer waits for the
+  // device first.
+  if (s->stage_pending) {
+    HIPTRY(hipEventSynchronize(s->stage_ev), "wait for the previous program upload");
+    s->stage_pending = false;
+  }
+  if (ob > s->opbuf.bytes || ob > s->stage_bytes) HIPTRY(hipStreamSynchronize(s->stream), "sync before buffer growth");
+  if (!s->opbuf.ensure(ob)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  if (ob > s->stage_bytes) {
+    if (s->stage) (void)hipHostFree(s->stage);
+    s->stage = nullptr;
+    s->stage_bytes = 0;
+    const size_t nb = std::max(ob, (size_t)1 << 20);
+    HIPTRY(hipHostMalloc(&s->stage, nb), "pinned program stage");
+    s->stage_bytes = nb;
+  }
+  if (!s->stage_ev) HIPTRY(hipEventCreateWithFlags(&s->stage_ev, hipEventDisableTiming), "stage event");
+  if (!prog.empty()) std::memcpy(s->stage, prog.data(), prog.size() * sizeof(RegOp<R>));
+  if (!blob.empty()) std::memcpy((char*)s->stage + pb, blob.data(), blob.size() * 2);
+  HIPTRY(hipMemcpyAsync(s->opbuf.p, s->stage, ob, hipMemcpyHostToDevice, s->stream), "program upload");
+  HIPTRY(hipEventRecord(s->stage_ev, s->stream), "stage event record");
+  s->stage_pending = true;
   for (size_t i = 0; i < args.size(); ++i) {
     const PassArgs& pa = args[i];
     s->timer.mark(s->stream);

@@ -126,6 +126,13 @@ struct State {
   int device = 0;
   NqpuRunStats stats{};
   DeviceScratch scratch, scratch2, opbuf;
+  // pinned staging of register tile programs: their upload is an
+  // asynchronous copy in stream order, so the host plans the next run while
+  // the device works; stage_ev marks the copy out of it done
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+  hipEvent_t stage_ev = nullptr;
+  bool stage_pending = false;
   PassTimer timer;
   ~State();
   size_t amp_size() const { return prec == NQPU_PRECISION_C64 ? 8 : 16; }
