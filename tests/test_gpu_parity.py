@@ -165,6 +165,26 @@ def test_moving_low_positions_match_oracle(oracle, n, tile, depth):
             assert infidelity(out, exp) < 1e-6, (compiled, fixed)
 
 
+def test_permuted_store_across_wavefronts_is_ordered():
+    """A 22-qubit brickwork whose closing permutation pass moves the positions
+    held by the wave slots without any LDS remap: wavefronts store amplitudes
+    other wavefronts load, so the pass must order all loads first (without
+    that, a few hundred amplitudes differed from run to run).  Repeated runs,
+    interpreted and compiled, equal the fixed-low plan's result."""
+    c = random_u3_brickwork(22, 12, 7)
+    nc = c.native()
+    assert nc.plan("c64", tile_qubits=12)["passes"] < nc.plan("c64", tile_qubits=12, fixed_low=True)["passes"]
+    with StateVector(22, "c64") as sv:
+        sv.run(nc, tile_qubits=12, fixed_low=True)
+        ref = sv.amplitudes()
+    for compiled in (False, True):
+        for _ in range(3):
+            with StateVector(22, "c64") as sv:
+                sv.run(nc, tile_qubits=12, compile=compiled)
+                out = sv.amplitudes()
+            assert np.max(np.abs(out - ref)) < 1e-6, compiled
+
+
 def test_compiled_passes_c1_bitwise_equal_interpreted(c1):
     """The compiled and interpreted kernels execute the same operations in the
     same order on the same registers: identical complex64 results."""
