@@ -545,6 +545,8 @@ static std::vector<Pass> plan_tiles_once(const std::vector<HostOp>& ops, int n, 
     emit_pass(S, {}, np);
     commit(np);
   }
+  for (int q = 0; move && q < n; ++q)
+    if (pi[q] != q) return {};  // not back at the identity: the caller keeps the fixed-low plan
   return passes;
 }
 
@@ -558,7 +560,7 @@ std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOp
   std::thread th([&] { fixed = plan_tiles_once(ops, n, po, false); });
   std::vector<Pass> moving = plan_tiles_once(ops, n, po, true);
   th.join();
-  return moving.size() < fixed.size() ? moving : fixed;
+  return !moving.empty() && moving.size() < fixed.size() ? moving : fixed;
 }
 
 // ------------------------------------------------------ reference fusion

@@ -137,6 +137,22 @@ def test_emulated_moving_low_positions(oracle, n, tile, depth):
         assert np.max(np.abs(got - exp)) < 1e-12, fl
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_emulated_moving_low_random_mixes(oracle, seed):
+    """Random gate mixes (every kind, controls and diagonals outside the tile)
+    on 15-17 qubits with tiles of 11-13: whichever plan wins (moving or fixed
+    low positions), the emulated state equals the oracle's."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(15, 18))
+    tile = int(rng.integers(11, 14))
+    c = random_circuit(n, int(rng.integers(80, 200)), 100 + seed)
+    a0 = random_state(n, seed)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    got = c.native().emulate(a0, "c64", tile_qubits=tile, low_qubits=5)
+    assert np.max(np.abs(got - exp)) < 1e-12
+
+
 def test_qft_controlled_phases_merge():
     """A 30-qubit QFT's controlled phases merge into one quadratic phase per
     Hadamard, so far fewer matrix-core blocks remain."""
