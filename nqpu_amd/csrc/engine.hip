@@ -2231,7 +2231,7 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
     }
     return NQPU_OK;
   }
-  PlanOptions po = plan_options(o, prec);
+  PlanOptions po = plan_options(o, prec, c.n);
   std::vector<Pass> passes = plan_tiles(ops, c.n, po);
   const int K = std::min(po.tile_qubits, c.n);
   const bool reg = po.registers && K >= 11 && K <= 13;
@@ -2413,7 +2413,7 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
 }
 
 NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o) {
-  const PlanOptions po = plan_options(o, NQPU_PRECISION_C64);
+  const PlanOptions po = plan_options(o, NQPU_PRECISION_C64, c.n);
   const int K = std::min(po.tile_qubits, c.n);
   if (!po.registers || K != 12) {
     set_error("compiled passes need complex64 register tiles of 12 qubits");
@@ -2436,10 +2436,18 @@ NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o) {
   return jit_check_compile(prog, args, K - 8);
 }
 
-PlanOptions plan_options(const NqpuRunOptions& o, int prec) {
+PlanOptions plan_options(const NqpuRunOptions& o, int prec, int n) {
   PlanOptions po;
   const int kmax = (prec == NQPU_PRECISION_C64) ? 13 : 12;
-  po.tile_qubits = o.tile_qubits ? std::min<int>((int)o.tile_qubits, kmax) : (prec == NQPU_PRECISION_C64 ? 12 : 11);
+  // Default tiles: 12 qubits (complex64) and 11 (complex128) for large
+  // states.  A state of at most 22 (complex64) / 20 (complex128) qubits has
+  // too few 12- / 11-qubit tiles to fill 256 CUs, and its passes are latency
+  // bound: one qubit less doubles the workgroups (random 20-qubit circuit of
+  // depth 40 on MI355X: 2.1 instead of 3.1 ms complex64, 1.7 instead of
+  // 2.0 ms complex128).  Compiled complex64 passes keep 12-qubit tiles.
+  const bool c64 = prec == NQPU_PRECISION_C64, small = n > 0 && n <= (c64 ? 22 : 20);
+  const int dflt = c64 ? ((small && !(o.flags & NQPU_RUN_COMPILE)) ? 11 : 12) : (small ? 10 : 11);
+  po.tile_qubits = o.tile_qubits ? std::min<int>((int)o.tile_qubits, kmax) : dflt;
   if (po.tile_qubits < 5) po.tile_qubits = 5;
   po.low_qubits = o.low_qubits ? (int)o.low_qubits : (prec == NQPU_PRECISION_C64 ? 5 : 4);
   po.low_qubits = std::min(po.low_qubits, po.tile_qubits);
@@ -2478,7 +2486,7 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
     }
   } else {
     std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
-    PlanOptions po = plan_options(o, prec);
+    PlanOptions po = plan_options(o, prec, c.n);
     std::vector<Pass> passes = plan_tiles(ops, c.n, po);
     st.passes = passes.size();
     st.device_ops = ops.size();
@@ -2533,7 +2541,7 @@ NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o) {
         }
       }
     } else {
-      PlanOptions po = plan_options(o, s->prec);
+      PlanOptions po = plan_options(o, s->prec, s->n);
       std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
       e = run_tiled<R>(s, ops, po);
     }

@@ -80,7 +80,7 @@ std::vector<HostOp> lower_circuit(const Circuit& c, bool reference_semantics);
 std::vector<HostOp> merge_single_qubit(const std::vector<HostOp>& ops);
 std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po);
 FusionResult fuse_gates_reference(const std::vector<Gate>& gates);
-PlanOptions plan_options(const NqpuRunOptions& o, int prec);
+PlanOptions plan_options(const NqpuRunOptions& o, int prec, int n = 0);  // n: state qubits (0: unknown)
 // What run_circuit would launch, computed on the host.
 NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec);
 // Host emulation of the tiled execution (plan, per-pass programs, kernel

@@ -115,7 +115,7 @@ def test_compiled_passes_match_oracle(oracle, name):
     for compiled in (True, False):
         with StateVector(n, "c64") as sv:
             sv.set_amplitudes(a0)
-            st = sv.run(c, compile=compiled)
+            st = sv.run(c, compile=compiled, tile_qubits=12)
             out = sv.amplitudes()
         assert st["passes"] > 0
         assert np.max(np.abs(out - exp)) < 2e-5, compiled
@@ -192,7 +192,7 @@ def test_compiled_passes_c1_bitwise_equal_interpreted(c1):
     outs = []
     for compiled in (False, True, True):  # the second compiled run reuses the cached kernels
         with StateVector(20, "c64") as sv:
-            sv.run(c, compile=compiled)
+            sv.run(c, compile=compiled, tile_qubits=12)  # the same plan for both kernels
             outs.append(sv.amplitudes())
     assert np.array_equal(outs[0], outs[1])
     assert np.array_equal(outs[1], outs[2])
