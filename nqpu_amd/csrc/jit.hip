@@ -56,9 +56,13 @@ void emit_op(std::ostringstream& o, const RegOp<float>& r, int k, int eb) {
       else
         o << "    nqpu::reg_xchg_reg<float2, " << eb << ", " << r.s0 << ", " << (r.s1 - 16) << ">(v);\n";
       return;
-    case R_QPHASE:
-      o << "    nqpu::reg_qphase<float, float2, " << eb << ">(v, ops[" << k << "], blob, tid, base);\n";
+    case R_QPHASE: {  // chunk masks are structure: compile-time constants
+      o << "    { const uint64_t qm[4] = {";
+      for (int c = 0; c < 4; ++c) o << (c ? ", " : "") << (c < r.pad ? qphase_chunk(r, c) : 0) << "ull";
+      o << "}; nqpu::reg_qphase_n<float, float2, " << eb << ", " << r.pad << ">(v, ops[" << k
+        << "], blob, tid, base, qm); }\n";
       return;
+    }
     default:
       break;
   }

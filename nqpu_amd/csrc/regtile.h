@@ -344,9 +344,35 @@ __device__ __forceinline__ float sub_half(unsigned h, float x) {
 // reduced again before they are combined, so Phi(r) stays within a few
 // revolutions and keeps an absolute error of a few 1e-7.  v_sin_f32 and
 // v_cos_f32 take their argument in revolutions.
-template <typename R, typename T, int EB>
-__device__ __forceinline__ void reg_qphase(T (&v)[1 << EB], const RegOp<R>& op, const unsigned char* blob, int tid,
-                                           uint64_t base) {
+// The summed chunk-table rows of one R_QPHASE op for the tile at `base`:
+// all row indices first, then all row loads, so their latencies overlap.
+// With masks known at compile time (generated pass kernels) the index
+// gathers reduce to scalar bit extractions.
+template <int NCH>
+__device__ __forceinline__ void qphase_rows(const float* ch, const uint64_t (&mask)[4], uint64_t base,
+                                            float (&ks)[16]) {
+  uint32_t p[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) {
+    int i = 0;
+#pragma unroll
+    for (int b = 0; b < 64; ++b)
+      if ((mask[k] >> b) & 1) p[k] |= (uint32_t)((base >> b) & 1) << i++;
+  }
+#pragma unroll
+  for (int x = 0; x < 16; ++x) ks[x] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) {
+    const float* e = ch + (size_t)k * kQPhaseChunkFloats + 16 * p[k];
+#pragma unroll
+    for (int x = 0; x < 16; ++x) ks[x] += e[x];
+  }
+}
+
+// R_QPHASE with NCH chunk tables whose qubit masks are `mask`.
+template <typename R, typename T, int EB, int NCH>
+__device__ __forceinline__ void reg_qphase_n(T (&v)[1 << EB], const RegOp<R>& op, const unsigned char* blob, int tid,
+                                             uint64_t base, const uint64_t (&mask)[4]) {
   const float* tab = reinterpret_cast<const float*>(blob + (size_t)op.s0 * kMfmaBlobBytes);
   float c = tab[tid];
   float l[EB];
@@ -354,20 +380,9 @@ __device__ __forceinline__ void reg_qphase(T (&v)[1 << EB], const RegOp<R>& op, 
   for (int j = 0; j < EB; ++j) l[j] = tab[256 * (1 + j) + tid];
   // qubits outside the tile: one table row per chunk, chosen by the tile's
   // base index (uniform over the workgroup)
-  const float* ch = tab + 256 * (1 + EB);
-  if (op.pad > 0) {
-    float ks[16];  // sum of the chunk rows (uniform)
-#pragma unroll
-    for (int i = 0; i < 16; ++i) ks[i] = 0.0f;
-    for (int k = 0; k < op.pad; ++k) {
-      uint32_t p = 0;
-      int i = 0;
-      for (uint64_t m = qphase_chunk(op, k); m; m &= m - 1, ++i)
-        p |= (uint32_t)((base >> __builtin_ctzll(m)) & 1) << i;
-      const float* e = ch + (size_t)k * kQPhaseChunkFloats + 16 * p;
-#pragma unroll
-      for (int x = 0; x < 16; ++x) ks[x] += e[x];
-    }
+  if constexpr (NCH > 0) {
+    float ks[16];
+    qphase_rows<NCH>(tab + 256 * (1 + EB), mask, base, ks);
     float kt = ks[0];
 #pragma unroll
     for (int t = 0; t < 8; ++t) kt += ((tid >> t) & 1) ? ks[1 + t] : 0.0f;
@@ -397,6 +412,21 @@ __device__ __forceinline__ void reg_qphase(T (&v)[1 << EB], const RegOp<R>& op, 
     z.x = __builtin_amdgcn_cosf(x);
     z.y = __builtin_amdgcn_sinf(x);
     v[r] = cmul_(v[r], z);
+  }
+}
+
+// R_QPHASE with the chunk masks read from the op (the interpreting kernel).
+template <typename R, typename T, int EB>
+__device__ __forceinline__ void reg_qphase(T (&v)[1 << EB], const RegOp<R>& op, const unsigned char* blob, int tid,
+                                           uint64_t base) {
+  uint64_t mask[4] = {0, 0, 0, 0};
+  for (int k = 0; k < op.pad && k < 4; ++k) mask[k] = qphase_chunk(op, k);
+  switch (op.pad) {
+    case 0: reg_qphase_n<R, T, EB, 0>(v, op, blob, tid, base, mask); break;
+    case 1: reg_qphase_n<R, T, EB, 1>(v, op, blob, tid, base, mask); break;
+    case 2: reg_qphase_n<R, T, EB, 2>(v, op, blob, tid, base, mask); break;
+    case 3: reg_qphase_n<R, T, EB, 3>(v, op, blob, tid, base, mask); break;
+    default: reg_qphase_n<R, T, EB, 4>(v, op, blob, tid, base, mask); break;
   }
 }
 
