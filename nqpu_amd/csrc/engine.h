@@ -75,6 +75,11 @@ struct BatchPass {
   int32_t op0, nops;     // register program
   int32_t term0, nterm;  // observable terms evaluated on this tile set
   int32_t q[16];         // tile qubits, ascending
+  // gate passes over HBM: first operand unit (kMfmaBlobBytes) of the
+  // program's matrix-core blocks and phase tables in the batch's operand
+  // buffer, and nonzero when the amplitudes go through the blocks scaled
+  // by 2^15 (as in the standalone passes)
+  int32_t blob_unit, scaled;
 };
 
 struct BatchTerm {
@@ -86,8 +91,8 @@ struct BatchTerm {
 };
 
 void launch_batch(void* states, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
-                  const RegOp<float>* prog, const BatchTerm* terms, unsigned* counter, double* out,
-                  int grid, hipStream_t s);
+                  const RegOp<float>* prog, const BatchTerm* terms, const unsigned char* blob, unsigned* counter,
+                  double* out, int grid, hipStream_t s);
 
 
 // ----------------------------------------------------------- kernel launches

@@ -1389,28 +1389,41 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       uint64_t cm = 0;
       for (int q : qs) cm |= bit(q);
       set_qphase_chunk(r, c, cm);
-      for (int pat = 0; pat < (1 << cnt); ++pat) {
-        double k0 = 0, kt[8] = {}, kr[8] = {};
-        for (int i = 0; i < cnt; ++i) {
-          if (!((pat >> i) & 1)) continue;
-          auto li = lg.find(qs[i]);
-          if (li != lg.end()) k0 += li->second;
-          for (int i2 = i + 1; i2 < cnt; ++i2) {
-            if (!((pat >> i2) & 1)) continue;
-            auto gi = agg.find({std::min(qs[i], qs[i2]), std::max(qs[i], qs[i2])});
-            if (gi != agg.end()) k0 += gi->second;
-          }
-          auto ti = agt.find(qs[i]);
-          if (ti != agt.end())
-            for (int t = 0; t < 8; ++t) kt[t] += ti->second[t];
-          auto ri = agr.find(qs[i]);
-          if (ri != agr.end())
-            for (int j = 0; j < eb; ++j) kr[j] += ri->second[j];
+      // per chunk qubit i: its linear term, its couplings to the chunk's other
+      // qubits and to the thread and register slots
+      double li[8] = {}, aii[8][8] = {}, kti[8][8] = {}, kri[8][8] = {};
+      for (int i = 0; i < cnt; ++i) {
+        auto lf = lg.find(qs[i]);
+        if (lf != lg.end()) li[i] = lf->second;
+        for (int i2 = 0; i2 < i; ++i2) {
+          auto gi = agg.find({qs[i2], qs[i]});
+          if (gi != agg.end()) aii[i][i2] = gi->second;
         }
+        auto ti = agt.find(qs[i]);
+        if (ti != agt.end())
+          for (int t = 0; t < 8; ++t) kti[i][t] = ti->second[t];
+        auto ri = agr.find(qs[i]);
+        if (ri != agr.end())
+          for (int j = 0; j < eb; ++j) kri[i][j] = ri->second[j];
+      }
+      // a pattern's terms are those of the pattern without its highest set
+      // bit i, plus i's linear term, couplings to the rest and slot terms
+      double v[256][17];
+      for (int x = 0; x < 17; ++x) v[0][x] = 0;
+      for (int pat = 1; pat < (1 << cnt); ++pat) {
+        const int i = 31 - __builtin_clz((unsigned)pat), rest = pat & ~(1 << i);
+        double k0 = v[rest][0] + li[i];
+        for (int i2 = 0; i2 < i; ++i2)
+          if ((rest >> i2) & 1) k0 += aii[i][i2];
+        v[pat][0] = k0;
+        for (int t = 0; t < 8; ++t) v[pat][1 + t] = v[rest][1 + t] + kti[i][t];
+        for (int j = 0; j < eb; ++j) v[pat][9 + j] = v[rest][9 + j] + kri[i][j];
+      }
+      for (int pat = 0; pat < (1 << cnt); ++pat) {
         R* e = C + (size_t)c * kQPhaseChunkFloats + 16 * pat;
-        e[0] = fr(k0);
-        for (int t = 0; t < 8; ++t) e[1 + t] = fr(kt[t]);
-        for (int j = 0; j < eb; ++j) e[9 + j] = fr(kr[j]);
+        e[0] = fr(v[pat][0]);
+        for (int t = 0; t < 8; ++t) e[1 + t] = fr(v[pat][1 + t]);
+        for (int j = 0; j < eb; ++j) e[9 + j] = fr(v[pat][9 + j]);
       }
     }
     out.qphase_gates += g.members.size();
@@ -1505,11 +1518,18 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
     // gave 27 % more blocks on a 30-qubit U3 + CNOT brickwork.
     std::vector<int> rem(ops.size());
     for (int k = 0; k < (int)ops.size(); ++k) rem[k] = k;
+    // tail0[i]: some op at or after rem[i] has no tile bits and could still
+    // join a block whose bits are all blocked
+    std::vector<char> tail0;
     auto cone = [&](uint32_t S, std::vector<int>* taken, std::vector<int>* left) {
       uint32_t blocked = 0;
       uint64_t G = 0;
       double w = 0;
-      for (int k : rem) {
+      for (size_t i = 0; i < rem.size(); ++i) {
+        // a weight-only scan ends once every bit of S is blocked: nothing
+        // after that point can join
+        if (!taken && (blocked & S) == S && !tail0[i]) break;
+        const int k = rem[i];
         const L& o = ops[k];
         const uint32_t qm = qmask(o);
         bool ok = (qm & blocked) == 0 && (qm & ~S) == 0 && fusable(o);
@@ -1537,6 +1557,9 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         rem.erase(rem.begin());
         continue;
       }
+      tail0.assign(rem.size() + 1, 0);
+      for (size_t i = rem.size(); i-- > 0;)
+        tail0[i] = tail0[i + 1] || (qmask(ops[rem[i]]) == 0 && fusable(ops[rem[i]]));
       uint32_t touched = 0;
       int seen = 0;
       for (int k : rem) {
@@ -1887,11 +1910,19 @@ static uint16_t to_half(double x) {
   return (uint16_t)(sign | h);
 }
 
+// Exact value of a binary16 (bit arithmetic: the operand blobs of a batch of
+// circuits convert millions of values).
 static double from_half(uint16_t h) {
-  const int sign = (h >> 15) & 1, exp = (h >> 10) & 0x1f, mant = h & 0x3ff;
-  double v = exp == 0 ? std::ldexp((double)mant, -24) : std::ldexp((double)(mant | 0x400), exp - 25);
-  if (exp == 31) v = mant ? NAN : INFINITY;
-  return sign ? -v : v;
+  const uint32_t sign = (uint32_t)(h >> 15) << 31, exp = (h >> 10) & 0x1f, mant = h & 0x3ff;
+  float f;
+  if (exp == 0) {  // zero or subnormal: mant * 2^-24, exact in float
+    f = (float)mant * 5.9604644775390625e-8f;
+    if (sign) f = -f;
+  } else {
+    const uint32_t u = sign | (exp == 31 ? 0x7f800000u : (exp + 112) << 23) | (mant << 13);
+    std::memcpy(&f, &u, 4);
+  }
+  return f;
 }
 
 // Operand blob of one R_MFMA op from its 16x16 block matrix (block-index order).
@@ -2227,7 +2258,7 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
       PlanOptions bpo;
       bpo.tile_qubits = KH;
       bpo.low_qubits = 5;
-      for (const Pass& p : plan_tiles(ops, c.n, bpo)) emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, EB, 8), EB);
+      for (const Pass& p : plan_tiles(ops, c.n, bpo)) emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, EB, 8, true, false, 1 << 20, true), EB);
     }
     return NQPU_OK;
   }
@@ -2910,6 +2941,7 @@ struct CircuitPlan {
   std::vector<BatchPass> bp;
   std::vector<RegOp<float>> prog;
   std::vector<BatchTerm> bt;
+  std::vector<uint16_t> blob;  // operand units of the HBM passes' blocks and phase tables
   double cost = 0;
   uint64_t amps = 0;  // HBM amplitudes (0 when resident)
 };
@@ -2968,10 +3000,27 @@ CircuitPlan plan_batch_circuit(const Circuit& c, const std::vector<PauliTerm>& t
     P.K = KH;
     P.tb = 8;
     for (int j = 0; j < KH; ++j) P.q[j] = p.qubits[j];
-    std::vector<RegOp<float>> pr = build_reg_program<float>(p, EB, 8).ops;
+    // matrix-core blocks and quadratic phases as in the standalone passes;
+    // LDS remaps only (the kernel loads and stores tiles in the canonical
+    // slot order)
+    RegProgram<float> pr = build_reg_program<float>(p, EB, 8, true, false, 1 << 20, true);
+    for (RegOp<float>& r : pr.ops)  // phase tables follow the blocks' operand blobs
+      if (r.kind == R_QPHASE) r.s0 += (int32_t)pr.blocks.size();
+    P.blob_unit = (int32_t)(cp.blob.size() / (kMfmaBlobBytes / 2));
+    P.scaled = pr.blocks.empty() ? 0 : 1;
+    for (const auto& M : pr.blocks) {
+      const size_t at = cp.blob.size();
+      cp.blob.resize(at + kMfmaBlobBytes / 2);
+      mfma_blob(M, cp.blob.data() + at);
+    }
+    if (!pr.qtab.empty()) {
+      const size_t at = cp.blob.size();
+      cp.blob.resize(at + pr.qtab.size() * 2);
+      std::memcpy(cp.blob.data() + at, pr.qtab.data(), pr.qtab.size() * sizeof(float));
+    }
     P.op0 = (int)cp.prog.size();
-    P.nops = (int)pr.size();
-    cp.prog.insert(cp.prog.end(), pr.begin(), pr.end());
+    P.nops = (int)pr.ops.size();
+    cp.prog.insert(cp.prog.end(), pr.ops.begin(), pr.ops.end());
     cp.bp.push_back(P);
   }
   // observable passes: each term on a tile set that holds its X bits
@@ -3044,7 +3093,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   }
   const size_t nc = cps.size();
   // offsets of each circuit's passes, program ops and terms in the tables
-  std::vector<size_t> bp_at(nc + 1, 0), pg_at(nc + 1, 0), bt_at(nc + 1, 0);
+  std::vector<size_t> bp_at(nc + 1, 0), pg_at(nc + 1, 0), bt_at(nc + 1, 0), bl_at(nc + 1, 0);
   std::vector<BatchCircuit> bc(nc);
   uint64_t state_amps = 0;
   NqpuBatchStats st{};
@@ -3057,6 +3106,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
     bp_at[k + 1] = bp_at[k] + cp.bp.size();
     pg_at[k + 1] = pg_at[k] + cp.prog.size();
     bt_at[k + 1] = bt_at[k] + cp.bt.size();
+    bl_at[k + 1] = bl_at[k] + cp.blob.size() / (kMfmaBlobBytes / 2);  // operand units
     BatchCircuit C = cp.C;
     C.pass0 += (int32_t)bp_at[k];
     C.red0 += (int32_t)bp_at[k];
@@ -3080,7 +3130,8 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   const size_t og = up(op_ + bp_at[nc] * sizeof(BatchPass));
   const size_t ot = up(og + pg_at[nc] * sizeof(RegOp<float>));
   const size_t ocnt = up(ot + bt_at[nc] * sizeof(BatchTerm));
-  const size_t total = ocnt + 256;
+  const size_t obl = ocnt + 256;  // operand units, 256-byte aligned
+  const size_t total = obl + bl_at[nc] * kMfmaBlobBytes;
   char* host = buffer(total);
   if (!host) {
     set_error("no host memory for the batch tables");
@@ -3099,8 +3150,11 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
           BatchPass P = cp.bp[i];
           P.op0 += (int32_t)pg_at[k];
           P.term0 += (int32_t)bt_at[k];
+          P.blob_unit += (int32_t)bl_at[k];
           std::memcpy(dp + i, &P, sizeof(BatchPass));
         }
+        if (!cp.blob.empty())
+          std::memcpy(host + obl + bl_at[k] * kMfmaBlobBytes, cp.blob.data(), cp.blob.size() * 2);
         if (!cp.prog.empty())
           std::memcpy(reinterpret_cast<RegOp<float>*>(host + og) + pg_at[k], cp.prog.data(),
                       cp.prog.size() * sizeof(RegOp<float>));
@@ -3121,6 +3175,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   plan.og = og;
   plan.ot = ot;
   plan.ocnt = ocnt;
+  plan.obl = obl;
   plan.total = total;
   plan.state_amps = state_amps;
   plan.ncirc = nc;
@@ -3174,8 +3229,9 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   (void)hipEventRecord(e0, b->stream);
   if (grid > 0)
     launch_batch(b->states.p, (const BatchCircuit*)(base + oc), (int)plan.ncirc, (const BatchPass*)(base + op_),
-                 (const RegOp<float>*)(base + og), (const BatchTerm*)(base + ot), (unsigned*)(base + ocnt),
-                 (double*)b->out.p, grid, b->stream);
+                 (const RegOp<float>*)(base + og), (const BatchTerm*)(base + ot),
+                 (const unsigned char*)(base + plan.obl), (unsigned*)(base + ocnt), (double*)b->out.p, grid,
+                 b->stream);
   NqpuError le = hip_check(hipGetLastError(), "batch launch");
   (void)hipEventRecord(e1, b->stream);
   NqpuError se = hip_check(hipStreamSynchronize(b->stream), "batch");

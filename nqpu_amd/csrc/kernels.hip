@@ -481,7 +481,8 @@ template <int EB>
 __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, const BatchCircuit* __restrict__ circs,
                                                   int ncirc, const BatchPass* __restrict__ passes,
                                                   const RegOp<float>* __restrict__ prog,
-                                                  const BatchTerm* __restrict__ terms, unsigned* counter,
+                                                  const BatchTerm* __restrict__ terms,
+                                                  const unsigned char* __restrict__ blob, unsigned* counter,
                                                   double* __restrict__ out) {
   using T = float2;
   constexpr int E = 1 << EB;
@@ -543,14 +544,23 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
             } else {
               v[r] = st[d];
             }
+            if (P.scaled) {
+              v[r].x *= kMfmaScale;
+              v[r].y *= kMfmaScale;
+            }
           }
-          reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true, base, nullptr);
+          reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true, base,
+                                    blob + (size_t)P.blob_unit * kMfmaBlobBytes);
 #pragma unroll
           for (int r = 0; r < E; ++r) {
             uint64_t d = base | dlo;
 #pragma unroll
             for (int j = 0; j < EB; ++j)
               if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
+            if (P.scaled) {
+              v[r].x *= 1.0f / kMfmaScale;
+              v[r].y *= 1.0f / kMfmaScale;
+            }
             st[d] = v[r];
           }
         }
@@ -589,11 +599,11 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
 }
 
 void launch_batch(void* states, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
-                  const RegOp<float>* prog, const BatchTerm* terms, unsigned* counter, double* out, int grid,
-                  hipStream_t s) {
+                  const RegOp<float>* prog, const BatchTerm* terms, const unsigned char* blob, unsigned* counter,
+                  double* out, int grid, hipStream_t s) {
   const size_t lds = sizeof(float2) << (8 + kBatchEB);
   hipLaunchKernelGGL(k_batch<kBatchEB>, dim3(grid), dim3(256), lds, s, (float2*)states, circs, ncirc, passes, prog,
-                     terms, counter, out);
+                     terms, blob, counter, out);
 }
 
 // out[c] = sum |a_i|^2 over chunk c of 2^cb amplitudes.

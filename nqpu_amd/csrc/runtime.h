@@ -182,7 +182,7 @@ struct Batch {
 // Host half of a batch: the tables k_batch reads, in one upload image.
 struct BatchPlan {
   char* host = nullptr;  // from the buffer callback of plan_batch (not owned)
-  size_t oc = 0, op = 0, og = 0, ot = 0, ocnt = 0, total = 0;
+  size_t oc = 0, op = 0, og = 0, ot = 0, ocnt = 0, obl = 0, total = 0;  // obl: operand blobs
   uint64_t state_amps = 0;
   size_t ncirc = 0, program_ops = 0;
   NqpuBatchStats st{};

@@ -6,7 +6,7 @@ import pytest
 from nqpu_amd import engine as E
 from nqpu_amd.engine import Batch, Circuit
 from nqpu_amd.workloads import variational_batch
-from tests.test_planner_emulation import random_circuit
+from tests.test_planner_emulation import diagonal_heavy_circuit, random_circuit
 
 pytestmark = pytest.mark.gpu
 
@@ -59,6 +59,21 @@ def test_general_pauli_terms(batch, oracle):
         terms = [(float(rng.normal()), "".join("IXYZ"[k] for k in rng.integers(0, 4, n))) for _ in range(6)]
         if n == 15:  # X on both ends: no standard tile set holds it
             terms.append((0.5, "X" + "I" * 13 + "Y"))
+        circs.append(c)
+        obs.append(terms)
+    e = batch.expectation(circs, obs)
+    ref = np.array([oracle_energy(oracle, c, t) for c, t in zip(circs, obs)])
+    assert np.max(np.abs(e - ref)) < 2e-5
+
+
+def test_quadratic_phase_passes_match_oracle(batch, oracle):
+    """Circuits above 12 qubits whose HBM passes hold matrix-core blocks and
+    quadratic phase ops, beside resident ones, in one launch."""
+    rng = np.random.default_rng(11)
+    circs, obs = [], []
+    for k, n in enumerate((13, 16, 10, 14)):
+        c = diagonal_heavy_circuit(n, 300, 60 + k)
+        terms = [(float(rng.normal()), "".join("IXYZ"[j] for j in rng.integers(0, 4, n))) for _ in range(5)]
         circs.append(c)
         obs.append(terms)
     e = batch.expectation(circs, obs)

@@ -172,3 +172,17 @@ def test_emulated_batch_layout_matches_oracle(oracle, n):
     oracle.apply_circuit(exp, n, c.gates)
     got = c.native().emulate(a0, "c64", batch_layout=True)
     assert np.max(np.abs(got - exp)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [13, 16])
+def test_emulated_batch_layout_quadratic_phases(oracle, n):
+    """The batched kernel's HBM passes with matrix-core blocks and quadratic
+    phase ops (diagonal runs over qubits inside and outside the tile)."""
+    c = diagonal_heavy_circuit(n, 300, 40 + n)
+    nc = c.native()
+    assert nc.plan("c64", tile_qubits=12)["qphase_gates"] > 0
+    a0 = random_state(n, 3 * n)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    got = nc.emulate(a0, "c64", batch_layout=True)
+    assert np.max(np.abs(got - exp)) < 1e-12
