@@ -202,7 +202,7 @@ typedef enum {
 typedef struct {
   uint32_t fusion;       /* NqpuFusion; default NQPU_FUSION_TILED */
   uint32_t semantics;    /* NqpuSemantics; default NQPU_SEMANTICS_STANDARD */
-  uint32_t tile_qubits;  /* 0 = autotuned default */
+  uint32_t tile_qubits;  /* 0 = default: 12 (complex64) / 11 (complex128), one less for small states */
   uint32_t low_qubits;   /* contiguous low qubits always in a tile; 0 = default */
   uint32_t flags;        /* NQPU_RUN_TIME_PASSES: time every pass with HIP events */
 } NqpuRunOptions;
@@ -268,6 +268,54 @@ NqpuError nqpu_expectation_pauli(const NqpuState* state, uint64_t xmask, uint64_
 NqpuError nqpu_expectation_pauli_sum(const NqpuState* state, const uint64_t* xmasks,
                                      const uint64_t* zmasks, const double* coeffs,
                                      uint64_t count, double* out);
+/* out[t] = <P_t> for count Pauli strings (masks as nqpu_expectation_pauli).
+ * Terms whose X/Y masks are equal share one pass over the state (all
+ * diagonal strings of a Hamiltonian in one pass; up to 32 terms per pass);
+ * nqpu_expectation_pauli_sum evaluates its terms the same way.  The
+ * reference evaluates each string in its own loop
+ * (expectation_hamiltonian, sdk/rust/src/lib.rs:518). */
+NqpuError nqpu_expectation_pauli_values(const NqpuState* state, const uint64_t* xmasks, const uint64_t* zmasks,
+                                        uint64_t count, double* out);
+
+/* Qubit order of a Pauli string. */
+typedef enum {
+  /* character i acts on qubit i = index bit i (expectation_pauli_string,
+   * sdk/rust/src/lib.rs:465-514) */
+  NQPU_ORDER_LSB = 0,
+  /* character i acts on qubit i of the Estimator's convention, index bit
+   * n-1-i (Observable::from_string + pauli_expectation,
+   * sdk/rust/src/core/foundations/primitives.rs:111-133, 683-728) */
+  NQPU_ORDER_MSB = 1
+} NqpuQubitOrder;
+
+/* <P> for a string of 'I', 'X', 'Y', 'Z' (either case) of length <= n.
+ * Longer strings -> NQPU_INVALID_QUBIT, other characters ->
+ * NQPU_INVALID_PARAMETER (the reference panics).  The value is the physical
+ * <psi|P|psi>; lib.rs:465-514 returns its negative for an odd number of Y
+ * factors, which NQPU_SEMANTICS_REFERENCE reproduces nowhere here. */
+NqpuError nqpu_expectation_pauli_string(const NqpuState* state, const char* paulis, uint32_t order, double* out);
+
+/* The Estimator's exact value of an observable (Estimator::run with
+ * shots = None, estimate_observable, primitives.rs:1021-1083): sum over terms
+ * t of coeffs[t] * pauli_expectation(term t) (primitives.rs:683-728).  Term t
+ * is the factors (qubits[k], paulis[k]) for k in [offsets[t], offsets[t+1]),
+ * qubit 0 being the most significant index bit; a state built by a circuit
+ * whose qubit q is this engine's qubit n-1-q.  Factors are applied as the
+ * reference does (each against the original index bits), so repeated
+ * qubits compose as there. */
+NqpuError nqpu_estimator_expectation(const NqpuState* state, const uint32_t* qubits, const char* paulis,
+                                     const uint64_t* offsets, const double* coeffs, uint64_t num_terms,
+                                     double* out);
+
+/* reset_qubit (sdk/rust/src/lib.rs:1705-1730): zero every amplitude whose
+ * bit `qubit` is 1, then divide by the square root of the remaining mass
+ * when it is positive (FP64, on the device). */
+NqpuError nqpu_state_reset_qubit(NqpuState* state, uint32_t qubit);
+/* initialize_from_amplitudes (sdk/rust/src/lib.rs:1918-1935): len must be
+ * 2^n and the FP64 norm at least 1e-10, else NQPU_INVALID_PARAMETER and the
+ * state is unchanged; the state becomes amplitudes / ||amplitudes||. */
+NqpuError nqpu_state_initialize(NqpuState* state, const NqpuComplex* amplitudes, uint64_t len);
+
 /* Scale the state to unit norm (FP64 norm). */
 NqpuError nqpu_state_normalize(NqpuState* state);
 /* Copy src's amplitudes into dst (same qubit count and precision), one
@@ -314,7 +362,11 @@ typedef struct {
   uint64_t observable_passes;
   double plan_ms;           /* host planning and upload */
   double kernel_ms;         /* the persistent launch */
-  double state_bytes;       /* HBM reserved for HBM-resident states */
+  double state_bytes;       /* HBM for the states of circuits above 12 qubits: one slot of the
+                               largest such circuit per workgroup (nqpu_debug_batch_plan: one slot) */
+  double hbm_bytes;         /* algorithmic state traffic of the launch: sum over circuits above 12
+                               qubits of (2 x gate passes - 1 + observable passes) x 2^n x 8 bytes
+                               (the first gate pass only writes) */
 } NqpuBatchStats;
 
 /* A reusable batch context (stream and device buffers).  Not thread-safe. */

@@ -69,7 +69,7 @@ class NqpuPauliTerm(ct.Structure):
 class NqpuBatchStats(ct.Structure):
     _fields_ = [("circuits", ct.c_uint64), ("resident", ct.c_uint64), ("gate_passes", ct.c_uint64),
                 ("observable_passes", ct.c_uint64), ("plan_ms", ct.c_double), ("kernel_ms", ct.c_double),
-                ("state_bytes", ct.c_double)]
+                ("state_bytes", ct.c_double), ("hbm_bytes", ct.c_double)]
 
 
 RUN_BATCH_LAYOUT = 4
@@ -98,6 +98,9 @@ class NqpuServerStats(ct.Structure):
     _fields_ = [("submitted", ct.c_uint64), ("completed", ct.c_uint64), ("failed", ct.c_uint64),
                 ("peak_running", ct.c_uint64), ("peak_bytes", ct.c_uint64), ("memory_bytes", ct.c_uint64)]
 
+
+ORDER_LSB = 0
+ORDER_MSB = 1
 
 REQUEST_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 WAIT_FOREVER = 0xFFFFFFFF
@@ -155,6 +158,11 @@ SIGNATURES = {
     "nqpu_expectation_z_all": (_E, [_P, _P, ct.c_uint32]),
     "nqpu_expectation_pauli": (_E, [_P, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_double)]),
     "nqpu_expectation_pauli_sum": (_E, [_P, _P, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),
+    "nqpu_expectation_pauli_values": (_E, [_P, _P, _P, ct.c_uint64, _P]),
+    "nqpu_expectation_pauli_string": (_E, [_P, ct.c_char_p, ct.c_uint32, ct.POINTER(ct.c_double)]),
+    "nqpu_estimator_expectation": (_E, [_P, _P, ct.c_char_p, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),
+    "nqpu_state_reset_qubit": (_E, [_P, ct.c_uint32]),
+    "nqpu_state_initialize": (_E, [_P, _P, ct.c_uint64]),
     "nqpu_state_normalize": (_E, [_P]),
     "nqpu_state_overlap": (_E, [_P, _P, ct.POINTER(NqpuComplex)]),
     "nqpu_state_copy": (_E, [_P, _P]),

@@ -6,6 +6,8 @@
 #include "runtime.h"
 
 #include <cmath>
+#include <algorithm>
+#include <cstring>
 #include <new>
 
 using namespace nqpu;
@@ -317,19 +319,114 @@ NqpuError nqpu_expectation_pauli(const NqpuState* st, uint64_t xmask, uint64_t z
   return pauli(S(st), xmask, zmask, out);
 }
 
+namespace {
+NqpuError check_masks(const NqpuState* st, const uint64_t* xm, const uint64_t* zm, uint64_t count) {
+  const int n = S(st)->n;
+  const uint64_t lim = n >= 64 ? ~0ull : ((1ull << n) - 1);
+  for (uint64_t k = 0; k < count; ++k)
+    if ((xm[k] | zm[k]) & ~lim) return NQPU_INVALID_QUBIT;
+  return NQPU_OK;
+}
+}  // namespace
+
 NqpuError nqpu_expectation_pauli_sum(const NqpuState* st, const uint64_t* xm, const uint64_t* zm,
                                      const double* coeffs, uint64_t count, double* out) {
   if (!st) return NQPU_NULL_POINTER;
   if (!out || (count && (!xm || !zm || !coeffs))) return NQPU_NULL_POINTER;
-  double acc = 0;
-  for (uint64_t k = 0; k < count; ++k) {
-    double v;
-    NqpuError e = nqpu_expectation_pauli(st, xm[k], zm[k], &v);
-    if (e != NQPU_OK) return e;
-    acc += coeffs[k] * v;
+  NqpuError e = check_masks(st, xm, zm, count);
+  if (e != NQPU_OK) return e;
+  if (count == 0) {
+    *out = 0.0;
+    return NQPU_OK;
   }
+  return pauli_sum(S(st), xm, zm, coeffs, count, out);
+}
+
+NqpuError nqpu_expectation_pauli_values(const NqpuState* st, const uint64_t* xm, const uint64_t* zm, uint64_t count,
+                                        double* out) {
+  if (!st) return NQPU_NULL_POINTER;
+  if (count && (!out || !xm || !zm)) return NQPU_NULL_POINTER;
+  NqpuError e = check_masks(st, xm, zm, count);
+  if (e != NQPU_OK) return e;
+  std::vector<PauliTerm> terms(count);
+  for (uint64_t t = 0; t < count; ++t) terms[t] = PauliTerm{xm[t], zm[t], 1.0};
+  std::vector<double> v;
+  e = pauli_values(S(st), terms, &v);
+  if (e != NQPU_OK) return e;
+  std::copy(v.begin(), v.end(), out);
+  return NQPU_OK;
+}
+
+NqpuError nqpu_expectation_pauli_string(const NqpuState* st, const char* paulis, uint32_t order, double* out) {
+  if (!st || !paulis || !out) return NQPU_NULL_POINTER;
+  if (order != NQPU_ORDER_LSB && order != NQPU_ORDER_MSB) return NQPU_INVALID_PARAMETER;
+  const int n = S(st)->n;
+  const size_t len = std::strlen(paulis);
+  if (len > (size_t)n) return NQPU_INVALID_QUBIT;
+  uint64_t x = 0, z = 0;
+  for (size_t i = 0; i < len; ++i) {
+    const int bitpos = order == NQPU_ORDER_LSB ? (int)i : n - 1 - (int)i;
+    switch (paulis[i]) {
+      case 'I': case 'i': break;
+      case 'X': case 'x': x |= 1ull << bitpos; break;
+      case 'Y': case 'y': x |= 1ull << bitpos; z |= 1ull << bitpos; break;
+      case 'Z': case 'z': z |= 1ull << bitpos; break;
+      default: return NQPU_INVALID_PARAMETER;
+    }
+  }
+  return pauli(S(st), x, z, out);
+}
+
+NqpuError nqpu_estimator_expectation(const NqpuState* st, const uint32_t* qubits, const char* paulis,
+                                     const uint64_t* offsets, const double* coeffs, uint64_t num_terms,
+                                     double* out) {
+  if (!st || !out) return NQPU_NULL_POINTER;
+  if (num_terms && (!offsets || !coeffs)) return NQPU_NULL_POINTER;
+  const int n = S(st)->n;
+  if (num_terms && offsets[num_terms] > offsets[0] && (!qubits || !paulis)) return NQPU_NULL_POINTER;
+  // A term's factors act on the index bits n-1-qubit.  They are applied as
+  // the reference does, each against the bits of the original index: X and
+  // Y flip, Y multiplies by i on |0> and -i on |1>, Z by -1 on |1>, so the
+  // term is i^(#Y) times a flip mask and a sign mask (repeated qubits
+  // compose in that way too).
+  std::vector<PauliTerm> terms(num_terms);
+  std::vector<int> phase(num_terms, 0);
+  for (uint64_t t = 0; t < num_terms; ++t) {
+    if (offsets[t + 1] < offsets[t]) return NQPU_INVALID_PARAMETER;
+    uint64_t x = 0, z = 0;
+    int ny = 0;
+    for (uint64_t k = offsets[t]; k < offsets[t + 1]; ++k) {
+      if (qubits[k] >= (uint32_t)n) return NQPU_INVALID_QUBIT;
+      const uint64_t b = 1ull << (n - 1 - (int)qubits[k]);
+      switch (paulis[k]) {
+        case 'I': case 'i': break;
+        case 'X': case 'x': x ^= b; break;
+        case 'Y': case 'y': x ^= b; z ^= b; ++ny; break;
+        case 'Z': case 'z': z ^= b; break;
+        default: return NQPU_INVALID_PARAMETER;
+      }
+    }
+    terms[t] = PauliTerm{x, z, coeffs[t]};
+    phase[t] = ny & 3;
+  }
+  std::vector<double> v;
+  NqpuError e = pauli_values(S(st), terms, &v, &phase);
+  if (e != NQPU_OK) return e;
+  double acc = 0;
+  for (uint64_t t = 0; t < num_terms; ++t) acc += coeffs[t] * v[t];
   *out = acc;
   return NQPU_OK;
+}
+
+NqpuError nqpu_state_reset_qubit(NqpuState* st, uint32_t qubit) {
+  if (!st) return NQPU_NULL_POINTER;
+  if ((int)qubit >= S(st)->n) return NQPU_INVALID_QUBIT;
+  return reset_qubit(S(st), (int)qubit);
+}
+
+NqpuError nqpu_state_initialize(NqpuState* st, const NqpuComplex* amplitudes, uint64_t len) {
+  if (!st || !amplitudes) return NQPU_NULL_POINTER;
+  return initialize_amplitudes(S(st), amplitudes, len);
 }
 
 NqpuError nqpu_sample(const NqpuState* st, uint32_t shots, uint64_t* out) {

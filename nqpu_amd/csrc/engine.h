@@ -56,7 +56,8 @@ struct DevOp {
 // k_batch: one persistent launch over many complex64 circuits.  Each
 // workgroup takes whole circuits from an atomic queue; a circuit of at most
 // 8 + kBatchEB qubits lives in registers/LDS for its whole life, a larger
-// one streams register-tile passes over its own HBM region.  Observables
+// one streams register-tile passes over the HBM slot of the workgroup that
+// runs it (slot_amps amplitudes per workgroup, reused circuit after circuit).  Observables
 // (Pauli sums) are evaluated on LDS tiles in FP64.
 constexpr int kBatchEB = 4;
 
@@ -67,7 +68,7 @@ struct BatchCircuit {
   int32_t red0, nred;    // observable passes (BatchPass table; resident: the gate pass)
   int32_t out;           // output slot
   int32_t pad;
-  uint64_t state_off;    // amplitude offset of the HBM region (not resident)
+  uint64_t amps;         // amplitudes of the HBM state (not resident)
 };
 
 struct BatchPass {
@@ -90,7 +91,7 @@ struct BatchTerm {
   int32_t pad;
 };
 
-void launch_batch(void* states, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
+void launch_batch(void* states, uint64_t slot_amps, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
                   const RegOp<float>* prog, const BatchTerm* terms, const unsigned char* blob, unsigned* counter,
                   double* out, int grid, hipStream_t s);
 
@@ -116,16 +117,9 @@ struct Launch {
   // FP64 reductions: per-chunk partials, then deterministic final sums.
   static void sumsq_chunks(const void* st, int n, int chunk_bits, double* out, hipStream_t s);
   static void zsum_chunks(const void* st, int n, int chunk_bits, double* out, hipStream_t s);
-  static void pauli_chunks(const void* st, int n, uint64_t xmask, uint64_t zmask, int chunk_bits,
-                           double* out, hipStream_t s);
-  static void probs(const void* st, int n, double* out, hipStream_t s);
   static void to_c128(const void* st, uint64_t count, uint64_t off, void* out, hipStream_t s);
   static void from_c128(void* st, uint64_t count, uint64_t off, const void* in, hipStream_t s);
   static void collapse(void* st, int n, int q, uint32_t outcome, double renorm, hipStream_t s);
-  static void sample_search(const void* st, int n, int chunk_bits, int sub_bits,
-                            const double* chunk_prefix, const double* sub_sums,
-                            const double* r, uint64_t shots, uint64_t* out, hipStream_t s);
-  static void sub_sums(const void* st, int n, int sub_bits, double* out, hipStream_t s);
 };
 
 // Reduce per-chunk double partials [nchunks][width] -> [width] in a fixed
@@ -136,6 +130,5 @@ void launch_copy(void* dst, const void* src, uint64_t dim, bool c64, hipStream_t
 // Per-chunk [Re, Im] of sum conj(a_i) b_i; a64/b64 select complex64 storage.
 void launch_overlap(const void* a, bool a64, const void* b, bool b64, int n, int cb, double* out,
                     hipStream_t s);
-void launch_exclusive_scan(const double* in, uint64_t count, double* out, hipStream_t s);
 
 }  // namespace nqpu

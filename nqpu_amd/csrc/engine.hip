@@ -2354,7 +2354,7 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
   if (compile && std::is_same<R, float>::value && eb == 4) {
     std::vector<RegOp<float>> progf(prog.size());
     std::memcpy(progf.data(), prog.data(), prog.size() * sizeof(RegOp<R>));
-    const NqpuError e = jit_pass_functions(progf, args, eb, &fns);
+    const NqpuError e = jit_pass_functions(progf, args, eb, &fns, &s->jit_hold);
     if (e != NQPU_OK) return e;
   }
   // Upload: program and operand blobs go through the pinned stage with one
@@ -2582,6 +2582,7 @@ NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o) {
     return e;
   }
   HIPTRY(hipStreamSynchronize(s->stream), "run");
+  s->jit_hold.clear();
   if (s->timer.on) {
     s->stats.pass_ms = s->timer.total_ms();
     s->stats.timed_passes = s->timer.used / 2;
@@ -2694,32 +2695,6 @@ NqpuError set_amplitudes(State* s, uint64_t off, uint64_t count, const NqpuCompl
 
 static int chunk_bits_for(int n) { return std::min(n, 16); }
 
-NqpuError probabilities(const State* s0, double* out) {
-  State* s = const_cast<State*>(s0);
-  const uint64_t dim = s->dim();
-  // the probs kernel covers the whole state; stage in slices of kXferChunk
-  if (dim <= kXferChunk) {
-    if (!s->scratch.ensure(dim * 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
-    dispatch(s, [&](auto* tag) {
-      using R = std::remove_pointer_t<decltype(tag)>;
-      Launch<R>::probs(s->amps, s->n, (double*)s->scratch.p, s->stream);
-    });
-    HIPTRY(hipGetLastError(), "probs");
-    HIPTRY(hipStreamSynchronize(s->stream), "probs");
-    HIPTRY(copy_on(s->stream, out, s->scratch.p, dim * 8, hipMemcpyDeviceToHost), "probs copy");
-    return NQPU_OK;
-  }
-  // large states: convert slices to c128 on the host side of the copy
-  std::vector<NqpuComplex> buf(kXferChunk);
-  for (uint64_t b = 0; b < dim; b += kXferChunk) {
-    uint64_t c = std::min(kXferChunk, dim - b);
-    NqpuError e = get_amplitudes(s, b, c, buf.data());
-    if (e != NQPU_OK) return e;
-    for (uint64_t i = 0; i < c; ++i) out[b + i] = buf[i].re * buf[i].re + buf[i].im * buf[i].im;
-  }
-  return NQPU_OK;
-}
-
 static NqpuError chunk_sums(const State* s0, std::vector<double>* sums, int* cb_out) {
   State* s = const_cast<State*>(s0);
   const int cb = chunk_bits_for(s->n);
@@ -2790,39 +2765,6 @@ NqpuError zsum_all(const State* s0, double* tot, std::vector<double>* s1) {
   return NQPU_OK;
 }
 
-NqpuError pauli(const State* s0, uint64_t x, uint64_t z, double* out) {
-  State* s = const_cast<State*>(s0);
-  const int n = s->n;
-  const uint64_t work = x ? (s->dim() >> 1) : s->dim();
-  const int cb = std::min(16, std::max(0, (int)(x ? n - 1 : n)));
-  const uint64_t nch = (work + (1ull << cb) - 1) >> cb;
-  if (!s->scratch.ensure(nch * 16)) return NQPU_OUT_OF_DEVICE_MEMORY;
-  dispatch(s, [&](auto* tag) {
-    using R = std::remove_pointer_t<decltype(tag)>;
-    Launch<R>::pauli_chunks(s->amps, n, x, z, cb, (double*)s->scratch.p, s->stream);
-  });
-  HIPTRY(hipGetLastError(), "pauli");
-  HIPTRY(hipStreamSynchronize(s->stream), "pauli");
-  std::vector<double> rows(nch * 2);
-  HIPTRY(copy_on(s->stream, rows.data(), s->scratch.p, nch * 16, hipMemcpyDeviceToHost), "pauli copy");
-  double sr = 0, si = 0;
-  for (uint64_t c = 0; c < nch; ++c) {
-    sr += rows[2 * c];
-    si += rows[2 * c + 1];
-  }
-  // <P> = Re(i^{#Y} * S)
-  int ny = __builtin_popcountll(x & z) & 3;
-  double r = 0;
-  switch (ny) {
-    case 0: r = sr; break;
-    case 1: r = -si; break;
-    case 2: r = -sr; break;
-    default: r = si; break;
-  }
-  *out = r;
-  return NQPU_OK;
-}
-
 NqpuError scale(State* s, double f) {
   dispatch(s, [&](auto* tag) {
     using R = std::remove_pointer_t<decltype(tag)>;
@@ -2833,59 +2775,41 @@ NqpuError scale(State* s, double f) {
   return NQPU_OK;
 }
 
-NqpuError sample(const State* s0, const std::vector<double>& r, uint64_t* out) {
-  State* s = const_cast<State*>(s0);
-  if (r.empty()) return NQPU_OK;
-  std::vector<double> sums;
-  int cb;
-  NqpuError e = chunk_sums(s, &sums, &cb);
-  if (e != NQPU_OK) return e;
-  const uint64_t nch = sums.size();
-  std::vector<double> pre(nch + 1);
-  pre[0] = 0;
-  for (uint64_t c = 0; c < nch; ++c) pre[c + 1] = pre[c] + sums[c];
-  const int sb = std::min(cb, 10);
-  const uint64_t nsub = s->dim() >> sb;
-  const uint64_t shots = r.size();
-  // scratch layout: [pre (nch+1)] [subs (nsub)] [r (shots)] [out (shots)]
-  size_t bytes = (nch + 1 + nsub + shots) * 8 + shots * 8;
-  if (!s->scratch.ensure(bytes)) return NQPU_OUT_OF_DEVICE_MEMORY;
-  double* dpre = (double*)s->scratch.p;
-  double* dsub = dpre + (nch + 1);
-  double* dr = dsub + nsub;
-  uint64_t* dout = (uint64_t*)(dr + shots);
-  HIPTRY(copy_on(s->stream, dpre, pre.data(), (nch + 1) * 8, hipMemcpyHostToDevice), "prefix upload");
-  HIPTRY(copy_on(s->stream, dr, r.data(), shots * 8, hipMemcpyHostToDevice), "uniforms upload");
-  dispatch(s, [&](auto* tag) {
-    using R = std::remove_pointer_t<decltype(tag)>;
-    Launch<R>::sub_sums(s->amps, s->n, sb, dsub, s->stream);
-    Launch<R>::sample_search(s->amps, s->n, cb, sb, dpre, dsub, dr, shots, dout, s->stream);
-  });
-  HIPTRY(hipGetLastError(), "sample");
-  HIPTRY(hipStreamSynchronize(s->stream), "sample");
-  HIPTRY(copy_on(s->stream, out, dout, shots * 8, hipMemcpyDeviceToHost), "sample copy");
-  return NQPU_OK;
-}
-
-// c_ffi.rs:473-500: p0 over bit q = 0, p1 = 1 - p0, collapse and renormalise.
+// c_ffi.rs:473-500: p0 over bit q = 0, p1 = 1 - p0 as the reported
+// probability; the outcome is drawn against p0.  The collapse divides by the
+// kept half's measured mass rather than by the reported probability: in
+// complex64 the norm drifts by ~1e-7, and an outcome whose measured mass is
+// zero (possible only through that drift) is replaced by the other one, so
+// the state never collapses to all zeros.
 NqpuError measure_qubit(State* s, int q, double r, uint32_t* outcome, double* prob) {
   double tot;
   std::vector<double> s1;
   NqpuError e = zsum_all(s, &tot, &s1);
   if (e != NQPU_OK) return e;
-  double p0 = tot - s1[q];
-  double p1 = 1.0 - p0;
+  const double m0 = tot - s1[q], m1 = s1[q];
+  const double p0 = m0;
+  const double p1 = 1.0 - p0;
   uint32_t o = r < p0 ? 0 : 1;
-  double po = o ? p1 : p0;
-  double rn = 1.0 / std::sqrt(po);
+  if ((o ? m1 : m0) <= 0.0) o ^= 1;
+  const double kept = o ? m1 : m0;
+  if (!(kept > 0.0)) {
+    set_error("measure: the state has no mass");
+    return NQPU_SIMULATION_FAILED;
+  }
+  e = collapse(s, q, o, 1.0 / std::sqrt(kept));
+  if (e != NQPU_OK) return e;
+  *outcome = o;
+  *prob = o ? p1 : p0;
+  return NQPU_OK;
+}
+
+NqpuError collapse(State* s, int q, uint32_t outcome, double f) {
   dispatch(s, [&](auto* tag) {
     using R = std::remove_pointer_t<decltype(tag)>;
-    Launch<R>::collapse(s->amps, s->n, q, o, rn, s->stream);
+    Launch<R>::collapse(s->amps, s->n, q, outcome, f, s->stream);
   });
   HIPTRY(hipGetLastError(), "collapse");
   HIPTRY(hipStreamSynchronize(s->stream), "collapse");
-  *outcome = o;
-  *prob = po;
   return NQPU_OK;
 }
 
@@ -2993,6 +2917,13 @@ CircuitPlan plan_batch_circuit(const Circuit& c, const std::vector<PauliTerm>& t
   po.tile_qubits = KH;
   po.low_qubits = 5;
   std::vector<Pass> passes = plan_tiles(ops, n, po);
+  if (passes.empty()) {
+    // no gate survives merging (an empty circuit, or ops that cancel): one
+    // pass with an empty program still writes the |0...0> start state
+    Pass p0;
+    for (int q = 0; q < KH; ++q) p0.qubits.push_back(q);
+    passes.push_back(p0);
+  }
   C.pass0 = 0;
   C.npass = (int)passes.size();
   for (const Pass& p : passes) {
@@ -3095,7 +3026,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   // offsets of each circuit's passes, program ops and terms in the tables
   std::vector<size_t> bp_at(nc + 1, 0), pg_at(nc + 1, 0), bt_at(nc + 1, 0), bl_at(nc + 1, 0);
   std::vector<BatchCircuit> bc(nc);
-  uint64_t state_amps = 0;
+  uint64_t slot_amps = 0;
   NqpuBatchStats st{};
   for (size_t k = 0; k < nc; ++k) {
     const CircuitPlan& cp = cps[k];
@@ -3111,10 +3042,13 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
     C.pass0 += (int32_t)bp_at[k];
     C.red0 += (int32_t)bp_at[k];
     if (!C.resident) {
-      C.state_off = state_amps;
-      state_amps += cp.amps;
+      C.amps = cp.amps;
+      slot_amps = std::max(slot_amps, cp.amps);
       st.gate_passes += C.npass;
       st.observable_passes += C.nred;
+      // gate passes write the state and read it (all but the first, which
+      // starts from |0...0>); observable passes read it
+      st.hbm_bytes += (2.0 * C.npass - 1.0 + C.nred) * (double)cp.amps * 8.0;
     } else {
       st.resident++;
     }
@@ -3177,11 +3111,11 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   plan.ocnt = ocnt;
   plan.obl = obl;
   plan.total = total;
-  plan.state_amps = state_amps;
+  plan.slot_amps = slot_amps;
   plan.ncirc = nc;
   plan.program_ops = pg_at[nc];
   st.circuits = circuits.size();
-  st.state_bytes = (double)state_amps * 8.0;
+  st.state_bytes = (double)slot_amps * 8.0;
   plan.st = st;
   return NQPU_OK;
 }
@@ -3209,8 +3143,13 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   NqpuError pe = plan_batch(circuits, terms, o, plan, pinned);
   if (pe != NQPU_OK) return pe;
   const size_t oc = plan.oc, op_ = plan.op, og = plan.og, ot = plan.ot, ocnt = plan.ocnt, total = plan.total;
-  const uint64_t state_amps = plan.state_amps;
   NqpuBatchStats st = plan.st;
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+  const int grid = (int)std::min<size_t>(circuits.size(), (size_t)cus * 3);  // 3 workgroups fit per CU
+  // one state slot per workgroup: a workgroup runs its circuits one after
+  // another, so the HBM it needs is that of the largest circuit
+  const uint64_t state_amps = plan.slot_amps * (uint64_t)std::max(grid, 1);
   HIPTRY(hipStreamSynchronize(b->stream), "sync");
   if (!b->tables.ensure(total) || !b->out.ensure(circuits.size() * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
   if (state_amps && !b->states.ensure(state_amps * 8)) {
@@ -3219,16 +3158,13 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   }
   HIPTRY(copy_on(b->stream, b->tables.p, plan.host, total, hipMemcpyHostToDevice), "batch table upload");
   char* base = (char*)b->tables.p;
-  int cus = 256;
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
-  const int grid = (int)std::min<size_t>(circuits.size(), (size_t)cus * 3);  // 3 workgroups fit per CU
   const auto t1 = clock::now();
   hipEvent_t e0, e1;
   HIPTRY(hipEventCreate(&e0), "event");
   HIPTRY(hipEventCreate(&e1), "event");
   (void)hipEventRecord(e0, b->stream);
   if (grid > 0)
-    launch_batch(b->states.p, (const BatchCircuit*)(base + oc), (int)plan.ncirc, (const BatchPass*)(base + op_),
+    launch_batch(b->states.p, plan.slot_amps, (const BatchCircuit*)(base + oc), (int)plan.ncirc, (const BatchPass*)(base + op_),
                  (const RegOp<float>*)(base + og), (const BatchTerm*)(base + ot),
                  (const unsigned char*)(base + plan.obl), (unsigned*)(base + ocnt), (double*)b->out.p, grid,
                  b->stream);

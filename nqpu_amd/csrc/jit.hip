@@ -9,8 +9,11 @@
 // so values stay in place.  Matrices, controls and remap tables are still
 // read from the uploaded program at run time: a module depends only on the
 // program's structure, and circuits that differ only in their angles reuse it.
-// Modules are compiled with hipRTC for the current device and kept for the
-// life of the process, keyed by their source text.
+// Modules are compiled with hipRTC for the current device and cached by their
+// source text.  The cache holds at most kJitCacheModules modules: past that,
+// the least recently used one leaves the cache, and it is unloaded once no
+// run that launched its kernels still holds it (a run holds its modules
+// until its stream is synchronised).
 #include <hip/hiprtc.h>
 
 #include <cstdio>
@@ -33,11 +36,27 @@ namespace {
 struct JitModule {
   hipModule_t mod = nullptr;
   std::vector<hipFunction_t> fns;
+  uint64_t last_use = 0;
+  ~JitModule() {
+    if (mod) (void)hipModuleUnload(mod);
+  }
 };
 
+constexpr size_t kJitCacheModules = 512;  // 8 passes per module
 std::mutex g_mu;
-std::map<std::string, std::unique_ptr<JitModule>> g_cache;  // kept until the process ends
+std::map<std::string, std::shared_ptr<JitModule>> g_cache;
 uint64_t g_compiled_passes = 0;
+uint64_t g_use_clock = 0;
+
+// Drop least recently used modules beyond the cap (caller holds g_mu).
+void evict_locked() {
+  while (g_cache.size() > kJitCacheModules) {
+    auto victim = g_cache.begin();
+    for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+      if (it->second->last_use < victim->second->last_use) victim = it;
+    g_cache.erase(victim);  // unloaded when the last run holding it lets go
+  }
+}
 
 void emit_op(std::ostringstream& o, const RegOp<float>& r, int k, int eb) {
   switch (r.kind) {
@@ -193,25 +212,29 @@ NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::ve
 }
 
 NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args,
-                             int eb, std::vector<hipFunction_t>* fns) {
+                             int eb, std::vector<hipFunction_t>* fns, std::vector<std::shared_ptr<void>>* hold) {
   // passes are split over a few modules compiled on parallel threads
   const int np = (int)args.size();
   const int per = kPassesPerModule;
   const std::vector<std::string> srcs = module_sources(prog, args, eb);
   dump_sources(srcs);
   const int nmod = (int)srcs.size();
-  std::vector<JitModule*> mods(nmod, nullptr);
+  std::vector<std::shared_ptr<JitModule>> mods(nmod);
   std::vector<int> todo;
   {
     std::lock_guard<std::mutex> lk(g_mu);
     for (int m = 0; m < nmod; ++m) {
       auto it = g_cache.find(srcs[m]);
-      if (it != g_cache.end()) mods[m] = it->second.get();
-      else todo.push_back(m);
+      if (it != g_cache.end()) {
+        mods[m] = it->second;
+        mods[m]->last_use = ++g_use_clock;
+      } else {
+        todo.push_back(m);
+      }
     }
   }
   if (!todo.empty()) {
-    std::vector<std::unique_ptr<JitModule>> built(todo.size());
+    std::vector<std::shared_ptr<JitModule>> built(todo.size());
     std::vector<NqpuError> errs(todo.size(), NQPU_OK);
     std::vector<std::string> msgs(todo.size());
     int dev = 0;
@@ -220,7 +243,7 @@ NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::v
     for (size_t t = 0; t < todo.size(); ++t) {
       th.emplace_back([&, t] {
         (void)hipSetDevice(dev);
-        built[t].reset(new JitModule);
+        built[t] = std::make_shared<JitModule>();
         const int m = todo[t];
         const int nfn = std::min(np, (m + 1) * per) - m * per;
         errs[t] = compile_module(srcs[m], nfn, built[t].get());
@@ -238,14 +261,17 @@ NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::v
       const int m = todo[t];
       auto& slot = g_cache[srcs[m]];
       if (!slot) {
-        slot = std::move(built[t]);
+        slot = built[t];
         g_compiled_passes += slot->fns.size();
       }
-      mods[m] = slot.get();
+      slot->last_use = ++g_use_clock;
+      mods[m] = slot;
     }
+    evict_locked();
   }
   fns->resize(np);
   for (int i = 0; i < np; ++i) (*fns)[i] = mods[i / per]->fns[i % per];
+  for (auto& m : mods) hold->push_back(m);
   return NQPU_OK;
 }
 

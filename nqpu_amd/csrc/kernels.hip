@@ -478,7 +478,8 @@ __device__ double tile_terms(const float2* sm, uint32_t TT, const BatchTerm* ter
 }
 
 template <int EB>
-__global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, const BatchCircuit* __restrict__ circs,
+__global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, uint64_t slot_amps,
+                                                  const BatchCircuit* __restrict__ circs,
                                                   int ncirc, const BatchPass* __restrict__ passes,
                                                   const RegOp<float>* __restrict__ prog,
                                                   const BatchTerm* __restrict__ terms,
@@ -520,7 +521,7 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
       acc = tile_terms(sm, 1u << P.K, terms + P.term0, P.nterm, 0, tid);
       __syncthreads();
     } else {
-      T* st = states + C.state_off;
+      T* st = states + (uint64_t)blockIdx.x * slot_amps;  // this workgroup's slot
       for (int p = 0; p < C.npass; ++p) {
         const BatchPass P = passes[C.pass0 + p];
         const uint64_t ntiles = 1ull << (C.n - P.K);
@@ -598,12 +599,12 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, c
   }
 }
 
-void launch_batch(void* states, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
+void launch_batch(void* states, uint64_t slot_amps, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
                   const RegOp<float>* prog, const BatchTerm* terms, const unsigned char* blob, unsigned* counter,
                   double* out, int grid, hipStream_t s) {
   const size_t lds = sizeof(float2) << (8 + kBatchEB);
-  hipLaunchKernelGGL(k_batch<kBatchEB>, dim3(grid), dim3(256), lds, s, (float2*)states, circs, ncirc, passes, prog,
-                     terms, blob, counter, out);
+  hipLaunchKernelGGL(k_batch<kBatchEB>, dim3(grid), dim3(256), lds, s, (float2*)states, slot_amps, circs, ncirc,
+                     passes, prog, terms, blob, counter, out);
 }
 
 // out[c] = sum |a_i|^2 over chunk c of 2^cb amplitudes.
@@ -651,46 +652,6 @@ __global__ __launch_bounds__(kBlock) void k_zsum(const T* st, int cb, double* ou
   }
 }
 
-// Pauli string: partial sums of S = sum_i (-1)^popc(i&z) conj(a_{i^x}) a_i
-// over amplitude pairs (i, i^x) with bit h of i clear (h = top bit of x), or
-// over all i when x == 0.  Row c = [Re S_c, Im S_c].
-template <typename R, typename T>
-__global__ __launch_bounds__(kBlock) void k_pauli(const T* st, uint64_t work, int cb, uint64_t x,
-                                                   uint64_t z, int h, double* out) {
-  __shared__ double red[kBlock / 64];
-  const uint64_t ch = 1ull << cb, beg = (uint64_t)blockIdx.x * ch, end = min(beg + ch, work);
-  double sr = 0, si = 0;
-  for (uint64_t k = beg + threadIdx.x; k < end; k += kBlock) {
-    if (x == 0) {
-      double p = nsq<R>(st[k]);
-      sr += (__popcll(k & z) & 1) ? -p : p;
-    } else {
-      uint64_t i = ins0(k, h), j = i ^ x;
-      T ai = st[i], aj = st[j];
-      double si_ = (__popcll(i & z) & 1) ? -1.0 : 1.0;
-      double sj_ = (__popcll(j & z) & 1) ? -1.0 : 1.0;
-      // s_i conj(a_j) a_i + s_j conj(a_i) a_j
-      double cr = (double)aj.x * ai.x + (double)aj.y * ai.y;   // Re(conj(aj) ai)
-      double ci = (double)aj.x * ai.y - (double)aj.y * ai.x;   // Im(conj(aj) ai)
-      sr += si_ * cr + sj_ * cr;
-      si += si_ * ci - sj_ * ci;
-    }
-  }
-  double a = block_sum<kBlock>(sr, red);
-  double b = block_sum<kBlock>(si, red);
-  if (threadIdx.x == 0) {
-    out[2 * blockIdx.x] = a;
-    out[2 * blockIdx.x + 1] = b;
-  }
-}
-
-template <typename R, typename T>
-__global__ void k_probs(const T* st, uint64_t dim, double* out) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    out[i] = nsq<R>(st[i]);
-}
-
 template <typename R, typename T>
 __global__ void k_to_c128(const T* st, uint64_t count, uint64_t off, double2* out) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
@@ -725,66 +686,6 @@ __global__ void k_collapse(T* st, uint64_t dim, int q, uint32_t outcome, double 
       v.y = 0;
     }
     st[i] = v;
-  }
-}
-
-// Sum of |a|^2 over sub-blocks of 2^sb amplitudes.
-template <typename R, typename T>
-__global__ void k_subsums(const T* st, uint64_t nsub, int sb, double* out) {
-  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nsub;
-       k += (uint64_t)gridDim.x * blockDim.x) {
-    double s = 0;
-    const uint64_t b = k << sb;
-    for (uint64_t i = 0; i < (1ull << sb); ++i) s += nsq<R>(st[b + i]);
-    out[k] = s;
-  }
-}
-
-// One thread per shot: outcome = first i with r < cumsum_i (the C ABI's
-// linear scan, c_ffi.rs:506-525), found by chunk prefix -> sub-block sums ->
-// amplitudes.  r >= total mass returns dim-1, as the reference does.
-template <typename R, typename T>
-__global__ void k_sample(const T* st, int n, int cb, int sb, const double* cpre,
-                         const double* subs, const double* rr, uint64_t shots, uint64_t* out) {
-  const uint64_t dim = 1ull << n, nch = dim >> cb;
-  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s < shots;
-       s += (uint64_t)gridDim.x * blockDim.x) {
-    const double r = rr[s];
-    // first chunk c with r < cpre[c+1]  (cpre has nch+1 entries)
-    uint64_t lo = 0, hi = nch;
-    while (lo < hi) {
-      uint64_t mid = (lo + hi) >> 1;
-      if (r < cpre[mid + 1]) hi = mid; else lo = mid + 1;
-    }
-    if (lo >= nch) {
-      out[s] = dim - 1;
-      continue;
-    }
-    double cum = cpre[lo];
-    const uint64_t sub0 = (lo << cb) >> sb, nsub = 1ull << (cb - sb);
-    uint64_t res = ((lo + 1) << cb) - 1;
-    bool found = false;
-    for (uint64_t k = 0; k < nsub && !found; ++k) {
-      const double ss = subs[sub0 + k];
-      if (r < cum + ss) {
-        const uint64_t b = (sub0 + k) << sb;
-        for (uint64_t i = 0; i < (1ull << sb); ++i) {
-          cum += nsq<R>(st[b + i]);
-          if (r < cum) {
-            res = b + i;
-            found = true;
-            break;
-          }
-        }
-        if (!found) {  // rounding between the sub-block sum and the scan
-          res = b + (1ull << sb) - 1;
-          found = true;
-        }
-      } else {
-        cum += ss;
-      }
-    }
-    out[s] = res;
   }
 }
 
@@ -994,25 +895,6 @@ void Launch<R>::zsum_chunks(const void* st, int n, int cb, double* out, hipStrea
   hipLaunchKernelGGL((k_zsum<R, T_>), dim3((unsigned)nch), dim3(kBlock), 0, s, (const T_*)st, cb, out);
 }
 template <typename R>
-void Launch<R>::pauli_chunks(const void* st, int n, uint64_t x, uint64_t z, int cb, double* out,
-                             hipStream_t s) {
-  uint64_t work = x ? (1ull << (n - 1)) : (1ull << n);
-  int h = 0;
-  for (int q = 63; q >= 0; --q)
-    if ((x >> q) & 1) {
-      h = q;
-      break;
-    }
-  uint64_t nch = (work + (1ull << cb) - 1) >> cb;
-  hipLaunchKernelGGL((k_pauli<R, T_>), dim3((unsigned)nch), dim3(kBlock), 0, s, (const T_*)st, work, cb, x, z,
-                     h, out);
-}
-template <typename R>
-void Launch<R>::probs(const void* st, int n, double* out, hipStream_t s) {
-  uint64_t dim = 1ull << n;
-  hipLaunchKernelGGL((k_probs<R, T_>), dim3(grid_for(dim)), dim3(kBlock), 0, s, (const T_*)st, dim, out);
-}
-template <typename R>
 void Launch<R>::to_c128(const void* st, uint64_t count, uint64_t off, void* out, hipStream_t s) {
   hipLaunchKernelGGL((k_to_c128<R, T_>), dim3(grid_for(count)), dim3(kBlock), 0, s, (const T_*)st, count, off,
                      (double2*)out);
@@ -1028,19 +910,6 @@ void Launch<R>::collapse(void* st, int n, int q, uint32_t outcome, double renorm
   hipLaunchKernelGGL((k_collapse<R, T_>), dim3(grid_for(dim)), dim3(kBlock), 0, s, (T_*)st, dim, q, outcome,
                      renorm);
 }
-template <typename R>
-void Launch<R>::sub_sums(const void* st, int n, int sb, double* out, hipStream_t s) {
-  uint64_t nsub = (1ull << n) >> sb;
-  hipLaunchKernelGGL((k_subsums<R, T_>), dim3(grid_for(nsub)), dim3(kBlock), 0, s, (const T_*)st, nsub, sb, out);
-}
-template <typename R>
-void Launch<R>::sample_search(const void* st, int n, int cb, int sb, const double* cpre,
-                              const double* subs, const double* r, uint64_t shots, uint64_t* out,
-                              hipStream_t s) {
-  hipLaunchKernelGGL((k_sample<R, T_>), dim3(grid_for(shots)), dim3(kBlock), 0, s, (const T_*)st, n, cb, sb, cpre,
-                     subs, r, shots, out);
-}
-
 void launch_reduce_rows(const double* in, uint64_t rows, int width, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(64), 0, s, in, rows, width, out);
 }

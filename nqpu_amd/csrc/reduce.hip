@@ -1,0 +1,602 @@
+// reduce.hip — FP64 reductions that read the state once for many results.
+//
+//  * Grouped Pauli sums: terms that share their X/Y mask share one pass over
+//    the state (all diagonal Z strings of a Hamiltonian in one pass; every
+//    term with the same flip pattern in one more), with one FP64 accumulator
+//    per term.  The reference evaluates each string with its own serial loop
+//    over all 2^n indices (expectation_pauli_string, sdk/rust/src/lib.rs:465-514;
+//    expectation_hamiltonian :518).
+//  * Sampling: sub-block masses in one coalesced pass (one wavefront per
+//    sub-block), chunk prefix sums on the device, then one wavefront per shot
+//    that walks chunk -> sub-block -> amplitude with wavefront prefix scans.
+//    It returns the C ABI's outcome, the first index i with r < cumsum(|a|^2)
+//    (c_ffi.rs:506-525), without a host round trip between the passes.
+//  * Probabilities of any size computed on the device (|a|^2 in FP64).
+//
+// Every sum runs in a fixed order for a given qubit count (fixed grids,
+// fixed block reductions, fixed final sums), so results are bitwise
+// reproducible and do not depend on what else runs on the device.
+#include "runtime.h"
+#include "regtile.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace nqpu {
+
+namespace {
+
+template <typename R> struct V2r;
+template <> struct V2r<float> { using T = float2; };
+template <> struct V2r<double> { using T = double2; };
+
+template <typename T>
+__device__ __forceinline__ double nsq2(T a) {
+  return (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+}
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wavefront (Hillis-Steele).
+__device__ __forceinline__ double wscan(double v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double u = __shfl_up(v, o);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// ------------------------------------------------------------ Pauli groups
+
+constexpr int kPauliGroup = 32;  // terms evaluated in one pass
+constexpr int kPauliThreads = 256;
+
+struct PauliGroupArgs {
+  uint64_t x;    // X/Y mask shared by the group's terms (0: diagonal strings)
+  int32_t h;     // highest bit of x
+  int32_t nt;    // terms in this launch
+  uint32_t odd;  // bit t: term t has an odd number of Y factors (its sum is imaginary)
+  uint32_t pad;
+  uint64_t z[kPauliGroup];  // Z/Y mask of each term
+};
+
+// Per workgroup and term t: A_t = sum over the workgroup's indices of
+//   x == 0: (-1)^popc(k & z_t) |a_k|^2
+//   x != 0: (-1)^popc(i & z_t) w_t(i)  over pairs (i, j = i ^ x), bit h of i clear,
+//           w_t = Re(conj(a_j) a_i) (even Y count) or Im(conj(a_j) a_i) (odd)
+// <P_t> follows on the host: x == 0: A; else 2A with the i^#Y phase.
+template <typename T, int NT>
+__device__ __forceinline__ void pauli_term_acc(const T* __restrict__ st, uint64_t k, const PauliGroupArgs& g,
+                                               double (&acc)[NT]) {
+  if (g.x == 0) {
+    const double p = nsq2(st[k]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] += (__popcll(k & g.z[t]) & 1) ? -p : p;
+  } else {
+    const uint64_t i = ins0(k, g.h), j = i ^ g.x;
+    const T ai = st[i], aj = st[j];
+    const double cr = (double)aj.x * ai.x + (double)aj.y * ai.y;
+    const double ci = (double)aj.x * ai.y - (double)aj.y * ai.x;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double w = ((g.odd >> t) & 1) ? ci : cr;
+      acc[t] += (__popcll(i & g.z[t]) & 1) ? -w : w;
+    }
+  }
+}
+
+// Per workgroup and term t: A_t = sum over the workgroup's indices of
+//   x == 0: (-1)^popc(k & z_t) |a_k|^2
+//   x != 0: (-1)^popc(i & z_t) w_t(i)  over pairs (i, j = i ^ x), bit h of i clear,
+//           w_t = Re(conj(a_j) a_i) (even Y count) or Im(conj(a_j) a_i) (odd)
+// <P_t> follows on the host: x == 0: A; else 2A with the i^#Y phase.
+// Each thread keeps four loads in flight (indices one grid stride apart).
+template <typename T, int NT>
+__global__ __launch_bounds__(kPauliThreads) void k_pauli_group(const T* __restrict__ st, uint64_t work,
+                                                              PauliGroupArgs g, double* __restrict__ part) {
+  double acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * kPauliThreads;
+  uint64_t k = (uint64_t)blockIdx.x * kPauliThreads + threadIdx.x;
+  for (; k + 3 * stride < work; k += 4 * stride) {
+    pauli_term_acc<T, NT>(st, k, g, acc);
+    pauli_term_acc<T, NT>(st, k + stride, g, acc);
+    pauli_term_acc<T, NT>(st, k + 2 * stride, g, acc);
+    pauli_term_acc<T, NT>(st, k + 3 * stride, g, acc);
+  }
+  for (; k < work; k += stride) pauli_term_acc<T, NT>(st, k, g, acc);
+  __shared__ double red[NT][kPauliThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const double v = wsum(acc[t]);
+    if (lane == 0) red[t][wave] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < NT) {
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < kPauliThreads / 64; ++w) s += red[threadIdx.x][w];
+    part[(uint64_t)blockIdx.x * NT + threadIdx.x] = s;
+  }
+}
+
+// out[c] = sum over rows r of in[r * width + c] for c < cols: one workgroup
+// per column; thread t adds rows t, t + 256, ... in order, then a fixed tree.
+__global__ __launch_bounds__(256) void k_sum_columns(const double* __restrict__ in, int rows, int width, int cols,
+                                                     double* __restrict__ out) {
+  const int c = blockIdx.x;
+  if (c >= cols) return;
+  double s = 0;
+  for (int r = threadIdx.x; r < rows; r += 256) s += in[(size_t)r * width + c];
+  __shared__ double red[4];
+  s = wsum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[c] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+constexpr int kPauliMaxGrid = 4096;  // partial rows of one group pass: kPauliMaxGrid * kPauliGroup doubles
+
+int pauli_grid(uint64_t work) {
+  const uint64_t g = work / (kPauliThreads * 16ull);
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, kPauliMaxGrid));
+}
+
+template <typename T>
+void launch_pauli_group(const T* st, uint64_t work, const PauliGroupArgs& g, int grid, double* part,
+                        hipStream_t s) {
+  const int nt = g.nt;
+  const dim3 gr(grid), bl(kPauliThreads);
+  if (nt <= 1) hipLaunchKernelGGL((k_pauli_group<T, 1>), gr, bl, 0, s, st, work, g, part);
+  else if (nt <= 2) hipLaunchKernelGGL((k_pauli_group<T, 2>), gr, bl, 0, s, st, work, g, part);
+  else if (nt <= 4) hipLaunchKernelGGL((k_pauli_group<T, 4>), gr, bl, 0, s, st, work, g, part);
+  else if (nt <= 8) hipLaunchKernelGGL((k_pauli_group<T, 8>), gr, bl, 0, s, st, work, g, part);
+  else if (nt <= 16) hipLaunchKernelGGL((k_pauli_group<T, 16>), gr, bl, 0, s, st, work, g, part);
+  else hipLaunchKernelGGL((k_pauli_group<T, 32>), gr, bl, 0, s, st, work, g, part);
+}
+
+int pow2_at_least(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Launch the passes for terms (xm[t], zm[t]), sorted so that equal X masks
+// are adjacent, and leave the raw sums A_t in res[t].  part holds
+// kPauliMaxGrid * kPauliGroup doubles.
+NqpuError pauli_launch(const State* s, const uint64_t* xm, const uint64_t* zm, uint64_t count, double* part,
+                       double* res) {
+  for (uint64_t a = 0; a < count;) {
+    uint64_t b = a;
+    const uint64_t x = xm[a];
+    while (b < count && xm[b] == x && b - a < (uint64_t)kPauliGroup) ++b;
+    PauliGroupArgs g{};
+    g.x = x;
+    g.h = x ? 63 - __builtin_clzll(x) : 0;
+    g.nt = (int32_t)(b - a);
+    for (uint64_t t = a; t < b; ++t) {
+      g.z[t - a] = zm[t];
+      if (__builtin_popcountll(x & zm[t]) & 1) g.odd |= 1u << (t - a);
+    }
+    const uint64_t work = x ? (s->dim() >> 1) : s->dim();
+    const int grid = pauli_grid(work);
+    const int width = pow2_at_least(g.nt);
+    if (s->prec == NQPU_PRECISION_C64) launch_pauli_group((const float2*)s->amps, work, g, grid, part, s->stream);
+    else launch_pauli_group((const double2*)s->amps, work, g, grid, part, s->stream);
+    HIPTRY(hipGetLastError(), "pauli group launch");
+    // raw sums of this group, columns in term order of the group
+    hipLaunchKernelGGL(k_sum_columns, dim3(g.nt), dim3(256), 0, s->stream, (const double*)part, grid, width,
+                       g.nt, res + a);
+    HIPTRY(hipGetLastError(), "pauli column sums");
+    a = b;
+  }
+  return NQPU_OK;
+}
+
+// Re(i^m S) for a term whose raw sum is A (see k_pauli_group): S = 2A for
+// an even count of bits in x & z (S real), S = 2iA for an odd count (S
+// imaginary); for x == 0, S = A.  m is the power of i the term carries: the
+// number of Y factors for a plain Pauli string.
+double pauli_value(uint64_t x, uint64_t z, int m, double a) {
+  const bool imag = x != 0 && (__builtin_popcountll(x & z) & 1);
+  if (x != 0) a *= 2.0;
+  const int e = ((imag ? m + 1 : m) % 4 + 4) % 4;  // Re(i^e * |S|-part)
+  switch (e) {
+    case 0: return a;
+    case 2: return -a;
+    default: return 0.0;
+  }
+}
+
+// Term order sorted by X mask (stable), so equal masks are adjacent.
+// in_place: an insertion sort that needs no temporary buffer.
+void group_order(const uint64_t* xm, uint64_t count, uint32_t* idx, bool in_place = false) {
+  for (uint64_t t = 0; t < count; ++t) idx[t] = (uint32_t)t;
+  if (!in_place) {
+    std::stable_sort(idx, idx + count, [&](uint32_t p, uint32_t q) { return xm[p] < xm[q]; });
+    return;
+  }
+  for (uint64_t t = 1; t < count; ++t) {
+    const uint32_t v = idx[t];
+    uint64_t u = t;
+    while (u > 0 && xm[idx[u - 1]] > xm[v]) {
+      idx[u] = idx[u - 1];
+      --u;
+    }
+    idx[u] = v;
+  }
+}
+
+// ---------------------------------------------------------------- sampling
+
+constexpr int kSubBits = 10;    // amplitudes per sub-block: 2^10
+constexpr int kChunkBits = 16;  // amplitudes per chunk: 2^16
+
+// out[b] = sum |a|^2 over sub-block b (2^sb amplitudes), one wavefront per
+// sub-block: lane l adds amplitudes l, l + 64, ... (coalesced).
+template <typename T>
+__global__ __launch_bounds__(256) void k_subsum_wave(const T* __restrict__ st, uint64_t nsub, int sb,
+                                                     double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const int per = sb >= 6 ? 1 << (sb - 6) : 1;
+  for (uint64_t b = w0; b < nsub; b += nw) {
+    const T* p = st + (b << sb);
+    double s = 0;
+    if (sb >= 6) {
+      for (int r = 0; r < per; ++r) s += nsq2(p[lane + 64 * r]);
+    } else if (lane < (1 << sb)) {
+      s = nsq2(p[lane]);
+    }
+    s = wsum(s);
+    if (lane == 0) out[b] = s;
+  }
+}
+
+// pre[c + 1] = sum of chunk sums 0..c, pre[0] = 0, where chunk c is the
+// 2^(cb - sb) sub-blocks c * 2^(cb - sb) ... (one workgroup of 1024 threads;
+// each thread owns a contiguous run of chunks).
+__global__ __launch_bounds__(1024) void k_chunk_prefix(const double* __restrict__ subs, uint64_t nch, int per_chunk,
+                                                       double* __restrict__ pre) {
+  __shared__ double tot[1024];
+  const int t = threadIdx.x;
+  const uint64_t m = (nch + 1023) / 1024;
+  const uint64_t c0 = std::min<uint64_t>(nch, (uint64_t)t * m), c1 = std::min<uint64_t>(nch, c0 + m);
+  double s = 0;
+  for (uint64_t c = c0; c < c1; ++c) {
+    double cs = 0;
+    for (int b = 0; b < per_chunk; ++b) cs += subs[c * per_chunk + b];
+    s += cs;
+    pre[c + 1] = cs;  // chunk sum for now
+  }
+  tot[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    double run = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const double v = tot[i];
+      tot[i] = run;
+      run += v;
+    }
+    pre[0] = 0;
+  }
+  __syncthreads();
+  double run = tot[t];
+  for (uint64_t c = c0; c < c1; ++c) {
+    run += pre[c + 1];
+    pre[c + 1] = run;
+  }
+}
+
+// One wavefront per shot: first chunk c with r < pre[c + 1] (binary search),
+// then the first sub-block and amplitude where the running sum exceeds r.
+// r at or beyond the total mass returns dim - 1, as the C ABI's scan does.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sample_wave(const T* __restrict__ st, int n, int cb, int sb,
+                                                     const double* __restrict__ pre, const double* __restrict__ subs,
+                                                     const double* __restrict__ rr, uint64_t shots,
+                                                     uint64_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t w0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  const uint64_t dim = 1ull << n, nch = dim >> cb;
+  const int nsc = 1 << (cb - sb);                    // sub-blocks per chunk (<= 64)
+  const int per = sb >= 6 ? 1 << (sb - 6) : 1;       // amplitudes per lane in a sub-block
+  const int used = sb >= 6 ? 64 : 1 << sb;           // lanes holding amplitudes
+  for (uint64_t s = w0; s < shots; s += nw) {
+    const double r = rr[s];
+    uint64_t lo = 0, hi = nch;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (r < pre[mid + 1]) hi = mid;
+      else lo = mid + 1;
+    }
+    uint64_t res;
+    if (lo >= nch) {
+      res = dim - 1;
+    } else {
+      const double c0 = pre[lo];
+      const uint64_t sub0 = (lo << cb) >> sb;
+      const double ss = lane < nsc ? subs[sub0 + lane] : 0.0;
+      const double inc = c0 + wscan(ss, lane);
+      const uint64_t hit = __ballot(lane < nsc && r < inc);
+      if (!hit) {
+        res = ((lo + 1) << cb) - 1;  // rounding between the chunk prefix and its sub-block sums
+      } else {
+        const int kb = __builtin_ctzll(hit);
+        const double cb0 = __shfl(inc - ss, kb);  // mass before sub-block kb
+        const uint64_t b = (sub0 + kb) << sb;
+        double ps[16];
+        double ls = 0;
+        if (lane < used) {
+          for (int i = 0; i < per && i < 16; ++i) {
+            ps[i] = nsq2(st[b + (uint64_t)lane * per + i]);
+            ls += ps[i];
+          }
+        }
+        const double inc2 = cb0 + wscan(ls, lane);
+        const uint64_t hit2 = __ballot(lane < used && r < inc2);
+        if (!hit2) {
+          res = b + (1ull << sb) - 1;  // rounding between the sub-block sum and its amplitudes
+        } else {
+          const int kl = __builtin_ctzll(hit2);
+          uint64_t found = b + (uint64_t)kl * per + per - 1;
+          if (lane == kl) {
+            double cum = inc2 - ls;
+            for (int i = 0; i < per && i < 16; ++i) {
+              cum += ps[i];
+              if (r < cum) {
+                found = b + (uint64_t)lane * per + i;
+                break;
+              }
+            }
+          }
+          res = __shfl(found, kl);
+        }
+      }
+    }
+    if (lane == 0) out[s] = res;
+  }
+}
+
+template <typename T>
+__global__ void k_probs_slice(const T* __restrict__ st, uint64_t off, uint64_t count, double* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = nsq2(st[off + i]);
+}
+
+// Per workgroup: sum |in_i|^2 over a grid-stride share of count amplitudes.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sumsq_part(const T* __restrict__ in, uint64_t count, double* __restrict__ part) {
+  double s = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (uint64_t)gridDim.x * 256)
+    s += nsq2(in[i]);
+  __shared__ double red[4];
+  s = wsum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// st[off + i] = in[i] * f, converted to the state's precision.
+template <typename T>
+__global__ void k_import_scaled(T* __restrict__ st, uint64_t off, uint64_t count, const double2* __restrict__ in,
+                                double f) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const double2 v = in[i];
+    T w;
+    w.x = (decltype(w.x))(v.x * f);
+    w.y = (decltype(w.y))(v.y * f);
+    st[off + i] = w;
+  }
+}
+
+unsigned grid_of(uint64_t threads, unsigned cap) {
+  const uint64_t g = (threads + 255) / 256;
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ hosts
+
+NqpuError pauli_values(const State* s0, const std::vector<PauliTerm>& terms, std::vector<double>* vals,
+                       const std::vector<int>* phase) {
+  State* s = const_cast<State*>(s0);
+  const uint64_t count = terms.size();
+  vals->assign(count, 0.0);
+  if (!count) return NQPU_OK;
+  std::vector<uint64_t> xm(count), zm(count);
+  for (uint64_t t = 0; t < count; ++t) {
+    xm[t] = terms[t].x;
+    zm[t] = terms[t].z;
+  }
+  std::vector<uint32_t> idx(count);
+  group_order(xm.data(), count, idx.data());
+  const size_t part = (size_t)kPauliMaxGrid * kPauliGroup;
+  if (!s->scratch.ensure((part + count) * sizeof(double))) return NQPU_OUT_OF_DEVICE_MEMORY;
+  double* dpart = (double*)s->scratch.p;
+  double* dres = dpart + part;
+  std::vector<uint64_t> xs(count), zs(count);
+  for (uint64_t t = 0; t < count; ++t) {
+    xs[t] = xm[idx[t]];
+    zs[t] = zm[idx[t]];
+  }
+  NqpuError e = pauli_launch(s, xs.data(), zs.data(), count, dpart, dres);
+  if (e != NQPU_OK) return e;
+  std::vector<double> raw(count);
+  HIPTRY(hipMemcpyAsync(raw.data(), dres, count * sizeof(double), hipMemcpyDeviceToHost, s->stream),
+         "pauli results copy");
+  HIPTRY(hipStreamSynchronize(s->stream), "pauli sums");
+  for (uint64_t t = 0; t < count; ++t) {
+    const int m = phase ? (*phase)[idx[t]] : __builtin_popcountll(xs[t] & zs[t]);
+    (*vals)[idx[t]] = pauli_value(xs[t], zs[t], m, raw[t]);
+  }
+  return NQPU_OK;
+}
+
+NqpuError pauli(const State* s, uint64_t x, uint64_t z, double* out) {
+  std::vector<double> v;
+  NqpuError e = pauli_values(s, {PauliTerm{x, z, 1.0}}, &v);
+  if (e != NQPU_OK) return e;
+  *out = v[0];
+  return NQPU_OK;
+}
+
+NqpuError pauli_sum(const State* s, const uint64_t* xm, const uint64_t* zm, const double* coeffs, uint64_t count,
+                    double* out) {
+  std::vector<PauliTerm> terms(count);
+  for (uint64_t t = 0; t < count; ++t) terms[t] = PauliTerm{xm[t], zm[t], coeffs[t]};
+  std::vector<double> v;
+  NqpuError e = pauli_values(s, terms, &v);
+  if (e != NQPU_OK) return e;
+  double acc = 0;
+  for (uint64_t t = 0; t < count; ++t) acc += coeffs[t] * v[t];
+  *out = acc;
+  return NQPU_OK;
+}
+
+// ------------------------------------------------------------- sampling
+
+NqpuError sample(const State* s0, const std::vector<double>& r, uint64_t* out) {
+  State* s = const_cast<State*>(s0);
+  if (r.empty()) return NQPU_OK;
+  const int n = s->n;
+  const int cb = std::min(n, kChunkBits), sb = std::min(cb, kSubBits);
+  const uint64_t nch = s->dim() >> cb, nsub = s->dim() >> sb;
+  const uint64_t shots = r.size();
+  // scratch: [pre (nch + 1)] [subs (nsub)] [r (shots)] [out (shots)]
+  const size_t bytes = (nch + 1 + nsub + shots) * 8 + shots * 8;
+  if (!s->scratch.ensure(bytes)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  double* dpre = (double*)s->scratch.p;
+  double* dsub = dpre + (nch + 1);
+  double* dr = dsub + nsub;
+  uint64_t* dout = (uint64_t*)(dr + shots);
+  HIPTRY(hipMemcpyAsync(dr, r.data(), shots * 8, hipMemcpyHostToDevice, s->stream), "uniforms upload");
+  const unsigned gsub = grid_of(nsub * 64, 8192);
+  const unsigned gshot = grid_of(shots * 64, 4096);
+  if (s->prec == NQPU_PRECISION_C64) {
+    hipLaunchKernelGGL(k_subsum_wave<float2>, dim3(gsub), dim3(256), 0, s->stream, (const float2*)s->amps, nsub, sb,
+                       dsub);
+  } else {
+    hipLaunchKernelGGL(k_subsum_wave<double2>, dim3(gsub), dim3(256), 0, s->stream, (const double2*)s->amps, nsub,
+                       sb, dsub);
+  }
+  HIPTRY(hipGetLastError(), "sub-block sums");
+  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, s->stream, (const double*)dsub, nch, 1 << (cb - sb),
+                     dpre);
+  HIPTRY(hipGetLastError(), "chunk prefix");
+  if (s->prec == NQPU_PRECISION_C64) {
+    hipLaunchKernelGGL(k_sample_wave<float2>, dim3(gshot), dim3(256), 0, s->stream, (const float2*)s->amps, n, cb,
+                       sb, (const double*)dpre, (const double*)dsub, (const double*)dr, shots, dout);
+  } else {
+    hipLaunchKernelGGL(k_sample_wave<double2>, dim3(gshot), dim3(256), 0, s->stream, (const double2*)s->amps, n,
+                       cb, sb, (const double*)dpre, (const double*)dsub, (const double*)dr, shots, dout);
+  }
+  HIPTRY(hipGetLastError(), "sample search");
+  HIPTRY(hipMemcpyAsync(out, dout, shots * 8, hipMemcpyDeviceToHost, s->stream), "sample copy");
+  HIPTRY(hipStreamSynchronize(s->stream), "sample");
+  return NQPU_OK;
+}
+
+// ------------------------------------------------------- state mutations
+
+// initialize_from_amplitudes (sdk/rust/src/lib.rs:1918-1935): the length
+// must be 2^n and the norm at least 1e-10, else the state is left as it was
+// (NQPU_INVALID_PARAMETER); then the state becomes in / ||in||.  The norm is
+// summed in FP64 on the device from the uploaded complex128 values; a state
+// of more than 2^24 amplitudes is uploaded twice (norm, then import), so the
+// old state survives a rejected call.
+NqpuError initialize_amplitudes(State* s, const NqpuComplex* in, uint64_t len) {
+  const uint64_t dim = s->dim();
+  if (len != dim) return NQPU_INVALID_PARAMETER;
+  const uint64_t slice = std::min<uint64_t>(dim, 1ull << 24);
+  constexpr int kGrid = 512;
+  if (!s->scratch.ensure(slice * 16 + (kGrid + 8) * 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  double2* stage = (double2*)s->scratch.p;
+  double* part = (double*)((char*)s->scratch.p + slice * 16);
+  double* tot = part + kGrid;
+  HIPTRY(hipStreamSynchronize(s->stream), "sync");
+  double norm2 = 0;
+  for (uint64_t off = 0; off < dim; off += slice) {
+    const uint64_t c = std::min(slice, dim - off);
+    HIPTRY(hipMemcpyAsync(stage, in + off, c * 16, hipMemcpyHostToDevice, s->stream), "amplitude upload");
+    hipLaunchKernelGGL(k_sumsq_part<double2>, dim3(kGrid), dim3(256), 0, s->stream, (const double2*)stage, c, part);
+    hipLaunchKernelGGL(k_sum_columns, dim3(1), dim3(256), 0, s->stream, (const double*)part, kGrid, 1, 1, tot);
+    HIPTRY(hipGetLastError(), "norm of the new amplitudes");
+    double v = 0;
+    HIPTRY(hipMemcpyAsync(&v, tot, 8, hipMemcpyDeviceToHost, s->stream), "norm copy");
+    HIPTRY(hipStreamSynchronize(s->stream), "norm");
+    norm2 += v;
+  }
+  const double nrm = std::sqrt(norm2);
+  if (!(nrm >= 1e-10)) {
+    set_error("initialize_from_amplitudes: norm below 1e-10");
+    return NQPU_INVALID_PARAMETER;
+  }
+  const double inv = 1.0 / nrm;
+  for (uint64_t off = 0; off < dim; off += slice) {
+    const uint64_t c = std::min(slice, dim - off);
+    if (slice < dim)  // the single-slice case still holds the upload
+      HIPTRY(hipMemcpyAsync(stage, in + off, c * 16, hipMemcpyHostToDevice, s->stream), "amplitude upload");
+    const unsigned g = grid_of(c, 65536);
+    if (s->prec == NQPU_PRECISION_C64)
+      hipLaunchKernelGGL(k_import_scaled<float2>, dim3(g), dim3(256), 0, s->stream, (float2*)s->amps, off, c,
+                         (const double2*)stage, inv);
+    else
+      hipLaunchKernelGGL(k_import_scaled<double2>, dim3(g), dim3(256), 0, s->stream, (double2*)s->amps, off, c,
+                         (const double2*)stage, inv);
+    HIPTRY(hipGetLastError(), "import");
+    HIPTRY(hipStreamSynchronize(s->stream), "import");
+  }
+  return NQPU_OK;
+}
+
+// reset_qubit (sdk/rust/src/lib.rs:1705-1730): zero the amplitudes with bit
+// q set and divide the rest by the square root of their mass when it is
+// positive (FP64 mass; one reduction pass and one collapse pass).
+NqpuError reset_qubit(State* s, int q) {
+  double tot;
+  std::vector<double> s1;
+  NqpuError e = zsum_all(s, &tot, &s1);
+  if (e != NQPU_OK) return e;
+  const double kept = tot - s1[q];
+  const double f = kept > 0.0 ? 1.0 / std::sqrt(kept) : 1.0;
+  return collapse(s, q, 0, f);
+}
+
+// ---------------------------------------------------------- probabilities
+
+NqpuError probabilities(const State* s0, double* out) {
+  State* s = const_cast<State*>(s0);
+  const uint64_t dim = s->dim();
+  const uint64_t slice = std::min<uint64_t>(dim, 1ull << 24);
+  if (!s->scratch.ensure(slice * 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  for (uint64_t off = 0; off < dim; off += slice) {
+    const uint64_t c = std::min(slice, dim - off);
+    const unsigned g = grid_of(c, 65536);
+    if (s->prec == NQPU_PRECISION_C64)
+      hipLaunchKernelGGL(k_probs_slice<float2>, dim3(g), dim3(256), 0, s->stream, (const float2*)s->amps, off, c,
+                         (double*)s->scratch.p);
+    else
+      hipLaunchKernelGGL(k_probs_slice<double2>, dim3(g), dim3(256), 0, s->stream, (const double2*)s->amps, off, c,
+                         (double*)s->scratch.p);
+    HIPTRY(hipGetLastError(), "probs");
+    HIPTRY(hipMemcpyAsync(out + off, s->scratch.p, c * 8, hipMemcpyDeviceToHost, s->stream), "probs copy");
+    HIPTRY(hipStreamSynchronize(s->stream), "probs");
+  }
+  return NQPU_OK;
+}
+
+}  // namespace nqpu

@@ -90,8 +90,9 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
 // Kernels of the given register tile passes (EB = eb, complex64) as
 // straight-line code, compiled with hipRTC for the current device on first use
 // and cached for the life of the process by program structure (jit.hip).
+// The modules behind *fns are appended to *hold; they stay loaded while held.
 NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb,
-                             std::vector<hipFunction_t>* fns);
+                             std::vector<hipFunction_t>* fns, std::vector<std::shared_ptr<void>>* hold);
 // Register tile passes compiled so far in this process.
 uint64_t jit_compiled_passes();
 // Generate and compile (no device needed, nothing loaded or cached) the
@@ -134,6 +135,9 @@ struct State {
   hipEvent_t stage_ev = nullptr;
   bool stage_pending = false;
   PassTimer timer;
+  // run-time compiled modules the launches in flight on `stream` use; released
+  // after the stream is synchronised
+  std::vector<std::shared_ptr<void>> jit_hold;
   ~State();
   size_t amp_size() const { return prec == NQPU_PRECISION_C64 ? 8 : 16; }
   uint64_t dim() const { return 1ull << n; }
@@ -164,7 +168,23 @@ NqpuError pauli(const State* s, uint64_t x, uint64_t z, double* out);
 NqpuError scale(State* s, double f);
 NqpuError overlap(const State* a, const State* b, NqpuComplex* out);
 NqpuError sample(const State* s, const std::vector<double>& r, uint64_t* out);
+// Reductions that share passes (reduce.hip).
+struct PauliTerm;
+// <P_t> of every term (no coefficients), terms with equal X masks in one pass.
+// phase[t] (optional): the term is i^phase[t] * (X on x & ~z, Y on x & z as
+// X Z products, Z on z & ~x) evaluated as Re(i^phase[t] * sum_i (-1)^popc(i & z)
+// conj(a_{i^x}) a_i); without it, phase = the number of Y factors.
+NqpuError pauli_values(const State* s, const std::vector<PauliTerm>& terms, std::vector<double>* vals,
+                       const std::vector<int>* phase = nullptr);
+// sum_t coeffs[t] <P_t>
+NqpuError pauli_sum(const State* s, const uint64_t* xm, const uint64_t* zm, const double* coeffs, uint64_t count,
+                    double* out);
 NqpuError measure_qubit(State* s, int q, double r, uint32_t* outcome, double* prob);
+// Keep the amplitudes whose bit q equals `outcome`, multiplied by f; zero the rest.
+NqpuError collapse(State* s, int q, uint32_t outcome, double f);
+// lib.rs:1705-1730 and lib.rs:1918-1935 (reduce.hip)
+NqpuError reset_qubit(State* s, int q);
+NqpuError initialize_amplitudes(State* s, const NqpuComplex* in, uint64_t len);
 
 // Batched expectation values over many complex64 circuits (k_batch).
 struct PauliTerm {
@@ -183,7 +203,7 @@ struct Batch {
 struct BatchPlan {
   char* host = nullptr;  // from the buffer callback of plan_batch (not owned)
   size_t oc = 0, op = 0, og = 0, ot = 0, ocnt = 0, obl = 0, total = 0;  // obl: operand blobs
-  uint64_t state_amps = 0;
+  uint64_t slot_amps = 0;  // largest HBM-resident circuit: each workgroup's state slot
   size_t ncirc = 0, program_ops = 0;
   NqpuBatchStats st{};
 };

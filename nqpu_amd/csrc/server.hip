@@ -132,10 +132,10 @@ void Server::execute(State& st, Request& r) {
   NqpuError e = reset_state(&st);
   if (e == NQPU_OK) e = run_circuit(&st, r.circuit, run);
   r.expectations.assign(r.terms.size(), 0.0);
-  for (size_t k = 0; e == NQPU_OK && k < r.terms.size(); ++k) {
-    double v = 0;
-    e = pauli(&st, r.terms[k].x, r.terms[k].z, &v);
-    r.expectations[k] = r.terms[k].coeff * v;
+  if (e == NQPU_OK && !r.terms.empty()) {
+    std::vector<double> v;
+    e = pauli_values(&st, r.terms, &v);  // terms with equal X masks share a pass
+    for (size_t k = 0; e == NQPU_OK && k < r.terms.size(); ++k) r.expectations[k] = r.terms[k].coeff * v[k];
   }
   if (e == NQPU_OK && r.shots) {
     uint64_t rs = rng_init(r.seed);

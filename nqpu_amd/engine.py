@@ -360,6 +360,65 @@ class StateVector:
                                                     len(xs), ct.byref(v)), "pauli_sum")
         return v.value
 
+    def expectation_pauli_values(self, paulis) -> np.ndarray:
+        """<P> of every string (or (xmask, zmask) pair) in one grouped
+        evaluation: strings with equal X/Y masks share a pass."""
+        xs, zs = [], []
+        for p in paulis:
+            x, z = pauli_masks(p) if isinstance(p, str) else (int(p[0]), int(p[1]))
+            xs.append(x)
+            zs.append(z)
+        xa = np.array(xs or [0], dtype=np.uint64)
+        za = np.array(zs or [0], dtype=np.uint64)
+        out = np.zeros(max(1, len(xs)), dtype=np.float64)
+        L.check(L.load().nqpu_expectation_pauli_values(self.ptr, xa.ctypes.data, za.ctypes.data, len(xs),
+                                                       out.ctypes.data), "pauli_values")
+        return out[: len(xs)]
+
+    def expectation_pauli_string(self, paulis: str, order="lsb") -> float:
+        """<P> for a string of I/X/Y/Z.  order="lsb": character i acts on
+        qubit i (lib.rs:465); order="msb": character i acts on the
+        Estimator's qubit i, index bit n-1-i (primitives.rs:111-133, 683)."""
+        o = {"lsb": L.ORDER_LSB, "msb": L.ORDER_MSB}[order]
+        v = ct.c_double()
+        L.check(L.load().nqpu_expectation_pauli_string(self.ptr, paulis.encode(), o, ct.byref(v)), "pauli_string")
+        return v.value
+
+    def estimator_expectation(self, terms) -> float:
+        """Exact Estimator value of an observable (primitives.rs:1070-1083):
+        terms = [(coeff, [(qubit, 'X'|'Y'|'Z'|'I'), ...]), ...] with qubit 0
+        the most significant index bit (primitives.rs:683-728)."""
+        qubits, paulis, offsets, coeffs = [], [], [0], []
+        for c, factors in terms:
+            for q, p in factors:
+                qubits.append(int(q))
+                paulis.append(str(p))
+            offsets.append(len(qubits))
+            coeffs.append(float(c))
+        qa = np.array(qubits or [0], dtype=np.uint32)
+        oa = np.array(offsets, dtype=np.uint64)
+        ca = np.array(coeffs or [0.0], dtype=np.float64)
+        v = ct.c_double()
+        L.check(L.load().nqpu_estimator_expectation(self.ptr, qa.ctypes.data, "".join(paulis).encode(), oa.ctypes.data,
+                                                    ca.ctypes.data, len(coeffs), ct.byref(v)), "estimator")
+        return v.value
+
+    def reset_qubit(self, q: int) -> None:
+        """reset_qubit (lib.rs:1705-1730) on the device."""
+        L.check(L.load().nqpu_state_reset_qubit(self.ptr, q), "reset_qubit")
+
+    def initialize(self, amplitudes) -> bool:
+        """initialize_from_amplitudes (lib.rs:1918-1935): False (state
+        unchanged) when the length is not 2^n or the norm is below 1e-10."""
+        a = np.ascontiguousarray(amplitudes, dtype=np.complex128)
+        code = L.load().nqpu_state_initialize(self.ptr, a.ctypes.data, len(a))
+        if code == L.NQPU_OK:
+            return True
+        if code == 2:  # INVALID_PARAMETER
+            return False
+        L.check(code, "initialize")
+        return False
+
     def sample(self, shots: int, seed: int | None = None) -> np.ndarray:
         lib = L.load()
         if seed is not None:

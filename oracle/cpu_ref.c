@@ -511,6 +511,9 @@ static __thread uint64_t rng_state = 0x853c49e6748fea9bULL;
 
 void orc_rng_seed(uint64_t seed) { rng_state = seed == 0 ? 0x853c49e6748fea9bULL : seed; }
 
+/* The generator's raw 64-bit state (for tests of the state transition). */
+uint64_t orc_rng_state(void) { return rng_state; }
+
 double orc_rng_next(void) {
   uint64_t s = rng_state;
   s ^= s >> 12;
@@ -575,4 +578,73 @@ uint32_t orc_measure_qubit(orc_c* a, int n, int q, double* prob) {
   }
   if (prob) *prob = po;
   return out;
+}
+
+/* ------------------------------------------------------- state mutations */
+
+/* reset_qubit (lib.rs:1705-1730): zero every amplitude whose bit q is 1, then
+ * divide by the square root of the remaining mass when it is positive. */
+void orc_reset_qubit(orc_c* a, int n, int q) {
+  i64 dim = (i64)1 << n, mask = (i64)1 << q;
+  for (i64 i = 0; i < dim; i++)
+    if (i & mask) a[i] = C(0, 0);
+  double s = 0;
+  for (i64 i = 0; i < dim; i++) s += nsq(a[i]);
+  s = sqrt(s);
+  if (s > 0.0) {
+    double inv = 1.0 / s;
+    for (i64 i = 0; i < dim; i++) a[i] = C(a[i].re * inv, a[i].im * inv);
+  }
+}
+
+/* initialize_from_amplitudes (lib.rs:1918-1935): the length must be 2^n and
+ * the norm at least 1e-10; then dst = src / ||src||.  Returns 0 (dst
+ * untouched) when either check fails, 1 otherwise. */
+int orc_initialize_from_amplitudes(orc_c* dst, int n, const orc_c* src, uint64_t len) {
+  i64 dim = (i64)1 << n;
+  if ((i64)len != dim) return 0;
+  double s = 0;
+  for (i64 i = 0; i < dim; i++) s += nsq(src[i]);
+  s = sqrt(s);
+  if (s < 1e-10) return 0;
+  double inv = 1.0 / s;
+  for (i64 i = 0; i < dim; i++) dst[i] = C(src[i].re * inv, src[i].im * inv);
+  return 1;
+}
+
+/* primitives::pauli_expectation (primitives.rs:683-728): qubit 0 is the MOST
+ * significant bit of the index (bit = n - 1 - qubit); for each basis index k
+ * the string maps |k> to phase |k'> (X flips, Y flips with phase i on |0> and
+ * -i on |1>, Z gives -1 on |1>), and the result is
+ * coeff * Re(sum_k conj(a[k']) * phase * a[k]).  paulis[j] is one of
+ * 'I', 'X', 'Y', 'Z' acting on qubits[j]. */
+double orc_pauli_expectation_msb(const orc_c* a, int n, const int32_t* qubits, const char* paulis, int count,
+                                 double coeff) {
+  i64 dim = (i64)1 << n;
+  double rr = 0, ri = 0;
+  for (i64 k = 0; k < dim; k++) {
+    orc_c ph = C(1, 0);
+    i64 kp = k;
+    for (int j = 0; j < count; j++) {
+      int bit = n - 1 - qubits[j];
+      int bv = (int)((k >> bit) & 1);
+      switch (paulis[j]) {
+        case 'X': kp ^= (i64)1 << bit; break;
+        case 'Y':
+          kp ^= (i64)1 << bit;
+          ph = cmul(ph, bv == 0 ? C(0, 1) : C(0, -1));
+          break;
+        case 'Z':
+          if (bv) ph = cmul(ph, C(-1, 0));
+          break;
+        default: break;
+      }
+    }
+    orc_c ck = C(a[kp].re, -a[kp].im);
+    orc_c t = cmul(cmul(ck, ph), a[k]);
+    rr += t.re;
+    ri += t.im;
+  }
+  (void)ri;
+  return coeff * rr;
 }

@@ -82,11 +82,18 @@ double orc_fidelity(const orc_c* a, const orc_c* b, int n);
 /* C-ABI RNG and sampling (c_ffi.rs:48-67, 1444-1452, 506-525, 473-500) */
 void orc_rng_seed(uint64_t seed);
 double orc_rng_next(void);
+uint64_t orc_rng_state(void);
 void orc_sample_cabi(const orc_c* a, int n, uint32_t shots, uint64_t* out);
 /* lib.rs:527-550 with caller-supplied uniforms r[k] (first index with cdf >= r). */
 void orc_sample_cdf(const orc_c* a, int n, const double* r, uint64_t shots, uint64_t* out);
 /* c_ffi.rs:473-500: measure one qubit with the C-ABI RNG, collapse, renormalise. */
 uint32_t orc_measure_qubit(orc_c* a, int n, int q, double* prob);
+
+/* lib.rs:1705-1730, lib.rs:1918-1935, primitives.rs:683-728 */
+void orc_reset_qubit(orc_c* a, int n, int q);
+int orc_initialize_from_amplitudes(orc_c* dst, int n, const orc_c* src, uint64_t len);
+double orc_pauli_expectation_msb(const orc_c* a, int n, const int32_t* qubits, const char* paulis, int count,
+                                 double coeff);
 
 int orc_num_threads(void);
 void orc_set_num_threads(int t);

@@ -70,6 +70,12 @@ def lib():
         L.orc_fidelity.restype = ct.c_double
         L.orc_rng_seed.argtypes = [ct.c_uint64]
         L.orc_rng_next.restype = ct.c_double
+        L.orc_rng_state.restype = ct.c_uint64
+        L.orc_reset_qubit.argtypes = [P, ct.c_int, ct.c_int]
+        L.orc_initialize_from_amplitudes.argtypes = [P, ct.c_int, P, ct.c_uint64]
+        L.orc_initialize_from_amplitudes.restype = ct.c_int
+        L.orc_pauli_expectation_msb.argtypes = [P, ct.c_int, P, ct.c_char_p, ct.c_int, ct.c_double]
+        L.orc_pauli_expectation_msb.restype = ct.c_double
         L.orc_sample_cabi.argtypes = [P, ct.c_int, ct.c_uint32, P]
         L.orc_sample_cdf.argtypes = [P, ct.c_int, P, ct.c_uint64, P]
         L.orc_measure_qubit.argtypes = [P, ct.c_int, ct.c_int, ct.POINTER(ct.c_double)]
@@ -192,6 +198,33 @@ def rng_seed(seed: int):
 
 def rng_next() -> float:
     return lib().orc_rng_next()
+
+
+def rng_state() -> int:
+    """Raw 64-bit state of the generator."""
+    return int(lib().orc_rng_state())
+
+
+def reset_qubit(a, n, q):
+    """reset_qubit (lib.rs:1705-1730), in place."""
+    lib().orc_reset_qubit(_ptr(a), n, q)
+
+
+def initialize_from_amplitudes(n, src):
+    """initialize_from_amplitudes (lib.rs:1918-1935): the normalised copy, or
+    None when the length is not 2^n or the norm is below 1e-10."""
+    src = np.ascontiguousarray(src, dtype=np.complex128)
+    dst = np.zeros(1 << n, dtype=np.complex128)
+    ok = lib().orc_initialize_from_amplitudes(_ptr(dst), n, _ptr(src), len(src))
+    return dst if ok else None
+
+
+def pauli_expectation_msb(a, n, term, coeff=1.0):
+    """primitives::pauli_expectation (primitives.rs:683-728): ``term`` is a
+    list of (qubit, 'I'|'X'|'Y'|'Z') with qubit 0 the most significant bit."""
+    qs = np.array([int(q) for q, _ in term] or [0], dtype=np.int32)
+    ps = "".join(p.upper() for _, p in term).encode()
+    return lib().orc_pauli_expectation_msb(_ptr(a), n, _ptr(qs), ps, len(term), float(coeff))
 
 
 def sample_cabi(a, n, shots, seed=None):

@@ -20,7 +20,19 @@ def test_plan_is_deterministic_and_counts_passes():
     small = sum(1 for v in vcs if v.circuit.num_qubits <= 12)
     assert a["resident"] == small
     assert a["gate_passes"] > 0 and a["observable_passes"] > 0
-    assert a["state_bytes"] == 8.0 * sum(1 << v.circuit.num_qubits for v in vcs if v.circuit.num_qubits > 12)
+    # the plan's state bytes are one workgroup slot: the largest HBM-resident circuit
+    assert a["state_bytes"] == 8.0 * max(1 << v.circuit.num_qubits for v in vcs if v.circuit.num_qubits > 12)
+
+
+def test_circuits_without_gate_passes_still_get_one():
+    """A 14-qubit circuit whose gates all cancel (x.x) or that has none still
+    gets one (empty) gate pass: that pass writes the |0...0> start state its
+    observable passes read."""
+    empty = Circuit(14)
+    cancel = Circuit(14).x(0).x(0).rz(3, 0.4).rz(3, -0.4)
+    for c in (empty, cancel):
+        p = Batch.plan([c.native()], [[(1.0, "Z" + "I" * 13)]])
+        assert p["gate_passes"] == 1 and p["observable_passes"] == 1
 
 
 def test_error_of_one_circuit_is_reported():

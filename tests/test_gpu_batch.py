@@ -31,7 +31,9 @@ def oracle_energy(oracle, circuit, terms):
 
 
 def test_variational_energies_match_oracle(batch, oracle):
-    vcs = variational_batch(40, seed=103, nmin=8, nmax=16, lmin=4, lmax=12)
+    """The full qubit range of the batched workload (8-20)."""
+    vcs = variational_batch(40, seed=103, nmin=8, nmax=20, lmin=4, lmax=12)
+    assert max(v.circuit.num_qubits for v in vcs) >= 19
     e = batch.expectation([v.circuit for v in vcs], [v.hamiltonian() for v in vcs])
     ref = np.array([oracle_energy(oracle, v.circuit, v.hamiltonian()) for v in vcs])
     assert np.max(np.abs(e - ref)) < 2e-5
@@ -79,6 +81,20 @@ def test_quadratic_phase_passes_match_oracle(batch, oracle):
     e = batch.expectation(circs, obs)
     ref = np.array([oracle_energy(oracle, c, t) for c, t in zip(circs, obs)])
     assert np.max(np.abs(e - ref)) < 2e-5
+
+
+def test_circuits_without_gate_passes(batch):
+    """Circuits above 12 qubits with no gate left after merging (empty, or
+    gates that cancel) start from |0...0>: <Z_0> = 1 and <X_0> = 0, however
+    the workgroup's state slot was left by an earlier circuit."""
+    noisy = [random_circuit(14, 200, 3), random_circuit(13, 200, 4)]
+    empty = [Circuit(14), Circuit(13).x(0).x(0).rz(5, 0.3).rz(5, -0.3), Circuit(14).h(2).h(2)]
+    circs = noisy + empty + noisy + empty
+    obs = [[(1.0, "Z" + "I" * (c.num_qubits - 1)), (1.0, "X" + "I" * (c.num_qubits - 1))] for c in circs]
+    for _ in range(2):
+        e = batch.expectation(circs, obs)
+        for k in (2, 3, 4, 7, 8, 9):
+            assert abs(e[k] - 1.0) < 1e-6, k
 
 
 def test_batch_argument_errors(batch):

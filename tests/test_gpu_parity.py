@@ -299,3 +299,37 @@ def test_large_state_c64_vs_c128_and_fusion_modes():
         amp_d = d.amplitudes()
     assert infidelity(amp_d, amp_a) < 1e-5
     assert np.max(np.abs(zd - za)) < 1e-5
+
+
+@pytest.fixture(scope="module")
+def c2_26(oracle):
+    """The single-large-state family (U3 + CNOT brickwork, depth 40, seed
+    102) at 26 qubits, where the FP64 oracle finishes in seconds."""
+    c = random_u3_brickwork(26, 40, 102)
+    return c, oracle.run(26, c.gates)
+
+
+def _zz(oracle, a, n):
+    return np.array([oracle.expectation_pauli(a, n, 0, (1 << i) | (1 << (i + 1))) for i in range(n - 1)])
+
+
+def test_c2_family_26_qubits_complex128_matches_oracle(oracle, c2_26):
+    c, exp = c2_26
+    with StateVector(26, "c128") as sv:
+        sv.run(c)
+        out = sv.amplitudes()
+        zz = [sv.expectation_pauli(xmask=0, zmask=(1 << i) | (1 << (i + 1))) for i in range(25)]
+    assert np.max(np.abs(out - exp)) < 1e-12
+    assert infidelity(out, exp) < 1e-12
+    assert np.max(np.abs(np.array(zz) - _zz(oracle, exp, 26))) < 1e-12
+
+
+@pytest.mark.parametrize("compile_", [False, True])
+def test_c2_family_26_qubits_complex64_matches_oracle(oracle, c2_26, compile_):
+    c, exp = c2_26
+    with StateVector(26, "c64") as sv:
+        sv.run(c, compile=compile_)
+        out = sv.amplitudes()
+        zz = [sv.expectation_pauli(xmask=0, zmask=(1 << i) | (1 << (i + 1))) for i in range(25)]
+    assert infidelity(out, exp) < 1e-5
+    assert np.max(np.abs(np.array(zz) - _zz(oracle, exp, 26))) < 1e-5

@@ -105,3 +105,25 @@ def test_destroy_with_queued_requests():
     for r in reqs:
         srv.submit(r.circuit, r.terms, r.shots, r.seed)
     srv.close()  # finishes the running request, fails the queued ones, returns
+
+
+def test_26_qubit_1024_shot_requests_match_oracle(oracle):
+    """The served workload's larger requests: 26 qubits, 1024 shots and
+    8 random Pauli strings each, complex64, against the FP64 oracle
+    (expectations) and the C ABI sampler on the request's own final state."""
+    reqs = served_stream(2, 100.0, seed=26, shots=1024, qubits=(26,), weights=(1.0,), dmin=10, dmax=14)
+    with Server(slots=2, precision="c64", memory_bytes=4 * GiB) as srv:
+        ids = [srv.submit(r.circuit, r.terms, r.shots, r.seed) for r in reqs]
+        res = [srv.result(i, timeout=120) for i in ids]
+    for r, got in zip(reqs, res):
+        assert got["state"] == "done"
+        n = r.circuit.num_qubits
+        assert n == 26 and r.shots == 1024 and len(r.terms) == 8
+        a = oracle.run(n, r.circuit.gates)
+        want = [c * oracle.expectation_pauli(a, n, *E.pauli_masks(p)) for c, p in r.terms]
+        assert np.max(np.abs(np.array(want) - got["expectations"])) < 1e-5
+        with StateVector(n, "c64") as sv:
+            sv.run(r.circuit)
+            amps = sv.amplitudes()
+        assert np.array_equal(got["samples"], oracle.sample_cabi(np.ascontiguousarray(amps), n, r.shots,
+                                                                   seed=r.seed))
