@@ -277,6 +277,17 @@ NqpuError nqpu_expectation_pauli_sum(const NqpuState* state, const uint64_t* xma
 NqpuError nqpu_expectation_pauli_values(const NqpuState* state, const uint64_t* xmasks, const uint64_t* zmasks,
                                         uint64_t count, double* out);
 
+/* nqpu_expectation_pauli_sum with every device buffer in a fixed array of
+ * the library (no counterpart in the reference).  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): it calls no hipMalloc/hipFree and no
+ * allocating routine.  At most 1024 terms (more -> NQPU_INVALID_PARAMETER).
+ * Every call on one device shares that array: calls that overlap on one
+ * device race and must be serialised by the caller. */
+NqpuError nqpu_expectation_pauli_sum_noalloc(const NqpuState* state, const uint64_t* xmasks,
+                                             const uint64_t* zmasks, const double* coeffs, uint64_t count,
+                                             double* out);
+
 /* Qubit order of a Pauli string. */
 typedef enum {
   /* character i acts on qubit i = index bit i (expectation_pauli_string,

@@ -158,6 +158,7 @@ SIGNATURES = {
     "nqpu_expectation_z_all": (_E, [_P, _P, ct.c_uint32]),
     "nqpu_expectation_pauli": (_E, [_P, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_double)]),
     "nqpu_expectation_pauli_sum": (_E, [_P, _P, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),
+    "nqpu_expectation_pauli_sum_noalloc": (_E, [_P, _P, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),
     "nqpu_expectation_pauli_values": (_E, [_P, _P, _P, ct.c_uint64, _P]),
     "nqpu_expectation_pauli_string": (_E, [_P, ct.c_char_p, ct.c_uint32, ct.POINTER(ct.c_double)]),
     "nqpu_estimator_expectation": (_E, [_P, _P, ct.c_char_p, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),

@@ -342,6 +342,19 @@ NqpuError nqpu_expectation_pauli_sum(const NqpuState* st, const uint64_t* xm, co
   return pauli_sum(S(st), xm, zm, coeffs, count, out);
 }
 
+NqpuError nqpu_expectation_pauli_sum_noalloc(const NqpuState* st, const uint64_t* xm, const uint64_t* zm,
+                                             const double* coeffs, uint64_t count, double* out) {
+  if (!st) return NQPU_NULL_POINTER;
+  if (!out || (count && (!xm || !zm || !coeffs))) return NQPU_NULL_POINTER;
+  if (count > kPauliSumNoAllocMaxTerms) {
+    set_error("nqpu_expectation_pauli_sum_noalloc: at most 1024 terms");
+    return NQPU_INVALID_PARAMETER;
+  }
+  NqpuError e = check_masks(st, xm, zm, count);
+  if (e != NQPU_OK) return e;
+  return pauli_sum_noalloc(S(st), xm, zm, coeffs, count, out);
+}
+
 NqpuError nqpu_expectation_pauli_values(const NqpuState* st, const uint64_t* xm, const uint64_t* zm, uint64_t count,
                                         double* out) {
   if (!st) return NQPU_NULL_POINTER;

@@ -12,6 +12,8 @@
 //    It returns the C ABI's outcome, the first index i with r < cumsum(|a|^2)
 //    (c_ffi.rs:506-525), without a host round trip between the passes.
 //  * Probabilities of any size computed on the device (|a|^2 in FP64).
+//  * A Pauli-sum entry point that allocates nothing: its partial sums live
+//    in a fixed device array of this file.
 //
 // Every sum runs in a fixed order for a given qubit count (fixed grids,
 // fixed block reductions, fixed final sums), so results are bitwise
@@ -22,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace nqpu {
 
@@ -461,6 +464,90 @@ NqpuError pauli_sum(const State* s, const uint64_t* xm, const uint64_t* zm, cons
   std::vector<double> v;
   NqpuError e = pauli_values(s, terms, &v);
   if (e != NQPU_OK) return e;
+  double acc = 0;
+  for (uint64_t t = 0; t < count; ++t) acc += coeffs[t] * v[t];
+  *out = acc;
+  return NQPU_OK;
+}
+
+// ------------------------------------------- the no-allocation Pauli sum
+//
+// Partial sums and raw results live in `scratch`, one fixed array per device
+// (the HIP runtime gives each device its own copy, held until the process
+// ends).  Its address is looked up once per device with hipGetSymbolAddress;
+// host code only ever passes that address, never the array's name, to
+// kernels and copies.  Every word a call reads was written earlier in the same
+// call (each group pass writes all partial rows its column sums read, and the
+// column sums write every result read back), so nothing needs clearing
+// between calls.  Calls that overlap on one device share the array and race.
+// Size: kPauliMaxGrid x kPauliGroup partial sums + kPauliSumNoAllocMaxTerms
+// results = 132096 doubles (1.0 MiB).
+
+constexpr size_t kStaticPart = (size_t)kPauliMaxGrid * kPauliGroup;  // partial rows of one group pass
+constexpr size_t kStaticScratchDoubles = kStaticPart + kPauliSumNoAllocMaxTerms;
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+NqpuError scratch_ptr(int dev, double** out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index out of range for the fixed scratch array");
+    return NQPU_DEVICE_ERROR;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPTRY(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != dev) HIPTRY(hipSetDevice(dev), "hipSetDevice");
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (rc != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(rc));
+      return NQPU_DEVICE_ERROR;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NQPU_OK;
+}
+}  // namespace
+
+NqpuError scratch_init(int dev) {
+  double* p = nullptr;
+  return scratch_ptr(dev, &p);
+}
+
+NqpuError pauli_sum_noalloc(const State* s, const uint64_t* xm, const uint64_t* zm, const double* coeffs,
+                            uint64_t count, double* out) {
+  // the caller (nqpu_expectation_pauli_sum_noalloc) has checked count <= kPauliSumNoAllocMaxTerms
+  if (count == 0) {
+    *out = 0.0;
+    return NQPU_OK;
+  }
+  double* sp = nullptr;
+  NqpuError e = scratch_ptr(s->device, &sp);
+  if (e != NQPU_OK) return e;
+  // host arrays on the stack: nothing here allocates
+  uint32_t idx[kPauliSumNoAllocMaxTerms];
+  uint64_t xs[kPauliSumNoAllocMaxTerms], zs[kPauliSumNoAllocMaxTerms];
+  double raw[kPauliSumNoAllocMaxTerms];
+  group_order(xm, count, idx, true);
+  for (uint64_t t = 0; t < count; ++t) {
+    xs[t] = xm[idx[t]];
+    zs[t] = zm[idx[t]];
+  }
+  double* dpart = sp;
+  double* dres = sp + kStaticPart;
+  e = pauli_launch(s, xs, zs, count, dpart, dres);
+  if (e != NQPU_OK) return e;
+  HIPTRY(hipMemcpyAsync(raw, dres, count * sizeof(double), hipMemcpyDeviceToHost, s->stream), "pauli results copy");
+  HIPTRY(hipStreamSynchronize(s->stream), "pauli sums");
+  double v[kPauliSumNoAllocMaxTerms];
+  for (uint64_t t = 0; t < count; ++t) v[idx[t]] = pauli_value(xs[t], zs[t], __builtin_popcountll(xs[t] & zs[t]), raw[t]);
   double acc = 0;
   for (uint64_t t = 0; t < count; ++t) acc += coeffs[t] * v[t];
   *out = acc;

@@ -179,6 +179,13 @@ NqpuError pauli_values(const State* s, const std::vector<PauliTerm>& terms, std:
 // sum_t coeffs[t] <P_t>
 NqpuError pauli_sum(const State* s, const uint64_t* xm, const uint64_t* zm, const double* coeffs, uint64_t count,
                     double* out);
+// The same sum with every device buffer in a fixed device array of
+// reduce.hip (nothing is allocated); at most kPauliSumNoAllocMaxTerms terms.
+constexpr uint64_t kPauliSumNoAllocMaxTerms = 1024;
+NqpuError pauli_sum_noalloc(const State* s, const uint64_t* xm, const uint64_t* zm, const double* coeffs,
+                            uint64_t count, double* out);
+// Look up the fixed scratch array's address on device `dev` (once per device).
+NqpuError scratch_init(int dev);
 NqpuError measure_qubit(State* s, int q, double r, uint32_t* outcome, double* prob);
 // Keep the amplitudes whose bit q equals `outcome`, multiplied by f; zero the rest.
 NqpuError collapse(State* s, int q, uint32_t outcome, double f);
