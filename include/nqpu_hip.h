@@ -480,6 +480,43 @@ NqpuError nqpu_server_take(NqpuServer* server, uint64_t request_id, double* expe
                            uint64_t* samples, uint32_t shots);
 NqpuError nqpu_server_stats(const NqpuServer* server, NqpuServerStats* out);
 
+/* ---- Request formats (JSON) ----
+ * A request is either the web API's CircuitRequest / SampleRequest
+ * (sdk/rust/src/web/mod.rs:34-131, 149-160): {"num_qubits": n, "gates":
+ * [{"type": "h", "target": 0}, {"type": "cnot", "control": 0, "target": 1},
+ * ...], "backend": "auto", "shots": 1024}; or circuit_serde's
+ * SerializableCircuit (sdk/rust/src/circuits/formats/circuit_serde.rs:
+ * 329-340, 870-930): {"version": 1, "num_qubits": n, "gates": [{"gate_type":
+ * "CNOT", "targets": [1], "controls": [0], "params": []}]}.  Either may add
+ * "shots" and "observables": [{"pauli": "ZZI", "coeff": -1.0}] (character q
+ * acts on qubit q).  Malformed requests -> NQPU_INVALID_PARAMETER (or
+ * NQPU_INVALID_QUBIT for qubit numbers) with the reason in
+ * nqpu_last_error_detail(). */
+
+/* Parse a request into a new circuit (destroy with nqpu_circuit_destroy);
+ * *shots = its "shots" (0 when absent; shots may be NULL). */
+NqpuError nqpu_circuit_from_json(const char* json, NqpuCircuit** circuit, uint32_t* shots);
+/* ExecuteResponse::execute (web/mod.rs:288-375): run the request from
+ * |0...0> (precision 0 = complex128) and write {"probabilities": [...],
+ * "time_ms", "backend_used", "num_gates", "fidelity"} (fidelity of the fused
+ * run against the one-pass-per-gate run) as a NUL-terminated string.
+ * *needed = the bytes this call requires (an upper bound, known before the
+ * run); a smaller buffer -> NQPU_INVALID_PARAMETER without running. */
+NqpuError nqpu_execute_json(const char* request, uint32_t precision, char* response, uint64_t len,
+                            uint64_t* needed);
+/* SampleResponse::sample (web/mod.rs:378-419) on the request server: the
+ * request needs "shots" or "observables"; its shots come from its own
+ * xorshift64* stream seeded by `seed`. */
+NqpuError nqpu_server_submit_json(NqpuServer* server, const char* request, uint64_t seed, uint64_t* request_id);
+/* Wait (up to timeout_ms; UINT32_MAX = no limit) and write {"outcomes":
+ * {"<MSB-first bitstring>": count, ...}, "shots", "time_ms", "latency_ms",
+ * "backend_used", "expectations": [...] (when observables were given)}.
+ * *needed = the exact size; with a smaller buffer the response is kept for
+ * the next call (NQPU_INVALID_PARAMETER); otherwise the request is
+ * forgotten. */
+NqpuError nqpu_server_take_json(NqpuServer* server, uint64_t request_id, uint32_t timeout_ms, char* response,
+                                uint64_t len, uint64_t* needed);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

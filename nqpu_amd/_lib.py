@@ -188,6 +188,10 @@ SIGNATURES = {
     "nqpu_server_wait": (_E, [_P, ct.c_uint64, ct.c_uint32, ct.POINTER(NqpuRequestStatus)]),
     "nqpu_server_take": (_E, [_P, ct.c_uint64, _P, ct.c_uint32, _P, ct.c_uint32]),
     "nqpu_server_stats": (_E, [_P, ct.POINTER(NqpuServerStats)]),
+    "nqpu_circuit_from_json": (_E, [ct.c_char_p, ct.POINTER(_P), ct.POINTER(ct.c_uint32)]),
+    "nqpu_execute_json": (_E, [ct.c_char_p, ct.c_uint32, _P, ct.c_uint64, ct.POINTER(ct.c_uint64)]),
+    "nqpu_server_submit_json": (_E, [_P, ct.c_char_p, ct.c_uint64, ct.POINTER(ct.c_uint64)]),
+    "nqpu_server_take_json": (_E, [_P, ct.c_uint64, ct.c_uint32, _P, ct.c_uint64, ct.POINTER(ct.c_uint64)]),
 }
 
 _lib = None

@@ -245,5 +245,7 @@ NqpuError server_wait(Server* s, uint64_t id, uint32_t timeout_ms, NqpuRequestSt
 NqpuError server_take(Server* s, uint64_t id, double* expectations, uint32_t num_terms, uint64_t* samples,
                       uint32_t shots);
 NqpuError server_stats(const Server* s, NqpuServerStats* out);
+// qubits, term count and shots of a known request
+NqpuError server_request_shape(Server* s, uint64_t id, int* n, uint32_t* num_terms, uint32_t* shots);
 
 }  // namespace nqpu

@@ -365,6 +365,16 @@ NqpuError server_take(Server* s, uint64_t id, double* expectations, uint32_t num
   return NQPU_OK;
 }
 
+NqpuError server_request_shape(Server* s, uint64_t id, int* n, uint32_t* num_terms, uint32_t* shots) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->requests.find(id);
+  if (it == s->requests.end()) return NQPU_INVALID_HANDLE;
+  *n = it->second->circuit.n;
+  *num_terms = (uint32_t)it->second->terms.size();
+  *shots = it->second->shots;
+  return NQPU_OK;
+}
+
 NqpuError server_stats(const Server* s0, NqpuServerStats* out) {
   Server* s = const_cast<Server*>(s0);
   std::lock_guard<std::mutex> lk(s->mu);

@@ -105,6 +105,13 @@ class Circuit:
 class NativeCircuit:
     """A circuit uploaded into the library's circuit IR."""
 
+    @classmethod
+    def _adopt(cls, ptr, num_qubits: int) -> "NativeCircuit":
+        nc = cls.__new__(cls)
+        nc.ptr = ptr
+        nc.num_qubits = num_qubits
+        return nc
+
     def __init__(self, circuit: Circuit):
         lib = L.load()
         self.num_qubits = circuit.num_qubits
@@ -586,6 +593,31 @@ class Server:
         st.update(expectations=e, samples=s)
         return st
 
+    def submit_json(self, text: str, seed: int = 0) -> int:
+        """Queue a SampleRequest / SerializableCircuit request given as JSON
+        (it needs "shots" or "observables")."""
+        rid = ct.c_uint64()
+        L.check(L.load().nqpu_server_submit_json(self.ptr, text.encode(), int(seed), ct.byref(rid)),
+                "nqpu_server_submit_json")
+        return rid.value
+
+    def take_json(self, rid: int, timeout: float | None = None) -> dict:
+        """Wait for a JSON request and return its SampleResponse-shaped result."""
+        import json as _json
+        lib = L.load()
+        ms = L.WAIT_FOREVER if timeout is None else max(0, int(timeout * 1000))
+        need = ct.c_uint64()
+        size = 1 << 16
+        while True:
+            buf = ct.create_string_buffer(size)
+            code = lib.nqpu_server_take_json(self.ptr, rid, ms, buf, size, ct.byref(need))
+            if code == L.NQPU_OK:
+                return _json.loads(buf.value.decode())
+            if code == 2 and need.value > size:
+                size = int(need.value)
+                continue
+            L.check(code, "nqpu_server_take_json")
+
     def stats(self) -> dict:
         st = L.NqpuServerStats()
         L.check(L.load().nqpu_server_stats(self.ptr, ct.byref(st)), "nqpu_server_stats")
@@ -607,6 +639,34 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def circuit_from_json(text: str):
+    """Parse a request in the web API's CircuitRequest/SampleRequest form
+    (web/mod.rs:34-160) or circuit_serde's SerializableCircuit form
+    (circuit_serde.rs:329-340): (NativeCircuit, shots)."""
+    import json as _json
+    lib = L.load()
+    ptr = ct.c_void_p()
+    shots = ct.c_uint32()
+    L.check(lib.nqpu_circuit_from_json(text.encode(), ct.byref(ptr), ct.byref(shots)), "circuit_from_json")
+    n = int(_json.loads(text)["num_qubits"])
+    return NativeCircuit._adopt(ptr.value, n), int(shots.value)
+
+
+def execute_json(text: str, precision="c128") -> dict:
+    """ExecuteResponse for a CircuitRequest (web/mod.rs:288-375): the run's
+    probabilities, time, backend, gate count and fidelity against the
+    one-pass-per-gate run."""
+    import json as _json
+    lib = L.load()
+    need = ct.c_uint64()
+    code = lib.nqpu_execute_json(text.encode(), _prec(precision), None, 0, ct.byref(need))
+    if code not in (L.NQPU_OK, 2):
+        L.check(code, "execute_json")
+    buf = ct.create_string_buffer(int(need.value))
+    L.check(lib.nqpu_execute_json(text.encode(), _prec(precision), buf, len(buf), ct.byref(need)), "execute_json")
+    return _json.loads(buf.value.decode())
 
 
 def simulate(circuit: Circuit, precision="c128", **run_kw) -> StateVector:

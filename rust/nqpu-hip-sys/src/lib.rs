@@ -405,6 +405,30 @@ extern "C" {
         shots: u32,
     ) -> NqpuError;
     pub fn nqpu_server_stats(server: *const NqpuServer, out: *mut NqpuServerStats) -> NqpuError;
+
+    // request formats (web/mod.rs:34-160, circuit_serde.rs:329-340)
+    pub fn nqpu_circuit_from_json(json: *const c_char, circuit: *mut *mut NqpuCircuit, shots: *mut u32) -> NqpuError;
+    pub fn nqpu_execute_json(
+        request: *const c_char,
+        precision: u32,
+        response: *mut c_char,
+        len: u64,
+        needed: *mut u64,
+    ) -> NqpuError;
+    pub fn nqpu_server_submit_json(
+        server: *mut NqpuServer,
+        request: *const c_char,
+        seed: u64,
+        request_id: *mut u64,
+    ) -> NqpuError;
+    pub fn nqpu_server_take_json(
+        server: *mut NqpuServer,
+        request_id: u64,
+        timeout_ms: u32,
+        response: *mut c_char,
+        len: u64,
+        needed: *mut u64,
+    ) -> NqpuError;
 }
 
 // ------------------------------------------------------- owning wrappers
