@@ -49,7 +49,7 @@ def measured_traffic(args):
     kernel and workload shape."""
     if args.lds or args.tile not in (0, 12) or args.low not in (0, 5) or args.qubits != 30 or args.depth != 40:
         return None
-    key = "k_tile_reg" if args.interpret else "nqpu_pass_compiled"
+    key = "k_tile_reg" if args.interpret else "nqpu_pass_compiled"  # the tuner's choice on MI355X
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             return json.load(f)[key]["bytes_per_launch"]
@@ -432,6 +432,42 @@ def bench_serve(args):
     return result
 
 
+def bench_reductions(args):
+    """Reductions and sampling on the headline state family: 1000 shots,
+    all Z_i Z_i+1 (one grouped pass), the transverse-field Ising energy
+    (2n-1 terms), all <Z_i>, the norm; best of 5 each, at 30 and 20 qubits
+    (complex64) and 30 qubits (complex128)."""
+    from nqpu_amd import engine as E
+    from nqpu_amd.workloads import random_u3_brickwork
+
+    out = {}
+    for n, prec in ((args.qubits, "c64"), (20, "c64"), (args.qubits, "c128")):
+        c = random_u3_brickwork(n, 6, 102)
+        zz = [(0, 3 << i) for i in range(n - 1)]
+        tfim = [(-1.0, "I" * i + "ZZ" + "I" * (n - i - 2)) for i in range(n - 1)] + \
+               [(-0.7, "I" * i + "X" + "I" * (n - i - 1)) for i in range(n)]
+        with E.StateVector(n, prec) as sv:
+            sv.run(c)
+            legs = {"sample_1000": lambda: sv.sample(1000, seed=7),
+                    "zz_all_grouped": lambda: sv.expectation_pauli_values(zz),
+                    "tfim_energy": lambda: sv.expectation_pauli_sum(tfim),
+                    "z_all": lambda: sv.expectation_z_all(),
+                    "norm": lambda: sv.norm()}
+            res = {}
+            for name, fn in legs.items():
+                fn()
+                best = None
+                for _ in range(5):
+                    t = time.perf_counter()
+                    fn()
+                    dt = time.perf_counter() - t
+                    best = dt if best is None else min(best, dt)
+                res[name + "_ms"] = best * 1e3
+        out[f"{n}q_{prec}"] = res
+        log(f"reductions {n}q {prec}: " + ", ".join(f"{k} {v:.2f}" for k, v in res.items()))
+    return out
+
+
 def bench_c1(args):
     """C1: 20 qubits, 20 layers of U3 on every qubit + CNOT brickwork, seed
     101.  The GPU runs it in complex128 and complex64 (best of 5 after a
@@ -538,7 +574,7 @@ def copy_probe(n):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="random30", choices=["random30", "batch", "qft34", "serve"],
+    ap.add_argument("--workload", default="random30", choices=["random30", "batch", "qft34", "serve", "reductions"],
                     help="random30: the 30-qubit random circuit headline with every other config as extra "
                          "keys (default); batch / qft34 / serve: one of those alone")
     ap.add_argument("--qps", type=float, default=40.0, help="serve: offered requests per second")
@@ -563,6 +599,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy runs")
     ap.add_argument("--no-per-gate", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner: fixed default plan with compiled complex64 passes")
     ap.add_argument("--only-headline", action="store_true",
                     help="the 30-qubit headline without the other configs (for profiling)")
     ap.add_argument("--lds", action="store_true", help="LDS-resident tile kernel instead of the register one")
@@ -587,6 +625,9 @@ def main():
     if args.workload == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
         return
+    if args.workload == "reductions":
+        print(json.dumps(bench_reductions(args)), flush=True)
+        return
 
     from nqpu_amd import engine as E
     from nqpu_amd.workloads import random_u3_brickwork
@@ -598,8 +639,18 @@ def main():
     circ = random_u3_brickwork(n, depth, 102)
     nc = circ.native()
     gates = len(circ.gates)
-    run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds,
-                  mfma=not args.no_mfma, compile=not (args.interpret or args.lds))
+    tuned = None
+    if args.no_tune or args.lds or args.interpret or args.tile or args.low or args.no_mfma:
+        run_kw = dict(fusion="tiled", tile_qubits=args.tile, low_qubits=args.low, lds_tiles=args.lds,
+                      mfma=not args.no_mfma, compile=not (args.interpret or args.lds))
+    else:
+        # the autotuner times the plan choices on this circuit (before the timed
+        # region) and its choice becomes the default plan for 30-qubit complex64
+        t0 = time.perf_counter()
+        tuned = E.autotune(n, "c64", probe=nc)
+        tuned["tuning_s"] = time.perf_counter() - t0
+        log(f"autotuned plan: {tuned}")
+        run_kw = dict(fusion="tiled")
     log(f"workload n={n} depth={depth} gates={gates} plan={nc.plan('c64', **run_kw)}")
 
     sv = E.StateVector(n, "c64")
@@ -649,12 +700,15 @@ def main():
                    "qubits": n, "depth": depth, "gates": gates, "passes_per_step": passes,
                    "parallelism": "single GPU"},
         "hbm_frac_per_step": hbm_frac_step,
-        "passes_compiled": bool(run_kw["compile"]),
+        "passes_compiled": bool(tuned["compile"]) if tuned else bool(run_kw["compile"]),
+        "tuned_plan": tuned,
         # first execution of the circuit includes generating and compiling its pass kernels
         "first_run_s": warm_s[0] if warm_s else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                     "kernel": "k_tile" if args.lds else ("nqpu_pass_* (compiled)" if run_kw["compile"] else "k_tile_reg"),
+                     "kernel": "k_tile" if args.lds else (
+                         "nqpu_pass_* (compiled)" if (tuned["compile"] if tuned else run_kw["compile"])
+                         else "k_tile_reg"),
                      "avg_launch_ms": avg_pass_ms,
                      "bytes_per_launch": bytes_per_pass},
     }
@@ -688,7 +742,11 @@ def main():
     result["roofline"]["frac_of_copy_rate"] = achieved / result["hbm_copy"]["achieved_gbs"]
     if not args.no_cpu:
         if not args.no_check:
-            acc, cpu = accuracy_vs_oracle(args, run_kw)
+            # the accuracy run uses the headline's plan shape (tuned or fixed)
+            acc_kw = run_kw if not tuned else dict(
+                fusion="tiled", tile_qubits=tuned["tile_qubits"], low_qubits=tuned["low_qubits"],
+                compile=tuned["compile"], fixed_low=tuned["fixed_low"])
+            acc, cpu = accuracy_vs_oracle(args, acc_kw)
             result["accuracy_vs_fp64_oracle"] = acc
             result["cpu_baseline"] = cpu
         else:
@@ -708,6 +766,8 @@ def main():
         args.cpu_seconds = min(args.cpu_seconds, 8.0)
         log("C1: 20-qubit parity circuit")
         result["c1"] = bench_c1(args)
+        log("reductions and sampling")
+        result["reductions"] = bench_reductions(args)
         log("C3: batched variational circuits")
         result["c3_batch"] = bench_batch(args)
         log("C4: served request stream")

@@ -202,7 +202,8 @@ typedef enum {
 typedef struct {
   uint32_t fusion;       /* NqpuFusion; default NQPU_FUSION_TILED */
   uint32_t semantics;    /* NqpuSemantics; default NQPU_SEMANTICS_STANDARD */
-  uint32_t tile_qubits;  /* 0 = default: 12 (complex64) / 11 (complex128), one less for small states */
+  uint32_t tile_qubits;  /* 0 = default: the plan nqpu_autotune measured for this qubit count and
+                            precision, else 12 (complex64) / 11 (complex128), one less for small states */
   uint32_t low_qubits;   /* contiguous low qubits always in a tile; 0 = default */
   uint32_t flags;        /* NQPU_RUN_TIME_PASSES: time every pass with HIP events */
 } NqpuRunOptions;
@@ -229,6 +230,8 @@ typedef struct {
  * tile instead of handing the tile's low positions to the qubits the next
  * pass works on (which needs passes that only permute at the end of a run) */
 #define NQPU_RUN_FIXED_LOW 64u
+/* ignore plans measured by nqpu_autotune; use the fixed defaults */
+#define NQPU_RUN_UNTUNED 128u
 
 /* Statistics of the last run on a state. */
 typedef struct {
@@ -479,6 +482,36 @@ NqpuError nqpu_server_wait(NqpuServer* server, uint64_t request_id, uint32_t tim
 NqpuError nqpu_server_take(NqpuServer* server, uint64_t request_id, double* expectations, uint32_t num_terms,
                            uint64_t* samples, uint32_t shots);
 NqpuError nqpu_server_stats(const NqpuServer* server, NqpuServerStats* out);
+
+/* ---- Plan autotuning ----
+ * The reference's tuner targets Apple GPUs (backends/runtime/auto_tuning.rs);
+ * this one times the engine's plan choices on this device: tile qubits, the
+ * contiguous low qubits of every tile, compiled (complex64) or interpreted
+ * pass kernels, fixed or moving low positions, and tiles per workgroup (the
+ * grid).  The fastest becomes the default plan for states of that qubit
+ * count and precision in this process (runs whose options leave tile_qubits,
+ * low_qubits and those flags at zero, without NQPU_RUN_UNTUNED), and is
+ * appended to the file named by NQPU_TUNE_CACHE when set (read back by
+ * later processes).  Results of any plan agree to rounding. */
+typedef struct {
+  uint32_t tile_qubits;
+  uint32_t low_qubits;
+  uint32_t flags;            /* NQPU_RUN_COMPILE / NQPU_RUN_FIXED_LOW of the chosen plan */
+  uint32_t tiles_per_group;  /* register tile passes: tiles per workgroup */
+  double ms;                 /* the chosen plan's time for the probe circuit */
+  double default_ms;         /* the fixed default plan's time for it */
+} NqpuTunedPlan;
+
+/* Time the candidates for an n-qubit state with `probe` (NULL: a random
+ * U3 + CNOT brickwork of depth 8; else its qubit count must be n); n >= 11. */
+NqpuError nqpu_autotune(uint32_t num_qubits, uint32_t precision, const NqpuCircuit* probe, NqpuTunedPlan* out);
+/* The tuned plan for (num_qubits, precision), NQPU_INVALID_PARAMETER if none. */
+NqpuError nqpu_tuned_plan(uint32_t num_qubits, uint32_t precision, NqpuTunedPlan* out);
+/* Read plans from a file in the NQPU_TUNE_CACHE format (lines "n precision
+ * tile low flags tiles_per_group ms default_ms"). */
+NqpuError nqpu_tune_load(const char* path);
+/* Forget every tuned plan of this process. */
+void nqpu_tune_reset(void);
 
 /* ---- Request formats (JSON) ----
  * A request is either the web API's CircuitRequest / SampleRequest

@@ -77,6 +77,7 @@ RUN_NO_MFMA = 8
 RUN_COMPILE = 16
 RUN_NO_QPHASE = 32
 RUN_FIXED_LOW = 64
+RUN_UNTUNED = 128
 
 
 class NqpuFusionCounts(ct.Structure):
@@ -104,6 +105,11 @@ ORDER_MSB = 1
 
 REQUEST_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 WAIT_FOREVER = 0xFFFFFFFF
+
+class NqpuTunedPlan(ct.Structure):
+    _fields_ = [("tile_qubits", ct.c_uint32), ("low_qubits", ct.c_uint32), ("flags", ct.c_uint32),
+                ("tiles_per_group", ct.c_uint32), ("ms", ct.c_double), ("default_ms", ct.c_double)]
+
 
 # name -> (restype, argtypes)
 _P = ct.c_void_p
@@ -188,6 +194,10 @@ SIGNATURES = {
     "nqpu_server_wait": (_E, [_P, ct.c_uint64, ct.c_uint32, ct.POINTER(NqpuRequestStatus)]),
     "nqpu_server_take": (_E, [_P, ct.c_uint64, _P, ct.c_uint32, _P, ct.c_uint32]),
     "nqpu_server_stats": (_E, [_P, ct.POINTER(NqpuServerStats)]),
+    "nqpu_autotune": (_E, [ct.c_uint32, ct.c_uint32, _P, ct.POINTER(NqpuTunedPlan)]),
+    "nqpu_tuned_plan": (_E, [ct.c_uint32, ct.c_uint32, ct.POINTER(NqpuTunedPlan)]),
+    "nqpu_tune_load": (_E, [ct.c_char_p]),
+    "nqpu_tune_reset": (None, []),
     "nqpu_circuit_from_json": (_E, [ct.c_char_p, ct.POINTER(_P), ct.POINTER(ct.c_uint32)]),
     "nqpu_execute_json": (_E, [ct.c_char_p, ct.c_uint32, _P, ct.c_uint64, ct.POINTER(ct.c_uint64)]),
     "nqpu_server_submit_json": (_E, [_P, ct.c_char_p, ct.c_uint64, ct.POINTER(ct.c_uint64)]),

@@ -739,3 +739,44 @@ NqpuError nqpu_server_stats(const NqpuServer* s, NqpuServerStats* out) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+static void to_public(const TunedPlan& t, NqpuTunedPlan* out) {
+  out->tile_qubits = (uint32_t)t.tile_qubits;
+  out->low_qubits = (uint32_t)t.low_qubits;
+  out->flags = t.flags;
+  out->tiles_per_group = (uint32_t)t.tiles_per_group;
+  out->ms = t.ms;
+  out->default_ms = t.default_ms;
+}
+
+NqpuError nqpu_autotune(uint32_t n, uint32_t precision, const NqpuCircuit* probe, NqpuTunedPlan* out) {
+  if (!out) return NQPU_NULL_POINTER;
+  if (precision != NQPU_PRECISION_C64 && precision != NQPU_PRECISION_C128) return NQPU_INVALID_PARAMETER;
+  const uint32_t maxq = precision == NQPU_PRECISION_C64 ? 34 : 33;
+  if (n < 1 || n > maxq) return NQPU_INVALID_QUBIT;
+  if (probe && (uint32_t)Cc(probe)->n != n) return NQPU_INVALID_PARAMETER;
+  TunedPlan t;
+  NqpuError e = autotune((int)n, (int)precision, probe ? Cc(probe) : nullptr, &t);
+  if (e != NQPU_OK) return e;
+  to_public(t, out);
+  return NQPU_OK;
+}
+
+NqpuError nqpu_tuned_plan(uint32_t n, uint32_t precision, NqpuTunedPlan* out) {
+  if (!out) return NQPU_NULL_POINTER;
+  TunedPlan t;
+  if (!tuned_lookup((int)n, (int)precision, &t)) return NQPU_INVALID_PARAMETER;
+  to_public(t, out);
+  return NQPU_OK;
+}
+
+NqpuError nqpu_tune_load(const char* path) {
+  if (!path) return NQPU_NULL_POINTER;
+  return tune_load(path);
+}
+
+void nqpu_tune_reset(void) { tune_forget(); }
+
+}  // extern "C"

@@ -106,6 +106,8 @@ struct PassArgs {
   // positions held by the wave slots) and no R_REMAP barrier separates the
   // loads from the stores: all loads of the workgroup complete first
   int32_t sync_store;
+  // tiles each workgroup processes (grid = tiles / tiles_per_group); 0 or 1: one
+  int32_t tiles_per_group;
 };
 constexpr float kMfmaScale = 32768.0f;
 

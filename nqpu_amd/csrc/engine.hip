@@ -2282,7 +2282,7 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
 
 template <typename R>
 static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb, bool mfma, bool compile,
-                               bool qphase) {
+                               bool qphase, int tiles_per_group) {
   std::vector<RegOp<R>> prog;
   std::vector<uint16_t> blob;  // R_MFMA operand blobs of the whole run
   std::vector<PassArgs> args;
@@ -2309,6 +2309,7 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     PassArgs pa{};
     pa.K = (int)p.qubits.size();
     pa.n = s->n;
+    pa.tiles_per_group = tiles_per_group;
     pa.op0 = (int)prog.size();
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
     RegProgram<R>& pr = progs[&p - passes.data()];
@@ -2392,7 +2393,8 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
       uint64_t ntiles = 1ull << (pa.n - pa.K);
       PassArgs a = pa;
       void* kargs[] = {&st, &a, &ntiles, &pg, &bl};
-      const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 1ull << 20);
+      const uint64_t tpg = (uint64_t)std::max(1, pa.tiles_per_group);
+      const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles / tpg, 1ull << 20));
       HIPTRY(hipModuleLaunchKernel(fns[i], grid, 1, 1, 256, 1, 1, (unsigned)pa.lds_remap, s->stream, kargs, nullptr),
              "compiled register tile pass launch");
     } else {
@@ -2412,7 +2414,7 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
   const int K = std::min(po.tile_qubits, s->n);
   if (po.registers && K >= 11 && K <= 13) {
     s->stats.device_ops += ops.size();
-    return run_tiled_reg<R>(s, passes, K - 8, po.mfma, po.compile, po.qphase);
+    return run_tiled_reg<R>(s, passes, K - 8, po.mfma, po.compile, po.qphase, po.tiles_per_group);
   }
   std::vector<DevOp<R>> dops;
   std::vector<PassArgs> args;
@@ -2487,6 +2489,18 @@ PlanOptions plan_options(const NqpuRunOptions& o, int prec, int n) {
   po.compile = (o.flags & NQPU_RUN_COMPILE) != 0;
   po.qphase = (o.flags & NQPU_RUN_NO_QPHASE) == 0;
   po.move_low = po.registers && prec == NQPU_PRECISION_C64 && (o.flags & NQPU_RUN_FIXED_LOW) == 0;
+  // a plan measured by the autotuner for (n, precision) replaces the defaults
+  // of a caller that chose none of these settings
+  const uint32_t chosen = NQPU_RUN_UNTUNED | NQPU_RUN_LDS_TILES | NQPU_RUN_NO_MFMA | NQPU_RUN_FIXED_LOW |
+                          NQPU_RUN_COMPILE;
+  TunedPlan t;
+  if (n > 0 && !(o.flags & chosen) && !o.tile_qubits && !o.low_qubits && tuned_lookup(n, prec, &t)) {
+    po.tile_qubits = std::min(t.tile_qubits, kmax);
+    po.low_qubits = std::min(t.low_qubits, po.tile_qubits);
+    po.compile = (t.flags & NQPU_RUN_COMPILE) != 0;
+    po.move_low = po.registers && prec == NQPU_PRECISION_C64 && (t.flags & NQPU_RUN_FIXED_LOW) == 0;
+    po.tiles_per_group = std::max(1, t.tiles_per_group);
+  }
   return po;
 }
 
@@ -2541,6 +2555,10 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
 }
 
 NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o) {
+  return run_circuit_with(s, c, o, plan_options(o, s->prec, s->n));
+}
+
+NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, const PlanOptions& po) {
   s->stats = NqpuRunStats{};
   s->stats.gates = c.gates.size();
   const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
@@ -2572,7 +2590,6 @@ NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o) {
         }
       }
     } else {
-      PlanOptions po = plan_options(o, s->prec, s->n);
       std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
       e = run_tiled<R>(s, ops, po);
     }

@@ -860,7 +860,8 @@ void Launch<R>::tile_pass(void* st, const PassArgs& pa, const void* dops, hipStr
 template <typename R, int EB>
 static void launch_tile_reg(void* st, const PassArgs& pa, const void* prog, const void* blob, hipStream_t s) {
   const uint64_t ntiles = 1ull << (pa.n - pa.K);
-  const uint64_t g = ntiles < (1ull << 20) ? ntiles : (1ull << 20);
+  const uint64_t tpg = pa.tiles_per_group > 1 ? (uint64_t)pa.tiles_per_group : 1;
+  const uint64_t g = std::max<uint64_t>(1, std::min<uint64_t>(ntiles / tpg, 1ull << 20));
   const size_t lds = (size_t)pa.lds_remap;
   auto fn = k_tile_reg<R, T_, EB>;
   static bool attr_set = false;

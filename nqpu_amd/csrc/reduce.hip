@@ -74,32 +74,17 @@ struct PauliGroupArgs {
 //   x != 0: (-1)^popc(i & z_t) w_t(i)  over pairs (i, j = i ^ x), bit h of i clear,
 //           w_t = Re(conj(a_j) a_i) (even Y count) or Im(conj(a_j) a_i) (odd)
 // <P_t> follows on the host: x == 0: A; else 2A with the i^#Y phase.
-template <typename T, int NT>
-__device__ __forceinline__ void pauli_term_acc(const T* __restrict__ st, uint64_t k, const PauliGroupArgs& g,
-                                               double (&acc)[NT]) {
-  if (g.x == 0) {
-    const double p = nsq2(st[k]);
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] += (__popcll(k & g.z[t]) & 1) ? -p : p;
-  } else {
-    const uint64_t i = ins0(k, g.h), j = i ^ g.x;
-    const T ai = st[i], aj = st[j];
-    const double cr = (double)aj.x * ai.x + (double)aj.y * ai.y;
-    const double ci = (double)aj.x * ai.y - (double)aj.y * ai.x;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const double w = ((g.odd >> t) & 1) ? ci : cr;
-      acc[t] += (__popcll(i & g.z[t]) & 1) ? -w : w;
-    }
-  }
-}
-
 // Per workgroup and term t: A_t = sum over the workgroup's indices of
 //   x == 0: (-1)^popc(k & z_t) |a_k|^2
 //   x != 0: (-1)^popc(i & z_t) w_t(i)  over pairs (i, j = i ^ x), bit h of i clear,
 //           w_t = Re(conj(a_j) a_i) (even Y count) or Im(conj(a_j) a_i) (odd)
 // <P_t> follows on the host: x == 0: A; else 2A with the i^#Y phase.
-// Each thread keeps four loads in flight (indices one grid stride apart).
+// With i = ins0(k, h) and zc_t = z_t with bit h removed, popc(i & z_t) =
+// popc(k & zc_t).  The grid is a power of two, so k = r * stride + base with
+// base = block * 256 + thread < stride: the sign splits into a per-thread
+// part (popc(base & zc_t), applied once at the end) and a part of the
+// iteration r, uniform over the grid (scalar).  Each term then costs one FP64
+// FMA per element; four loads are kept in flight.
 template <typename T, int NT>
 __global__ __launch_bounds__(kPauliThreads) void k_pauli_group(const T* __restrict__ st, uint64_t work,
                                                               PauliGroupArgs g, double* __restrict__ part) {
@@ -107,20 +92,41 @@ __global__ __launch_bounds__(kPauliThreads) void k_pauli_group(const T* __restri
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * kPauliThreads;
-  uint64_t k = (uint64_t)blockIdx.x * kPauliThreads + threadIdx.x;
-  for (; k + 3 * stride < work; k += 4 * stride) {
-    pauli_term_acc<T, NT>(st, k, g, acc);
-    pauli_term_acc<T, NT>(st, k + stride, g, acc);
-    pauli_term_acc<T, NT>(st, k + 2 * stride, g, acc);
-    pauli_term_acc<T, NT>(st, k + 3 * stride, g, acc);
+  const uint64_t base = (uint64_t)blockIdx.x * kPauliThreads + threadIdx.x;
+  const uint64_t hmask = (1ull << g.h) - 1;
+  auto one = [&](uint64_t r0) __attribute__((always_inline)) {
+    const uint64_t k = r0 + base;
+    if (k >= work) return;
+    double w0, w1;
+    if (g.x == 0) {
+      w0 = w1 = nsq2(st[k]);
+    } else {
+      const uint64_t i = ((k & ~hmask) << 1) | (k & hmask), j = i ^ g.x;
+      const T ai = st[i], aj = st[j];
+      w0 = (double)aj.x * ai.x + (double)aj.y * ai.y;  // Re(conj(a_j) a_i)
+      w1 = (double)aj.x * ai.y - (double)aj.y * ai.x;  // Im(conj(a_j) a_i)
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double sg = (__popcll(r0 & g.z[t]) & 1) ? -1.0 : 1.0;  // uniform over the grid
+      acc[t] = fma(((g.odd >> t) & 1) ? w1 : w0, sg, acc[t]);
+    }
+  };
+  uint64_t r = 0;
+  for (; r + 3 * stride < work; r += 4 * stride) {
+    one(r);
+    one(r + stride);
+    one(r + 2 * stride);
+    one(r + 3 * stride);
   }
-  for (; k < work; k += stride) pauli_term_acc<T, NT>(st, k, g, acc);
+  for (; r < work; r += stride) one(r);
   __shared__ double red[NT][kPauliThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const double v = wsum(acc[t]);
-    if (lane == 0) red[t][wave] = v;
+    const double v = (__popcll(base & g.z[t]) & 1) ? -acc[t] : acc[t];  // the per-thread part of the sign
+    const double w = wsum(v);
+    if (lane == 0) red[t][wave] = w;
   }
   __syncthreads();
   if ((int)threadIdx.x < NT) {
@@ -148,6 +154,7 @@ __global__ __launch_bounds__(256) void k_sum_columns(const double* __restrict__ 
 
 constexpr int kPauliMaxGrid = 4096;  // partial rows of one group pass: kPauliMaxGrid * kPauliGroup doubles
 
+// A power of two (work is one), so the grid stride splits the index bits.
 int pauli_grid(uint64_t work) {
   const uint64_t g = work / (kPauliThreads * 16ull);
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, kPauliMaxGrid));
@@ -186,8 +193,12 @@ NqpuError pauli_launch(const State* s, const uint64_t* xm, const uint64_t* zm, u
     g.h = x ? 63 - __builtin_clzll(x) : 0;
     g.nt = (int32_t)(b - a);
     for (uint64_t t = a; t < b; ++t) {
-      g.z[t - a] = zm[t];
-      if (__builtin_popcountll(x & zm[t]) & 1) g.odd |= 1u << (t - a);
+      // the kernel indexes pairs by k with i = k with a zero inserted at bit
+      // h, so it takes z with bit h removed
+      const uint64_t z = zm[t];
+      const uint64_t lo = g.h ? (z & ((1ull << g.h) - 1)) : 0;
+      g.z[t - a] = x ? (lo | ((z >> (g.h + 1)) << g.h)) : z;
+      if (__builtin_popcountll(x & z) & 1) g.odd |= 1u << (t - a);
     }
     const uint64_t work = x ? (s->dim() >> 1) : s->dim();
     const int grid = pauli_grid(work);

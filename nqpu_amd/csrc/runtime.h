@@ -72,7 +72,27 @@ struct PlanOptions {
   // complex64 register passes: the physical low positions may hold any
   // qubits of the previous tile (plan_tiles)
   bool move_low = false;
+  // register tile passes: tiles per workgroup (the launch grid is the tile
+  // count divided by this)
+  int tiles_per_group = 1;
 };
+
+// A measured plan choice for states of n qubits at one precision (tune.hip).
+struct TunedPlan {
+  int tile_qubits = 0, low_qubits = 0;
+  uint32_t flags = 0;  // NQPU_RUN_COMPILE / NQPU_RUN_FIXED_LOW as measured
+  int tiles_per_group = 1;
+  double ms = 0, default_ms = 0;
+};
+bool tuned_lookup(int n, int prec, TunedPlan* out);
+// Measure the candidate plans for an n-qubit state (probe: the circuit to
+// time, or null for a random U3 + CNOT brickwork of depth 8) and make the
+// fastest the default for (n, prec).
+NqpuError autotune(int n, int prec, const Circuit* probe, TunedPlan* out);
+// Forget every tuned plan of this process (the cache file is not read again).
+void tune_forget();
+// Read tuned plans from a file in the NQPU_TUNE_CACHE format.
+NqpuError tune_load(const char* path);
 
 // lowering and planning (pure host code; unit-testable without a device)
 std::vector<HostOp> lower_gate(const Gate& g, bool reference_semantics);
@@ -158,6 +178,8 @@ NqpuError create_state(int n, int prec, State** out);
 NqpuError clone_state(const State* s, State** out);
 NqpuError reset_state(State* s);
 NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o);
+// run_circuit with explicit plan options for the tiled path
+NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, const PlanOptions& po);
 NqpuError apply_gate_now(State* s, const Gate& g);
 NqpuError get_amplitudes(const State* s, uint64_t off, uint64_t count, NqpuComplex* out);
 NqpuError set_amplitudes(State* s, uint64_t off, uint64_t count, const NqpuComplex* in);

@@ -124,11 +124,11 @@ class NativeCircuit:
         L.check(lib.nqpu_circuit_add_gates(self.ptr, ops, len(circuit.gates)), "nqpu_circuit_add_gates")
 
     def plan(self, precision="c64", fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
-             lds_tiles=False, mfma=True, compile=False, qphase=True, fixed_low=False):
+             lds_tiles=False, mfma=True, compile=False, qphase=True, fixed_low=False, untuned=False):
         lib = L.load()
         st = L.NqpuRunStats()
         o = run_options(fusion, semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma,
-                        compile=compile, qphase=qphase, fixed_low=fixed_low)
+                        compile=compile, qphase=qphase, fixed_low=fixed_low, untuned=untuned)
         L.check(lib.nqpu_circuit_plan(self.ptr, ct.byref(o), _prec(precision), ct.byref(st)), "plan")
         return _stats(st)
 
@@ -178,7 +178,7 @@ _SEMANTICS = {"standard": L.SEMANTICS_STANDARD, "reference": L.SEMANTICS_REFEREN
 
 def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
                 time_passes=False, lds_tiles=False, mfma=True, compile=False, qphase=True,
-                fixed_low=False) -> L.NqpuRunOptions:
+                fixed_low=False, untuned=False) -> L.NqpuRunOptions:
     """mfma=False keeps every gate of complex64 register tiles on the vector
     ALUs (no fused blocks on the matrix cores); compile=True runs complex64
     register tile passes as straight-line kernels compiled at run time;
@@ -192,7 +192,8 @@ def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=
     o.low_qubits = low_qubits
     o.flags = ((L.RUN_TIME_PASSES if time_passes else 0) | (L.RUN_LDS_TILES if lds_tiles else 0)
                | (0 if mfma else L.RUN_NO_MFMA) | (L.RUN_COMPILE if compile else 0)
-               | (0 if qphase else L.RUN_NO_QPHASE) | (L.RUN_FIXED_LOW if fixed_low else 0))
+               | (0 if qphase else L.RUN_NO_QPHASE) | (L.RUN_FIXED_LOW if fixed_low else 0)
+               | (L.RUN_UNTUNED if untuned else 0))
     return o
 
 
@@ -285,11 +286,11 @@ class StateVector:
     # --- execution
     def run(self, circuit: Circuit | NativeCircuit, fusion="tiled", semantics="standard",
             tile_qubits=0, low_qubits=0, time_passes=False, lds_tiles=False, mfma=True, compile=False,
-            qphase=True, fixed_low=False) -> dict:
+            qphase=True, fixed_low=False, untuned=False) -> dict:
         lib = L.load()
         nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
         o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles, mfma, compile, qphase,
-                        fixed_low)
+                        fixed_low, untuned)
         L.check(lib.nqpu_run(nc.ptr, self.ptr, ct.byref(o)), "nqpu_run")
         return self.last_stats()
 
@@ -639,6 +640,36 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def _tuned(t: L.NqpuTunedPlan) -> dict:
+    return {"tile_qubits": t.tile_qubits, "low_qubits": t.low_qubits,
+            "compile": bool(t.flags & L.RUN_COMPILE), "fixed_low": bool(t.flags & L.RUN_FIXED_LOW),
+            "tiles_per_group": t.tiles_per_group, "ms": t.ms, "default_ms": t.default_ms}
+
+
+def autotune(num_qubits: int, precision="c64", probe=None) -> dict:
+    """Time the engine's plan choices for an n-qubit state on this device
+    and make the fastest the default for (n, precision) in this process."""
+    t = L.NqpuTunedPlan()
+    nc = None if probe is None else (probe if isinstance(probe, NativeCircuit) else probe.native())
+    L.check(L.load().nqpu_autotune(num_qubits, _prec(precision), nc.ptr if nc else None, ct.byref(t)),
+            "nqpu_autotune")
+    return _tuned(t)
+
+
+def tuned_plan(num_qubits: int, precision="c64"):
+    t = L.NqpuTunedPlan()
+    code = L.load().nqpu_tuned_plan(num_qubits, _prec(precision), ct.byref(t))
+    return _tuned(t) if code == L.NQPU_OK else None
+
+
+def tune_load(path: str) -> None:
+    L.check(L.load().nqpu_tune_load(path.encode()), "nqpu_tune_load")
+
+
+def tune_reset() -> None:
+    L.load().nqpu_tune_reset()
 
 
 def circuit_from_json(text: str):
