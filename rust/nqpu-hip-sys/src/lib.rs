@@ -144,6 +144,7 @@ pub const NQPU_RUN_NO_MFMA: u32 = 8;
 pub const NQPU_RUN_COMPILE: u32 = 16;
 pub const NQPU_RUN_NO_QPHASE: u32 = 32;
 pub const NQPU_RUN_FIXED_LOW: u32 = 64;
+pub const NQPU_RUN_UNTUNED: u32 = 128;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default)]
@@ -169,6 +170,17 @@ pub struct NqpuRunStats {
     pub mfma_gates: u64,
     pub exchanges: u64,
     pub qphase_gates: u64,
+}
+
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default)]
+pub struct NqpuTunedPlan {
+    pub tile_qubits: u32,
+    pub low_qubits: u32,
+    pub flags: u32,
+    pub tiles_per_group: u32,
+    pub ms: f64,
+    pub default_ms: f64,
 }
 
 #[repr(C)]
@@ -405,6 +417,12 @@ extern "C" {
         shots: u32,
     ) -> NqpuError;
     pub fn nqpu_server_stats(server: *const NqpuServer, out: *mut NqpuServerStats) -> NqpuError;
+
+    // plan autotuning
+    pub fn nqpu_autotune(num_qubits: u32, precision: u32, probe: *const NqpuCircuit, out: *mut NqpuTunedPlan) -> NqpuError;
+    pub fn nqpu_tuned_plan(num_qubits: u32, precision: u32, out: *mut NqpuTunedPlan) -> NqpuError;
+    pub fn nqpu_tune_load(path: *const c_char) -> NqpuError;
+    pub fn nqpu_tune_reset();
 
     // request formats (web/mod.rs:34-160, circuit_serde.rs:329-340)
     pub fn nqpu_circuit_from_json(json: *const c_char, circuit: *mut *mut NqpuCircuit, shots: *mut u32) -> NqpuError;

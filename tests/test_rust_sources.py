@@ -57,7 +57,8 @@ def _rust_fields(name):
 
 @pytest.mark.parametrize("name", ["NqpuComplex", "NqpuMeasurement", "NqpuVersion", "NqpuGateOp", "NqpuRunOptions",
                                   "NqpuRunStats", "NqpuFusionCounts", "NqpuPauliTerm", "NqpuBatchStats",
-                                  "NqpuServerConfig", "NqpuRequestStatus", "NqpuServerStats"])
+                                  "NqpuServerConfig", "NqpuRequestStatus", "NqpuServerStats",
+                                  "NqpuTunedPlan"])
 def test_struct_fields_match_the_header(name):
     assert _c_fields(name) == _rust_fields(name)
 
