@@ -232,6 +232,12 @@ typedef struct {
 #define NQPU_RUN_FIXED_LOW 64u
 /* ignore plans measured by nqpu_autotune; use the fixed defaults */
 #define NQPU_RUN_UNTUNED 128u
+/* return once every pass of the run is queued on the state's stream: the
+ * host can plan the next run while this one executes.  Every later call on
+ * the state is ordered after it; nqpu_state_synchronize waits for it.  With
+ * NQPU_RUN_TIME_PASSES, pass_ms then covers all runs queued since the last
+ * synchronisation (read it after nqpu_state_synchronize). */
+#define NQPU_RUN_ASYNC 256u
 
 /* Statistics of the last run on a state. */
 typedef struct {
@@ -261,6 +267,8 @@ NqpuCircuit* nqpu_circuit_create_ex(uint32_t num_qubits);
 /* Execute with explicit fusion/semantics options (NULL = defaults). */
 NqpuError nqpu_run(const NqpuCircuit* circuit, NqpuState* state, const NqpuRunOptions* opts);
 NqpuError nqpu_last_run_stats(const NqpuState* state, NqpuRunStats* out);
+/* Wait for the runs queued on the state with NQPU_RUN_ASYNC. */
+NqpuError nqpu_state_synchronize(NqpuState* state);
 
 /* FP64 reductions on the device. */
 NqpuError nqpu_expectation_z_all(const NqpuState* state, double* out, uint32_t len);

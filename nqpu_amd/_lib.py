@@ -78,6 +78,7 @@ RUN_COMPILE = 16
 RUN_NO_QPHASE = 32
 RUN_FIXED_LOW = 64
 RUN_UNTUNED = 128
+RUN_ASYNC = 256
 
 
 class NqpuFusionCounts(ct.Structure):
@@ -161,6 +162,7 @@ SIGNATURES = {
     "nqpu_circuit_create_ex": (_P, [ct.c_uint32]),
     "nqpu_run": (_E, [_P, _P, ct.POINTER(NqpuRunOptions)]),
     "nqpu_last_run_stats": (_E, [_P, ct.POINTER(NqpuRunStats)]),
+    "nqpu_state_synchronize": (_E, [_P]),
     "nqpu_expectation_z_all": (_E, [_P, _P, ct.c_uint32]),
     "nqpu_expectation_pauli": (_E, [_P, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_double)]),
     "nqpu_expectation_pauli_sum": (_E, [_P, _P, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),

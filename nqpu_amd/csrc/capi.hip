@@ -527,6 +527,11 @@ NqpuError nqpu_run(const NqpuCircuit* c, NqpuState* st, const NqpuRunOptions* op
 
 NqpuError nqpu_circuit_execute(const NqpuCircuit* c, NqpuState* st) { return nqpu_run(c, st, nullptr); }
 
+NqpuError nqpu_state_synchronize(NqpuState* st) {
+  if (!st) return NQPU_NULL_POINTER;
+  return synchronize_state(S(st));
+}
+
 NqpuError nqpu_last_run_stats(const NqpuState* st, NqpuRunStats* out) {
   if (!st || !out) return NQPU_NULL_POINTER;
   *out = S(st)->stats;

@@ -2563,7 +2563,10 @@ NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, 
   s->stats.gates = c.gates.size();
   const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
   NqpuError e = NQPU_OK;
-  s->timer.begin((o.flags & NQPU_RUN_TIME_PASSES) != 0);
+  // runs queued without waiting (NQPU_RUN_ASYNC) keep adding to the pass
+  // timer until the state is synchronised
+  if (!s->async_open) s->timer.begin((o.flags & NQPU_RUN_TIME_PASSES) != 0);
+  else if (o.flags & NQPU_RUN_TIME_PASSES) s->timer.on = true;
   dispatch(s, [&](auto* tag) {
     using R = std::remove_pointer_t<decltype(tag)>;
     if (o.fusion == NQPU_FUSION_NONE) {
@@ -2595,9 +2598,22 @@ NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, 
     }
   });
   if (e != NQPU_OK) {
+    (void)hipStreamSynchronize(s->stream);
+    s->jit_hold.clear();
+    s->async_open = false;
     s->timer.begin(false);
     return e;
   }
+  if (o.flags & NQPU_RUN_ASYNC) {
+    // the passes are queued in stream order; the host may plan the next run
+    // while they execute (its program upload is ordered after them too)
+    s->async_open = true;
+    return NQPU_OK;
+  }
+  return synchronize_state(s);
+}
+
+NqpuError synchronize_state(State* s) {
   HIPTRY(hipStreamSynchronize(s->stream), "run");
   s->jit_hold.clear();
   if (s->timer.on) {
@@ -2605,6 +2621,7 @@ NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, 
     s->stats.timed_passes = s->timer.used / 2;
   }
   s->timer.begin(false);
+  s->async_open = false;
   return NQPU_OK;
 }
 

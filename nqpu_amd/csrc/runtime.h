@@ -158,6 +158,8 @@ struct State {
   // run-time compiled modules the launches in flight on `stream` use; released
   // after the stream is synchronised
   std::vector<std::shared_ptr<void>> jit_hold;
+  // runs were queued with NQPU_RUN_ASYNC and not yet waited for
+  bool async_open = false;
   ~State();
   size_t amp_size() const { return prec == NQPU_PRECISION_C64 ? 8 : 16; }
   uint64_t dim() const { return 1ull << n; }
@@ -178,6 +180,8 @@ NqpuError create_state(int n, int prec, State** out);
 NqpuError clone_state(const State* s, State** out);
 NqpuError reset_state(State* s);
 NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o);
+// Wait for the runs queued on the state; completes their statistics.
+NqpuError synchronize_state(State* s);
 // run_circuit with explicit plan options for the tiled path
 NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, const PlanOptions& po);
 NqpuError apply_gate_now(State* s, const Gate& g);

@@ -145,6 +145,7 @@ pub const NQPU_RUN_COMPILE: u32 = 16;
 pub const NQPU_RUN_NO_QPHASE: u32 = 32;
 pub const NQPU_RUN_FIXED_LOW: u32 = 64;
 pub const NQPU_RUN_UNTUNED: u32 = 128;
+pub const NQPU_RUN_ASYNC: u32 = 256;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default)]
@@ -303,6 +304,7 @@ extern "C" {
     pub fn nqpu_circuit_create_ex(num_qubits: u32) -> *mut NqpuCircuit;
     pub fn nqpu_run(circuit: *const NqpuCircuit, state: *mut NqpuState, opts: *const NqpuRunOptions) -> NqpuError;
     pub fn nqpu_last_run_stats(state: *const NqpuState, out: *mut NqpuRunStats) -> NqpuError;
+    pub fn nqpu_state_synchronize(state: *mut NqpuState) -> NqpuError;
     pub fn nqpu_expectation_z_all(state: *const NqpuState, out: *mut f64, len: u32) -> NqpuError;
     pub fn nqpu_expectation_pauli(state: *const NqpuState, xmask: u64, zmask: u64, out: *mut f64) -> NqpuError;
     pub fn nqpu_expectation_pauli_sum(

@@ -333,3 +333,26 @@ def test_c2_family_26_qubits_complex64_matches_oracle(oracle, c2_26, compile_):
         zz = [sv.expectation_pauli(xmask=0, zmask=(1 << i) | (1 << (i + 1))) for i in range(25)]
     assert infidelity(out, exp) < 1e-5
     assert np.max(np.abs(np.array(zz) - _zz(oracle, exp, 26))) < 1e-5
+
+
+def test_async_runs_queue_and_match_sync_runs(oracle):
+    """Runs queued with async_=True (the host plans the next while the device
+    works) give the synchronous result; reads are ordered after them and
+    pass_ms covers every queued run."""
+    c = random_u3_brickwork(18, 10, 5)
+    exp = oracle.run(18, c.gates)
+    nc = c.native()
+    with StateVector(18, "c128") as sv:
+        for _ in range(3):
+            sv.reset()
+            sv.run(nc, async_=True, time_passes=True)
+        st = sv.synchronize()
+        out = sv.amplitudes()
+        per_run = sv.run(nc, time_passes=True)["timed_passes"]
+    assert np.max(np.abs(out - exp)) < 1e-12
+    assert st["timed_passes"] == 3 * per_run > 0
+    with StateVector(18, "c64") as sv:
+        sv.run(nc, async_=True, compile=True)
+        z = sv.expectation_z_all()  # ordered after the queued run
+    zexp = np.array([oracle.expectation_z(exp, 18, q) for q in range(18)])
+    assert np.max(np.abs(z - zexp)) < 1e-5
