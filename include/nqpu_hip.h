@@ -217,7 +217,8 @@ typedef struct {
 /* complex64 register tiles: apply every gate on the vector ALUs instead of
  * fusing runs of gates on four qubits into dense blocks for the matrix cores */
 #define NQPU_RUN_NO_MFMA 8u
-/* complex64 register tiles: run each pass as its own straight-line kernel,
+/* register tiles (complex64: 12-qubit tiles; complex128: any width): run each
+ * pass as its own straight-line kernel,
  * generated from the pass's register program and compiled with hipRTC on
  * first use (about a second per eight passes, on parallel threads); the
  * kernels depend only on the program's structure, not on gate angles, and
@@ -423,6 +424,9 @@ NqpuError nqpu_debug_emulate(const NqpuCircuit* circuit, const NqpuRunOptions* o
  * NQPU_ERROR_DEVICE_ERROR with the compiler log in nqpu_last_error_detail()
  * when compilation fails. */
 NqpuError nqpu_debug_compile_passes(const NqpuCircuit* circuit, const NqpuRunOptions* opts);
+/* The same for a state of the given precision (complex128: register tiles
+ * of 11-13 qubits). */
+NqpuError nqpu_debug_compile_passes_ex(const NqpuCircuit* circuit, const NqpuRunOptions* opts, uint32_t precision);
 
 /* Detail string of the last error on the calling thread. */
 const char* nqpu_last_error_detail(void);

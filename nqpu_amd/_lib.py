@@ -183,6 +183,7 @@ SIGNATURES = {
     "nqpu_circuit_fusion_counts": (_E, [_P, ct.POINTER(NqpuFusionCounts)]),
     "nqpu_debug_emulate": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, _P, ct.c_uint64]),
     "nqpu_debug_compile_passes": (_E, [_P, ct.POINTER(NqpuRunOptions)]),
+    "nqpu_debug_compile_passes_ex": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32]),
     "nqpu_batch_create": (_P, []),
     "nqpu_batch_destroy": (None, [_P]),
     "nqpu_batch_expectation": (_E, [_P, _P, ct.c_uint64, _P, _P, ct.POINTER(NqpuRunOptions), _P]),

@@ -554,7 +554,16 @@ NqpuError nqpu_debug_compile_passes(const NqpuCircuit* c, const NqpuRunOptions* 
   NqpuRunOptions o{NQPU_FUSION_TILED, NQPU_SEMANTICS_STANDARD, 0, 0, 0};
   if (opts) o = *opts;
   if (o.semantics > NQPU_SEMANTICS_REFERENCE) return NQPU_INVALID_PARAMETER;
-  return jit_check_circuit(*Cc(c), o);
+  return jit_check_circuit(*Cc(c), o, NQPU_PRECISION_C64);
+}
+
+NqpuError nqpu_debug_compile_passes_ex(const NqpuCircuit* c, const NqpuRunOptions* opts, uint32_t precision) {
+  if (!c) return NQPU_NULL_POINTER;
+  if (precision != NQPU_PRECISION_C64 && precision != NQPU_PRECISION_C128) return NQPU_INVALID_PARAMETER;
+  NqpuRunOptions o{NQPU_FUSION_TILED, NQPU_SEMANTICS_STANDARD, 0, 0, 0};
+  if (opts) o = *opts;
+  if (o.semantics > NQPU_SEMANTICS_REFERENCE) return NQPU_INVALID_PARAMETER;
+  return jit_check_circuit(*Cc(c), o, (int)precision);
 }
 
 NqpuError nqpu_debug_emulate(const NqpuCircuit* c, const NqpuRunOptions* opts, uint32_t precision,

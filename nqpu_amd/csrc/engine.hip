@@ -2352,10 +2352,10 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
   const size_t pb = (std::max<size_t>(prog.size(), 1) * sizeof(RegOp<R>) + 255) & ~size_t(255);
   const size_t ob = pb + blob.size() * 2;
   std::vector<hipFunction_t> fns;
-  if (compile && std::is_same<R, float>::value && eb == 4) {
-    std::vector<RegOp<float>> progf(prog.size());
-    std::memcpy(progf.data(), prog.data(), prog.size() * sizeof(RegOp<R>));
-    const NqpuError e = jit_pass_functions(progf, args, eb, &fns, &s->jit_hold);
+  // compiled passes: complex64 12-qubit tiles (matrix-core blocks need EB = 4)
+  // and complex128 register tiles of any width
+  if (compile && (std::is_same<R, double>::value || eb == 4)) {
+    const NqpuError e = jit_pass_functions<R>(prog, args, eb, &fns, &s->jit_hold);
     if (e != NQPU_OK) return e;
   }
   // Upload: program and operand blobs go through the pinned stage with one
@@ -2445,28 +2445,36 @@ static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanO
   return NQPU_OK;
 }
 
-NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o) {
-  const PlanOptions po = plan_options(o, NQPU_PRECISION_C64, c.n);
+template <typename R>
+static NqpuError jit_check_circuit_t(const Circuit& c, const NqpuRunOptions& o) {
+  constexpr bool c64 = std::is_same<R, float>::value;
+  const int prec = c64 ? NQPU_PRECISION_C64 : NQPU_PRECISION_C128;
+  const PlanOptions po = plan_options(o, prec, c.n);
   const int K = std::min(po.tile_qubits, c.n);
-  if (!po.registers || K != 12) {
-    set_error("compiled passes need complex64 register tiles of 12 qubits");
+  if (!po.registers || (c64 ? K != 12 : (K < 11 || K > 13))) {
+    set_error(c64 ? "compiled passes need complex64 register tiles of 12 qubits"
+                  : "compiled passes need complex128 register tiles of 11-13 qubits");
     return NQPU_INVALID_PARAMETER;
   }
   const std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, o.semantics == NQPU_SEMANTICS_REFERENCE));
-  std::vector<RegOp<float>> prog;
+  std::vector<RegOp<R>> prog;
   std::vector<PassArgs> args;
   for (const Pass& p : plan_tiles(ops, c.n, po)) {
-    const RegProgram<float> pr = build_reg_program<float>(p, K - 8, 8, po.mfma, true, 1 << 20, po.qphase);
+    const RegProgram<R> pr = build_reg_program<R>(p, K - 8, 8, po.mfma && c64, c64, 1 << 20, po.qphase && c64);
     PassArgs pa{};
     pa.op0 = (int)prog.size();
     pa.nops = (int)pr.ops.size();
     bool has_remap = false;
-    for (const RegOp<float>& r : pr.ops) has_remap = has_remap || r.kind == R_REMAP;
-    pa.lds_remap = has_remap ? 8 << K : 0;
+    for (const RegOp<R>& r : pr.ops) has_remap = has_remap || r.kind == R_REMAP;
+    pa.lds_remap = has_remap ? (int)(2 * sizeof(R)) << K : 0;
     prog.insert(prog.end(), pr.ops.begin(), pr.ops.end());
     args.push_back(pa);
   }
-  return jit_check_compile(prog, args, K - 8);
+  return jit_check_compile<R>(prog, args, K - 8);
+}
+
+NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o, int prec) {
+  return prec == NQPU_PRECISION_C128 ? jit_check_circuit_t<double>(c, o) : jit_check_circuit_t<float>(c, o);
 }
 
 PlanOptions plan_options(const NqpuRunOptions& o, int prec, int n) {

@@ -58,10 +58,24 @@ void evict_locked() {
   }
 }
 
-void emit_op(std::ostringstream& o, const RegOp<float>& r, int k, int eb) {
+// C++ names of the amplitude types in generated source
+template <typename R> struct JitNames;
+template <> struct JitNames<float> {
+  static constexpr const char* r = "float";
+  static constexpr const char* t = "float2";
+};
+template <> struct JitNames<double> {
+  static constexpr const char* r = "double";
+  static constexpr const char* t = "double2";
+};
+
+template <typename R>
+void emit_op(std::ostringstream& o, const RegOp<R>& r, int k, int eb) {
+  const char* RT = JitNames<R>::r;
+  const char* TT = JitNames<R>::t;
   switch (r.kind) {
     case R_REMAP:
-      o << "    nqpu::reg_remap<float, float2, " << eb << ">(v, sm, ops[" << k << "], tid, true);\n";
+      o << "    nqpu::reg_remap<" << RT << ", " << TT << ", " << eb << ">(v, sm, ops[" << k << "], tid, true);\n";
       return;
     case R_MFMA:
       o << "    { int var = 0;";
@@ -73,7 +87,7 @@ void emit_op(std::ostringstream& o, const RegOp<float>& r, int k, int eb) {
       if (r.s1 < 16)
         o << "    nqpu::reg_xchg_lane<float2, " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, lane);\n";
       else
-        o << "    nqpu::reg_xchg_reg<float2, " << eb << ", " << r.s0 << ", " << (r.s1 - 16) << ">(v);\n";
+        o << "    nqpu::reg_xchg_reg<" << TT << ", " << eb << ", " << r.s0 << ", " << (r.s1 - 16) << ">(v);\n";
       return;
     case R_QPHASE: {  // chunk masks are structure: compile-time constants
       o << "    { const uint64_t qm[4] = {";
@@ -88,33 +102,35 @@ void emit_op(std::ostringstream& o, const RegOp<float>& r, int k, int eb) {
   o << "    if ((base & " << r.gc << "ull) == " << r.gc << "ull) { const bool on = (tid & " << r.ct << "u) == "
     << r.ct << "u; (void)on; ";
   switch (r.kind) {
-    case R_U1: o << "nqpu::reg_u1<float, float2, " << eb << ", " << r.s0 << ">(v, ops[" << k << "], on);"; break;
-    case R_FLIP: o << "nqpu::reg_flip<float, float2, " << eb << ", " << r.s0 << ">(v, ops[" << k << "], on);"; break;
+    case R_U1: o << "nqpu::reg_u1<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ">(v, ops[" << k << "], on);"; break;
+    case R_FLIP: o << "nqpu::reg_flip<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ">(v, ops[" << k << "], on);"; break;
     case R_D1:
-    case R_D2: o << "nqpu::reg_diag<float, float2, " << eb << ">(v, ops[" << k << "], tid, on, base);"; break;
+    case R_D2: o << "nqpu::reg_diag<" << RT << ", " << TT << ", " << eb << ">(v, ops[" << k << "], tid, on, base);"; break;
     case R_U2:
-      o << "nqpu::reg_u2<float, float2, " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, ops[" << k << "], on);";
+      o << "nqpu::reg_u2<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, ops[" << k << "], on);";
       break;
     case R_SWAP:
-      o << "nqpu::reg_swap<float, float2, " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, ops[" << k << "], on);";
+      o << "nqpu::reg_swap<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, ops[" << k << "], on);";
       break;
     default: break;
   }
   o << " }\n";
 }
 
-std::string kernel_source(int idx, const std::vector<RegOp<float>>& prog, const PassArgs& pa, int eb) {
+template <typename R>
+std::string kernel_source(int idx, const std::vector<RegOp<R>>& prog, const PassArgs& pa, int eb) {
   const int op0 = pa.op0, nops = pa.nops;
+  const char* RT = JitNames<R>::r;
+  const char* TT = JitNames<R>::t;
   std::ostringstream o;
-  o << "extern \"C\" __global__ __launch_bounds__(256, 1) void nqpu_pass_" << idx
-    << "(float2* __restrict__ st, nqpu::PassArgs pa, uint64_t ntiles,\n"
-       "    const nqpu::RegOp<float>* __restrict__ prog, const unsigned char* __restrict__ blob) {\n"
-       "  const nqpu::RegOp<float>* ops = prog + pa.op0;\n";
-  o << "  nqpu::reg_tile_pass<float, float2, "
-    << eb << ">(st, pa, ntiles, [&](float2 (&v)[" << (1 << eb)
-    << "], float2* sm, uint64_t base, int tid) __attribute__((always_inline)) {\n";
+  o << "extern \"C\" __global__ __launch_bounds__(256, 1) void nqpu_pass_" << idx << "(" << TT
+    << "* __restrict__ st, nqpu::PassArgs pa, uint64_t ntiles,\n"
+       "    const nqpu::RegOp<" << RT << ">* __restrict__ prog, const unsigned char* __restrict__ blob) {\n"
+       "  const nqpu::RegOp<" << RT << ">* ops = prog + pa.op0;\n";
+  o << "  nqpu::reg_tile_pass<" << RT << ", " << TT << ", " << eb << ">(st, pa, ntiles, [&](" << TT << " (&v)["
+    << (1 << eb) << "], " << TT << "* sm, uint64_t base, int tid) __attribute__((always_inline)) {\n";
   o << "    const int lane = tid & 63; (void)lane; (void)sm; (void)ops; (void)blob;\n";
-  for (int k = 0; k < nops; ++k) emit_op(o, prog[op0 + k], k, eb);
+  for (int k = 0; k < nops; ++k) emit_op<R>(o, prog[op0 + k], k, eb);
   o << "  });\n}\n";
   return o.str();
 }
@@ -168,7 +184,8 @@ NqpuError compile_module(const std::string& src, int nfn, JitModule* out) {
 
 constexpr int kPassesPerModule = 8;
 
-std::vector<std::string> module_sources(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args,
+template <typename R>
+std::vector<std::string> module_sources(const std::vector<RegOp<R>>& prog, const std::vector<PassArgs>& args,
                                         int eb) {
   const int np = (int)args.size(), per = kPassesPerModule;
   std::vector<std::string> srcs((np + per - 1) / per);
@@ -176,7 +193,7 @@ std::vector<std::string> module_sources(const std::vector<RegOp<float>>& prog, c
     std::ostringstream o;
     o << kJitPrelude;
     for (int i = m * per; i < std::min(np, (m + 1) * per); ++i)
-      o << kernel_source(i - m * per, prog, args[i], eb);
+      o << kernel_source<R>(i - m * per, prog, args[i], eb);
     srcs[m] = o.str();
   }
   return srcs;
@@ -199,9 +216,9 @@ void dump_sources(const std::vector<std::string>& srcs) {
 
 }  // namespace
 
-NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args,
-                            int eb) {
-  const std::vector<std::string> srcs = module_sources(prog, args, eb);
+template <typename R>
+NqpuError jit_check_compile(const std::vector<RegOp<R>>& prog, const std::vector<PassArgs>& args, int eb) {
+  const std::vector<std::string> srcs = module_sources<R>(prog, args, eb);
   dump_sources(srcs);
   for (const std::string& src : srcs) {
     std::vector<char> code;
@@ -211,12 +228,13 @@ NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::ve
   return NQPU_OK;
 }
 
-NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args,
-                             int eb, std::vector<hipFunction_t>* fns, std::vector<std::shared_ptr<void>>* hold) {
+template <typename R>
+NqpuError jit_pass_functions(const std::vector<RegOp<R>>& prog, const std::vector<PassArgs>& args, int eb,
+                             std::vector<hipFunction_t>* fns, std::vector<std::shared_ptr<void>>* hold) {
   // passes are split over a few modules compiled on parallel threads
   const int np = (int)args.size();
   const int per = kPassesPerModule;
-  const std::vector<std::string> srcs = module_sources(prog, args, eb);
+  const std::vector<std::string> srcs = module_sources<R>(prog, args, eb);
   dump_sources(srcs);
   const int nmod = (int)srcs.size();
   std::vector<std::shared_ptr<JitModule>> mods(nmod);
@@ -274,6 +292,13 @@ NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::v
   for (auto& m : mods) hold->push_back(m);
   return NQPU_OK;
 }
+
+template NqpuError jit_check_compile<float>(const std::vector<RegOp<float>>&, const std::vector<PassArgs>&, int);
+template NqpuError jit_check_compile<double>(const std::vector<RegOp<double>>&, const std::vector<PassArgs>&, int);
+template NqpuError jit_pass_functions<float>(const std::vector<RegOp<float>>&, const std::vector<PassArgs>&, int,
+                                             std::vector<hipFunction_t>*, std::vector<std::shared_ptr<void>>*);
+template NqpuError jit_pass_functions<double>(const std::vector<RegOp<double>>&, const std::vector<PassArgs>&, int,
+                                              std::vector<hipFunction_t>*, std::vector<std::shared_ptr<void>>*);
 
 uint64_t jit_compiled_passes() {
   std::lock_guard<std::mutex> lk(g_mu);

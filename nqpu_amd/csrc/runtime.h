@@ -111,14 +111,16 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
 // straight-line code, compiled with hipRTC for the current device on first use
 // and cached for the life of the process by program structure (jit.hip).
 // The modules behind *fns are appended to *hold; they stay loaded while held.
-NqpuError jit_pass_functions(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb,
+template <typename R>
+NqpuError jit_pass_functions(const std::vector<RegOp<R>>& prog, const std::vector<PassArgs>& args, int eb,
                              std::vector<hipFunction_t>* fns, std::vector<std::shared_ptr<void>>* hold);
 // Register tile passes compiled so far in this process.
 uint64_t jit_compiled_passes();
 // Generate and compile (no device needed, nothing loaded or cached) the
 // straight-line kernels nqpu_run with NQPU_RUN_COMPILE would use for c.
-NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o);
-NqpuError jit_check_compile(const std::vector<RegOp<float>>& prog, const std::vector<PassArgs>& args, int eb);
+NqpuError jit_check_circuit(const Circuit& c, const NqpuRunOptions& o, int prec = NQPU_PRECISION_C64);
+template <typename R>
+NqpuError jit_check_compile(const std::vector<RegOp<R>>& prog, const std::vector<PassArgs>& args, int eb);
 
 struct DeviceScratch {
   void* p = nullptr;

@@ -146,12 +146,13 @@ class NativeCircuit:
                 "emulate")
         return a
 
-    def check_compile(self, semantics="standard", mfma=True, qphase=True):
+    def check_compile(self, semantics="standard", mfma=True, qphase=True, precision="c64", tile_qubits=0):
         """Generate and compile (hipRTC, no device) the straight-line pass
-        kernels a complex64 run with compile=True would use; raises with the
-        compiler log on failure."""
-        o = run_options("tiled", semantics, mfma=mfma, compile=True, qphase=qphase)
-        L.check(L.load().nqpu_debug_compile_passes(self.ptr, ct.byref(o)), "compile passes")
+        kernels a run with compile=True would use; raises with the compiler
+        log on failure."""
+        o = run_options("tiled", semantics, tile_qubits=tile_qubits, mfma=mfma, compile=True, qphase=qphase,
+                        untuned=True)
+        L.check(L.load().nqpu_debug_compile_passes_ex(self.ptr, ct.byref(o), _prec(precision)), "compile passes")
 
     def fusion_counts(self):
         lib = L.load()

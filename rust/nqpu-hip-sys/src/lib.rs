@@ -390,6 +390,11 @@ extern "C" {
         len: u64,
     ) -> NqpuError;
     pub fn nqpu_debug_compile_passes(circuit: *const NqpuCircuit, opts: *const NqpuRunOptions) -> NqpuError;
+    pub fn nqpu_debug_compile_passes_ex(
+        circuit: *const NqpuCircuit,
+        opts: *const NqpuRunOptions,
+        precision: u32,
+    ) -> NqpuError;
     pub fn nqpu_last_error_detail() -> *const c_char;
     pub fn nqpu_device_count() -> u32;
     pub fn nqpu_server_create(config: *const NqpuServerConfig) -> *mut NqpuServer;

@@ -356,3 +356,19 @@ def test_async_runs_queue_and_match_sync_runs(oracle):
         z = sv.expectation_z_all()  # ordered after the queued run
     zexp = np.array([oracle.expectation_z(exp, 18, q) for q in range(18)])
     assert np.max(np.abs(z - zexp)) < 1e-5
+
+
+@pytest.mark.parametrize("tile", [11, 12])
+def test_compiled_complex128_passes_match_oracle(oracle, c1, tile):
+    """complex128 register tile passes as compiled straight-line kernels,
+    against the oracle to 1e-12 and bit for bit against the interpreting
+    kernel (the same plan and the same arithmetic)."""
+    c, exp = c1
+    with StateVector(20, "c128") as sv:
+        sv.run(c, tile_qubits=tile, compile=True)
+        out = sv.amplitudes()
+        sv.reset()
+        sv.run(c, tile_qubits=tile, compile=False)
+        interp = sv.amplitudes()
+    assert np.max(np.abs(out - exp)) < 1e-12
+    assert np.array_equal(out, interp)

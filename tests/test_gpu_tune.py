@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 # (tile, low, flags, tiles_per_group): the candidates of the tuner (tune.hip)
 C64 = [(12, 5, 0, 1), (11, 5, 0, 1), (12, 5, 16, 1), (12, 5, 16 | 64, 1), (12, 5, 16, 2), (12, 6, 16, 1),
        (13, 5, 0, 1), (10, 5, 0, 1)]
-C128 = [(11, 4, 0, 1), (10, 4, 0, 1), (11, 4, 0, 2), (11, 5, 0, 1), (12, 4, 0, 1)]
+C128 = [(11, 4, 0, 1), (10, 4, 0, 1), (11, 4, 16, 1), (12, 4, 16, 1), (11, 4, 0, 2), (11, 5, 0, 1),
+        (12, 4, 0, 1)]
 
 
 @pytest.fixture(autouse=True)

@@ -19,3 +19,11 @@ def test_reference_semantics_and_vector_only_passes_compile():
     c = tfim_trotter(13, 2, circuit=qft(13)).native()
     c.check_compile(semantics="reference")
     c.check_compile(mfma=False)
+
+
+@pytest.mark.parametrize("tile", [11, 12])
+def test_complex128_passes_compile(tile):
+    """complex128 register tile passes (LDS remaps and vector-ALU gates) as
+    straight-line kernels: every op kind, controls and diagonals outside the
+    tile."""
+    CIRCUITS["mixed16"]().native().check_compile(precision="c128", tile_qubits=tile)
