@@ -271,6 +271,39 @@ NqpuError nqpu_last_run_stats(const NqpuState* state, NqpuRunStats* out);
 /* Wait for the runs queued on the state with NQPU_RUN_ASYNC. */
 NqpuError nqpu_state_synchronize(NqpuState* state);
 
+/* Run graphs: a tiled run recorded once as a HIP graph and replayed.
+ * No counterpart in the reference, whose simulators dispatch every gate from
+ * the host (QuantumSimulator, sdk/rust/src/lib.rs:1738-2070); this serves
+ * callers that run the same circuit structure many times: variational loops,
+ * repeated Trotter steps, a service replaying a circuit template.
+ * nqpu_graph_create plans `circuit` (NQPU_FUSION_TILED) for `state` and
+ * records its passes (after a |0...0> fill with NQPU_GRAPH_FROM_ZERO) into one
+ * HIP graph on the state's stream.  The graph reads its register programs and
+ * operand blobs from a device buffer it owns.  nqpu_graph_update plans another
+ * circuit on the same qubits (new angles, for one) and copies its numbers into
+ * that buffer in stream order, after the replays queued before it; the graph
+ * is recorded again only when the new plan launches other kernels or the same
+ * kernels with other arguments.  nqpu_graph_launch replays the graph on the
+ * state's stream (NQPU_RUN_ASYNC: return once queued).  A graph belongs to
+ * its state, which must outlive it; like the state, it is not thread-safe. */
+typedef struct NqpuRunGraph NqpuRunGraph;
+#define NQPU_GRAPH_FROM_ZERO 1u /* the graph first sets the state to |0...0> */
+typedef struct {
+  uint64_t launches;    /* kernel launches one replay makes (passes, plus the |0...0> fill) */
+  uint64_t passes;      /* full-state gate passes among them */
+  uint64_t replays;     /* nqpu_graph_launch calls */
+  uint64_t updates;     /* nqpu_graph_update calls */
+  uint64_t recaptures;  /* updates that recorded the graph again */
+  uint64_t image_bytes; /* device buffer of register programs and operand blobs */
+  double plan_ms;       /* host planning time of the last create or update */
+} NqpuGraphInfo;
+NqpuError nqpu_graph_create(const NqpuCircuit* circuit, NqpuState* state, const NqpuRunOptions* opts,
+                            uint32_t graph_flags, NqpuRunGraph** out);
+NqpuError nqpu_graph_update(NqpuRunGraph* graph, const NqpuCircuit* circuit);
+NqpuError nqpu_graph_launch(NqpuRunGraph* graph, uint32_t flags);
+NqpuError nqpu_graph_info(const NqpuRunGraph* graph, NqpuGraphInfo* out);
+void nqpu_graph_destroy(NqpuRunGraph* graph);
+
 /* FP64 reductions on the device. */
 NqpuError nqpu_expectation_z_all(const NqpuState* state, double* out, uint32_t len);
 /* <P> for the Pauli string with X on xmask&~zmask, Y on xmask&zmask, Z on

@@ -112,6 +112,15 @@ class NqpuTunedPlan(ct.Structure):
                 ("tiles_per_group", ct.c_uint32), ("ms", ct.c_double), ("default_ms", ct.c_double)]
 
 
+GRAPH_FROM_ZERO = 1
+
+
+class NqpuGraphInfo(ct.Structure):
+    _fields_ = [("launches", ct.c_uint64), ("passes", ct.c_uint64), ("replays", ct.c_uint64),
+                ("updates", ct.c_uint64), ("recaptures", ct.c_uint64), ("image_bytes", ct.c_uint64),
+                ("plan_ms", ct.c_double)]
+
+
 # name -> (restype, argtypes)
 _P = ct.c_void_p
 _E = ct.c_int
@@ -163,6 +172,11 @@ SIGNATURES = {
     "nqpu_run": (_E, [_P, _P, ct.POINTER(NqpuRunOptions)]),
     "nqpu_last_run_stats": (_E, [_P, ct.POINTER(NqpuRunStats)]),
     "nqpu_state_synchronize": (_E, [_P]),
+    "nqpu_graph_create": (_E, [_P, _P, ct.POINTER(NqpuRunOptions), ct.c_uint32, ct.POINTER(_P)]),
+    "nqpu_graph_update": (_E, [_P, _P]),
+    "nqpu_graph_launch": (_E, [_P, ct.c_uint32]),
+    "nqpu_graph_info": (_E, [_P, ct.POINTER(NqpuGraphInfo)]),
+    "nqpu_graph_destroy": (None, [_P]),
     "nqpu_expectation_z_all": (_E, [_P, _P, ct.c_uint32]),
     "nqpu_expectation_pauli": (_E, [_P, ct.c_uint64, ct.c_uint64, ct.POINTER(ct.c_double)]),
     "nqpu_expectation_pauli_sum": (_E, [_P, _P, _P, _P, ct.c_uint64, ct.POINTER(ct.c_double)]),

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libnqpu_hip.so")
-SOURCES = ["kernels.hip", "engine.hip", "capi.hip", "server.hip", "jit.hip", "reduce.hip", "wire.hip", "tune.hip"]
+SOURCES = ["kernels.hip", "engine.hip", "capi.hip", "server.hip", "jit.hip", "reduce.hip", "wire.hip", "tune.hip", "graph.hip"]
 # device code that run-time compiled passes embed (jit.hip), in this order
 JIT_HEADERS = ["devtypes.h", "regtile.h"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -2280,17 +2280,18 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
   return NQPU_OK;
 }
 
+// Register tile passes of a plan: programs built on parallel host threads,
+// then laid out as one device image (programs, then each pass's matrix-core
+// operand blobs and phase tables) with one launch per pass.  Matrix-core
+// blocks, lane exchanges and quadratic phases (float tables) exist for
+// complex64 only; complex128 programs use LDS remaps and vector-ALU gates.
 template <typename R>
-static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb, bool mfma, bool compile,
-                               bool qphase, int tiles_per_group) {
+static NqpuError prepare_reg(const std::vector<Pass>& passes, int n, int eb, bool mfma, bool compile, bool qphase,
+                             int tiles_per_group, PreparedRun* out) {
   std::vector<RegOp<R>> prog;
   std::vector<uint16_t> blob;  // R_MFMA operand blobs of the whole run
   std::vector<PassArgs> args;
   std::vector<size_t> blob_at;  // byte offset of each pass's operand blobs
-  // the passes' programs are independent: built on parallel host threads.
-  // Matrix-core blocks, lane exchanges and quadratic phases (float tables)
-  // exist for complex64 only; complex128 programs use LDS remaps and
-  // vector-ALU gates.
   constexpr bool c64 = std::is_same<R, float>::value;
   std::vector<RegProgram<R>> progs(passes.size());
   {
@@ -2305,10 +2306,11 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     work();
     for (auto& x : th) x.join();
   }
+  NqpuRunStats& st = out->stats;
   for (const Pass& p : passes) {
     PassArgs pa{};
     pa.K = (int)p.qubits.size();
-    pa.n = s->n;
+    pa.n = n;
     pa.tiles_per_group = tiles_per_group;
     pa.op0 = (int)prog.size();
     for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
@@ -2322,8 +2324,8 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     blob_at.push_back(blob.size() * 2);
     bool has_remap = false;
     for (RegOp<R>& r : pr.ops) {
-      s->stats.remaps += r.kind == R_REMAP;
-      s->stats.exchanges += r.kind == R_XCHG;
+      st.remaps += r.kind == R_REMAP;
+      st.exchanges += r.kind == R_XCHG;
       has_remap = has_remap || r.kind == R_REMAP;
     }
     pa.lds_remap = has_remap ? (int)(sizeof(R) * 2) << pa.K : 0;
@@ -2341,28 +2343,131 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
       blob.resize(at + pr.qtab.size() * sizeof(R) / 2);
       std::memcpy(blob.data() + at, pr.qtab.data(), pr.qtab.size() * sizeof(R));
     }
-    s->stats.qphase_gates += pr.qphase_gates;
-    s->stats.mfma_blocks += pr.blocks.size();
-    s->stats.mfma_gates += pr.fused_ops;
+    st.qphase_gates += pr.qphase_gates;
+    st.mfma_blocks += pr.blocks.size();
+    st.mfma_gates += pr.fused_ops;
     pa.scaled = pr.blocks.empty() ? 0 : 1;
     pa.nops = (int)pr.ops.size();
     prog.insert(prog.end(), pr.ops.begin(), pr.ops.end());
     args.push_back(pa);
   }
   const size_t pb = (std::max<size_t>(prog.size(), 1) * sizeof(RegOp<R>) + 255) & ~size_t(255);
-  const size_t ob = pb + blob.size() * 2;
   std::vector<hipFunction_t> fns;
   // compiled passes: complex64 12-qubit tiles (matrix-core blocks need EB = 4)
   // and complex128 register tiles of any width
   if (compile && (std::is_same<R, double>::value || eb == 4)) {
-    const NqpuError e = jit_pass_functions<R>(prog, args, eb, &fns, &s->jit_hold);
+    const NqpuError e = jit_pass_functions<R>(prog, args, eb, &fns, &out->hold);
     if (e != NQPU_OK) return e;
   }
-  // Upload: program and operand blobs go through the pinned stage with one
-  // asynchronous copy in stream order (the previous run's kernels have read
-  // the device buffer before it lands); the stage is reused once the
-  // previous copy out of it is done.  Growing either buffer waits for the
-  // device first.
+  out->prog_bytes = pb;
+  out->image.assign(pb + blob.size() * 2, 0);
+  if (!prog.empty()) std::memcpy(out->image.data(), prog.data(), prog.size() * sizeof(RegOp<R>));
+  if (!blob.empty()) std::memcpy(out->image.data() + pb, blob.data(), blob.size() * 2);
+  for (size_t i = 0; i < args.size(); ++i) {
+    RunLaunch l;
+    l.kind = fns.empty() ? RunLaunch::REG : RunLaunch::COMPILED;
+    l.pa = args[i];
+    l.eb = eb;
+    l.blob_at = pb + blob_at[i];
+    l.fn = fns.empty() ? nullptr : fns[i];
+    out->launches.push_back(l);
+  }
+  return NQPU_OK;
+}
+
+// LDS tile passes of a plan (tiles outside 11..13 qubits, or NQPU_RUN_LDS_TILES).
+template <typename R>
+static void prepare_lds(const std::vector<Pass>& passes, int n, PreparedRun* out) {
+  std::vector<DevOp<R>> dops;
+  for (const Pass& p : passes) {
+    RunLaunch l;
+    l.kind = RunLaunch::LDS;
+    l.pa.K = (int)p.qubits.size();
+    l.pa.n = n;
+    l.pa.op0 = (int)dops.size();
+    l.pa.nops = (int)p.ops.size();
+    for (int j = 0; j < l.pa.K; ++j) l.pa.q[j] = p.qubits[j];
+    append_dev_ops<R>(p, dops);
+    out->launches.push_back(l);
+  }
+  out->prog_bytes = std::max<size_t>(dops.size(), 1) * sizeof(DevOp<R>);
+  out->image.assign(out->prog_bytes, 0);
+  if (!dops.empty()) std::memcpy(out->image.data(), dops.data(), dops.size() * sizeof(DevOp<R>));
+  out->stats.device_ops += dops.size();
+}
+
+template <typename R>
+static NqpuError prepare_tiled_ops(const std::vector<HostOp>& ops, int n, const PlanOptions& po, PreparedRun* out) {
+  *out = PreparedRun{};
+  out->prec = std::is_same<R, float>::value ? NQPU_PRECISION_C64 : NQPU_PRECISION_C128;
+  out->n = n;
+  std::vector<Pass> passes = plan_tiles(ops, n, po);
+  const int K = std::min(po.tile_qubits, n);
+  NqpuError e = NQPU_OK;
+  if (po.registers && K >= 11 && K <= 13) {
+    out->stats.device_ops += ops.size();
+    e = prepare_reg<R>(passes, n, K - 8, po.mfma, po.compile, po.qphase, po.tiles_per_group, out);
+  } else {
+    prepare_lds<R>(passes, n, out);
+  }
+  out->stats.passes += passes.size();
+  out->stats.bytes_moved += (double)passes.size() * 2.0 * (double)(sizeof(R) * 2 << n);
+  return e;
+}
+
+NqpuError prepare_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, int n, const PlanOptions& po,
+                        PreparedRun* out) {
+  if (o.fusion != NQPU_FUSION_TILED) {
+    set_error("only tiled runs (NQPU_FUSION_TILED) are prepared ahead of their launch");
+    return NQPU_INVALID_PARAMETER;
+  }
+  const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
+  std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
+  NqpuError e = prec == NQPU_PRECISION_C128 ? prepare_tiled_ops<double>(ops, n, po, out)
+                                            : prepare_tiled_ops<float>(ops, n, po, out);
+  out->stats.gates = c.gates.size();
+  return e;
+}
+
+template <typename R>
+static NqpuError launch_prepared_t(const PreparedRun& r, void* amps, const void* image, hipStream_t st,
+                                   PassTimer* timer) {
+  for (const RunLaunch& l : r.launches) {
+    if (timer) timer->mark(st);
+    if (l.kind == RunLaunch::COMPILED) {
+      void* a0 = amps;
+      const void* pg = image;
+      const void* bl = (const char*)image + l.blob_at;
+      uint64_t ntiles = 1ull << (l.pa.n - l.pa.K);
+      PassArgs a = l.pa;
+      void* kargs[] = {&a0, &a, &ntiles, &pg, &bl};
+      const uint64_t tpg = (uint64_t)std::max(1, l.pa.tiles_per_group);
+      const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles / tpg, 1ull << 20));
+      HIPTRY(hipModuleLaunchKernel(l.fn, grid, 1, 1, 256, 1, 1, (unsigned)l.pa.lds_remap, st, kargs, nullptr),
+             "compiled register tile pass launch");
+    } else if (l.kind == RunLaunch::REG) {
+      Launch<R>::tile_pass_reg(amps, l.pa, l.eb, image, (const char*)image + l.blob_at, st);
+    } else {
+      Launch<R>::tile_pass(amps, l.pa, image, st);
+    }
+    if (timer) timer->mark(st);
+    HIPTRY(hipGetLastError(), "tile pass launch");
+  }
+  return NQPU_OK;
+}
+
+NqpuError launch_prepared(const PreparedRun& r, void* amps, const void* image, hipStream_t st, PassTimer* timer) {
+  return r.prec == NQPU_PRECISION_C128 ? launch_prepared_t<double>(r, amps, image, st, timer)
+                                       : launch_prepared_t<float>(r, amps, image, st, timer);
+}
+
+// Upload a prepared run's image into the state's program buffer: through the
+// pinned stage with one asynchronous copy in stream order (the previous run's
+// kernels have read the device buffer before it lands); the stage is reused
+// once the previous copy out of it is done.  Growing either buffer waits for
+// the device first.
+static NqpuError upload_image(State* s, const std::vector<char>& image) {
+  const size_t ob = image.size();
   if (s->stage_pending) {
     HIPTRY(hipEventSynchronize(s->stage_ev), "wait for the previous program upload");
     s->stage_pending = false;
@@ -2378,70 +2483,34 @@ static NqpuError run_tiled_reg(State* s, const std::vector<Pass>& passes, int eb
     s->stage_bytes = nb;
   }
   if (!s->stage_ev) HIPTRY(hipEventCreateWithFlags(&s->stage_ev, hipEventDisableTiming), "stage event");
-  if (!prog.empty()) std::memcpy(s->stage, prog.data(), prog.size() * sizeof(RegOp<R>));
-  if (!blob.empty()) std::memcpy((char*)s->stage + pb, blob.data(), blob.size() * 2);
+  std::memcpy(s->stage, image.data(), ob);
   HIPTRY(hipMemcpyAsync(s->opbuf.p, s->stage, ob, hipMemcpyHostToDevice, s->stream), "program upload");
   HIPTRY(hipEventRecord(s->stage_ev, s->stream), "stage event record");
   s->stage_pending = true;
-  for (size_t i = 0; i < args.size(); ++i) {
-    const PassArgs& pa = args[i];
-    s->timer.mark(s->stream);
-    if (!fns.empty()) {
-      void* st = s->amps;
-      const void* pg = s->opbuf.p;
-      const void* bl = (char*)s->opbuf.p + pb + blob_at[i];
-      uint64_t ntiles = 1ull << (pa.n - pa.K);
-      PassArgs a = pa;
-      void* kargs[] = {&st, &a, &ntiles, &pg, &bl};
-      const uint64_t tpg = (uint64_t)std::max(1, pa.tiles_per_group);
-      const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(ntiles / tpg, 1ull << 20));
-      HIPTRY(hipModuleLaunchKernel(fns[i], grid, 1, 1, 256, 1, 1, (unsigned)pa.lds_remap, s->stream, kargs, nullptr),
-             "compiled register tile pass launch");
-    } else {
-      Launch<R>::tile_pass_reg(s->amps, pa, eb, s->opbuf.p, (char*)s->opbuf.p + pb + blob_at[i], s->stream);
-    }
-    s->timer.mark(s->stream);
-    HIPTRY(hipGetLastError(), "register tile pass launch");
-  }
-  s->stats.passes += passes.size();
-  s->stats.bytes_moved += (double)passes.size() * 2.0 * (double)(s->amp_size() << s->n);
   return NQPU_OK;
+}
+
+static void add_stats(NqpuRunStats& a, const NqpuRunStats& b) {
+  a.passes += b.passes;
+  a.device_ops += b.device_ops;
+  a.bytes_moved += b.bytes_moved;
+  a.remaps += b.remaps;
+  a.exchanges += b.exchanges;
+  a.mfma_blocks += b.mfma_blocks;
+  a.mfma_gates += b.mfma_gates;
+  a.qphase_gates += b.qphase_gates;
 }
 
 template <typename R>
 static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanOptions& po) {
-  std::vector<Pass> passes = plan_tiles(ops, s->n, po);
-  const int K = std::min(po.tile_qubits, s->n);
-  if (po.registers && K >= 11 && K <= 13) {
-    s->stats.device_ops += ops.size();
-    return run_tiled_reg<R>(s, passes, K - 8, po.mfma, po.compile, po.qphase, po.tiles_per_group);
-  }
-  std::vector<DevOp<R>> dops;
-  std::vector<PassArgs> args;
-  for (const Pass& p : passes) {
-    PassArgs pa{};
-    pa.K = (int)p.qubits.size();
-    pa.n = s->n;
-    pa.op0 = (int)dops.size();
-    pa.nops = (int)p.ops.size();
-    for (int j = 0; j < pa.K; ++j) pa.q[j] = p.qubits[j];
-    append_dev_ops<R>(p, dops);
-    args.push_back(pa);
-  }
-  size_t ob = std::max<size_t>(dops.size(), 1) * sizeof(DevOp<R>);
-  HIPTRY(hipStreamSynchronize(s->stream), "sync before op upload");
-  if (!s->opbuf.ensure(ob)) return NQPU_OUT_OF_DEVICE_MEMORY;
-  if (!dops.empty())
-    HIPTRY(copy_on(s->stream, s->opbuf.p, dops.data(), dops.size() * sizeof(DevOp<R>), hipMemcpyHostToDevice), "op upload");
-  for (const PassArgs& pa : args) {
-    s->timer.mark(s->stream);
-    Launch<R>::tile_pass(s->amps, pa, s->opbuf.p, s->stream);
-    s->timer.mark(s->stream);
-    HIPTRY(hipGetLastError(), "tile pass launch");
-  }
-  s->stats.passes += passes.size();
-  s->stats.device_ops += dops.size();
-  s->stats.bytes_moved += (double)passes.size() * 2.0 * (double)(s->amp_size() << s->n);
+  PreparedRun r;
+  NqpuError e = prepare_tiled_ops<R>(ops, s->n, po, &r);
+  if (e != NQPU_OK) return e;
+  if ((e = upload_image(s, r.image)) != NQPU_OK) return e;
+  if ((e = launch_prepared_t<R>(r, s->amps, s->opbuf.p, s->stream, &s->timer)) != NQPU_OK) return e;
+  // the run's compiled modules stay loaded until the stream is synchronised
+  s->jit_hold.insert(s->jit_hold.end(), r.hold.begin(), r.hold.end());
+  add_stats(s->stats, r.stats);
   return NQPU_OK;
 }
 

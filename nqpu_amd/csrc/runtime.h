@@ -167,6 +167,34 @@ struct State {
   uint64_t dim() const { return 1ull << n; }
 };
 
+// One kernel launch of a tiled run.
+struct RunLaunch {
+  enum Kind { LDS = 0, REG = 1, COMPILED = 2 };
+  int kind = LDS;        // k_tile, k_tile_reg, or a run-time compiled pass kernel
+  PassArgs pa{};
+  int eb = 0;            // register passes: register bits per thread
+  size_t blob_at = 0;    // register passes: byte offset of the pass's operand blobs in the image
+  hipFunction_t fn = nullptr;
+};
+// A tiled run planned on the host: the image its kernels read from device
+// memory (register programs and operand blobs, or the LDS kernel's ops) and
+// its launches, which read the image at whatever device address it is
+// uploaded to.
+struct PreparedRun {
+  int prec = NQPU_PRECISION_C64, n = 0;
+  std::vector<char> image;
+  size_t prog_bytes = 0;  // programs; operand blobs follow
+  std::vector<RunLaunch> launches;
+  std::vector<std::shared_ptr<void>> hold;  // compiled modules the launches use
+  NqpuRunStats stats{};
+};
+// Plan circuit c (NQPU_FUSION_TILED; c.n <= n) for an n-qubit state of precision prec.
+NqpuError prepare_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, int n, const PlanOptions& po,
+                        PreparedRun* out);
+// Enqueue the run's launches on st over the state at amps, the image at
+// `image` in device memory (timer: optional HIP events around each launch).
+NqpuError launch_prepared(const PreparedRun& r, void* amps, const void* image, hipStream_t st, PassTimer* timer);
+
 // Error plumbing: the last error detail is thread-local.
 void set_error(const std::string& s);
 const char* last_error();

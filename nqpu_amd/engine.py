@@ -456,6 +456,57 @@ class StateVector:
         return int(m.outcome), float(m.probability)
 
 
+class RunGraph:
+    """A tiled run of `circuit` on `state` recorded once as a HIP graph
+    (nqpu_graph_create) and replayed with launch(); update(circuit) swaps in
+    a circuit of the same structure (new angles) by copying its register
+    programs into the graph's device buffer, and records the graph again only
+    when the new plan launches other kernels.  from_zero=True makes every
+    replay start from |0...0>.  The state must outlive the graph."""
+
+    def __init__(self, state: "StateVector", circuit: Circuit | NativeCircuit, from_zero=True,
+                 semantics="standard", tile_qubits=0, low_qubits=0, lds_tiles=False, mfma=True, compile=False,
+                 qphase=True, fixed_low=False, untuned=False):
+        lib = L.load()
+        self.state = state
+        nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
+        o = run_options("tiled", semantics, tile_qubits, low_qubits, False, lds_tiles, mfma, compile, qphase,
+                        fixed_low, untuned)
+        p = ct.c_void_p()
+        L.check(lib.nqpu_graph_create(nc.ptr, state.ptr, ct.byref(o), L.GRAPH_FROM_ZERO if from_zero else 0,
+                                      ct.byref(p)), "nqpu_graph_create")
+        self.ptr = p.value
+
+    def update(self, circuit: Circuit | NativeCircuit) -> None:
+        nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
+        L.check(L.load().nqpu_graph_update(self.ptr, nc.ptr), "nqpu_graph_update")
+
+    def launch(self, async_=False) -> None:
+        L.check(L.load().nqpu_graph_launch(self.ptr, L.RUN_ASYNC if async_ else 0), "nqpu_graph_launch")
+
+    def info(self) -> dict:
+        st = L.NqpuGraphInfo()
+        L.check(L.load().nqpu_graph_info(self.ptr, ct.byref(st)), "nqpu_graph_info")
+        return {k: getattr(st, k) for k, _ in L.NqpuGraphInfo._fields_}
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            L.load().nqpu_graph_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 def _terms(obs):
     """[(coeff, pauli string)] or [(coeff, xmask, zmask)] -> [(x, z, coeff)]."""
     out = []

@@ -31,6 +31,10 @@ pub struct NqpuBatch {
     _private: [u8; 0],
 }
 #[repr(C)]
+pub struct NqpuRunGraph {
+    _private: [u8; 0],
+}
+#[repr(C)]
 pub struct NqpuServer {
     _private: [u8; 0],
 }
@@ -173,6 +177,21 @@ pub struct NqpuRunStats {
     pub qphase_gates: u64,
 }
 
+/// Counters of a run graph (nqpu_graph_info).
+#[repr(C)]
+#[derive(Clone, Copy, Debug, Default)]
+pub struct NqpuGraphInfo {
+    pub launches: u64,
+    pub passes: u64,
+    pub replays: u64,
+    pub updates: u64,
+    pub recaptures: u64,
+    pub image_bytes: u64,
+    pub plan_ms: f64,
+}
+
+pub const NQPU_GRAPH_FROM_ZERO: u32 = 1;
+
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default)]
 pub struct NqpuTunedPlan {
@@ -305,6 +324,17 @@ extern "C" {
     pub fn nqpu_run(circuit: *const NqpuCircuit, state: *mut NqpuState, opts: *const NqpuRunOptions) -> NqpuError;
     pub fn nqpu_last_run_stats(state: *const NqpuState, out: *mut NqpuRunStats) -> NqpuError;
     pub fn nqpu_state_synchronize(state: *mut NqpuState) -> NqpuError;
+    pub fn nqpu_graph_create(
+        circuit: *const NqpuCircuit,
+        state: *mut NqpuState,
+        opts: *const NqpuRunOptions,
+        graph_flags: u32,
+        out: *mut *mut NqpuRunGraph,
+    ) -> NqpuError;
+    pub fn nqpu_graph_update(graph: *mut NqpuRunGraph, circuit: *const NqpuCircuit) -> NqpuError;
+    pub fn nqpu_graph_launch(graph: *mut NqpuRunGraph, flags: u32) -> NqpuError;
+    pub fn nqpu_graph_info(graph: *const NqpuRunGraph, out: *mut NqpuGraphInfo) -> NqpuError;
+    pub fn nqpu_graph_destroy(graph: *mut NqpuRunGraph);
     pub fn nqpu_expectation_z_all(state: *const NqpuState, out: *mut f64, len: u32) -> NqpuError;
     pub fn nqpu_expectation_pauli(state: *const NqpuState, xmask: u64, zmask: u64, out: *mut f64) -> NqpuError;
     pub fn nqpu_expectation_pauli_sum(

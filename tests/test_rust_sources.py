@@ -58,7 +58,7 @@ def _rust_fields(name):
 @pytest.mark.parametrize("name", ["NqpuComplex", "NqpuMeasurement", "NqpuVersion", "NqpuGateOp", "NqpuRunOptions",
                                   "NqpuRunStats", "NqpuFusionCounts", "NqpuPauliTerm", "NqpuBatchStats",
                                   "NqpuServerConfig", "NqpuRequestStatus", "NqpuServerStats",
-                                  "NqpuTunedPlan"])
+                                  "NqpuTunedPlan", "NqpuGraphInfo"])
 def test_struct_fields_match_the_header(name):
     assert _c_fields(name) == _rust_fields(name)
 
