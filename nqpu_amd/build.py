@@ -13,10 +13,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libnqpu_hip.so")
+# host-only C++ (x86 code the device pass must not see), compiled with the host compiler
+HOST_SOURCES = ["blob.cpp"]
 SOURCES = ["kernels.hip", "engine.hip", "capi.hip", "server.hip", "jit.hip", "reduce.hip", "wire.hip", "tune.hip", "graph.hip"]
 # device code that run-time compiled passes embed (jit.hip), in this order
 JIT_HEADERS = ["devtypes.h", "regtile.h"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -60,6 +63,13 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
         cmd = [HIPCC, *FLAGS, "-I", objdir, "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    for src in HOST_SOURCES:
+        obj = os.path.join(objdir, src.replace(".cpp", ".o"))
+        objs.append(obj)
+        cmd = [CXX, "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -50,6 +50,7 @@ enum RegOpKind : int32_t {
 };
 
 constexpr int kMfmaBlobBytes = 64 * 64;  // 64 lanes x (2 K-steps x (hi, lo) x 8 fp16)
+constexpr int kMfmaMatrixBytes = 256 * 8;  // a block's 16x16 complex64 matrix (the blob's source)
 constexpr int kQPhaseChunkFloats = 256 * 16;  // one chunk table
 
 template <typename R>

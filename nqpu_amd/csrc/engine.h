@@ -96,6 +96,11 @@ void launch_batch(void* states, uint64_t slot_amps, const BatchCircuit* circs, i
                   double* out, int grid, hipStream_t s);
 
 
+// Operand blobs of nblk matrix-core blocks from their complex64 matrices
+// (kMfmaMatrixBytes each, row-major in block-index order) into consecutive
+// kMfmaBlobBytes units: the layout and fp16 high/low split reg_mfma reads.
+void launch_blob_split(const float2* mats, unsigned char* blobs, size_t nblk, hipStream_t s);
+
 // ----------------------------------------------------------- kernel launches
 // All launches are asynchronous on `s`; R = float (c64) or double (c128).
 template <typename R>

@@ -12,6 +12,9 @@
 #include <functional>
 #include <atomic>
 #include <thread>
+#include <mutex>
+#include <unordered_map>
+#include <memory>
 
 namespace nqpu {
 
@@ -120,8 +123,7 @@ static HostOp op_flip(int a, uint64_t cm) {
 
 // Lower one gate.  Reference semantics reproduce what apply_gate_to_state
 // (sdk/rust/src/circuits/visualization/ascii_viz.rs:124-179) executes.
-std::vector<HostOp> lower_gate(const Gate& g, bool ref) {
-  std::vector<HostOp> v;
+static void lower_gate_into(const Gate& g, bool ref, std::vector<HostOp>& v) {
   const int t0 = g.t[0], t1 = g.t[1];
   const uint64_t c0 = g.nc > 0 ? bit(g.c[0]) : 0, c01 = c0 | (g.nc > 1 ? bit(g.c[1]) : 0);
   const double th = g.p[0];
@@ -218,16 +220,18 @@ std::vector<HostOp> lower_gate(const Gate& g, bool ref) {
     }
     default: break;
   }
+}
+
+std::vector<HostOp> lower_gate(const Gate& g, bool ref) {
+  std::vector<HostOp> v;
+  lower_gate_into(g, ref, v);
   return v;
 }
 
 std::vector<HostOp> lower_circuit(const Circuit& c, bool ref) {
   std::vector<HostOp> v;
-  v.reserve(c.gates.size() * 2);
-  for (const Gate& g : c.gates) {
-    auto w = lower_gate(g, ref);
-    v.insert(v.end(), w.begin(), w.end());
-  }
+  v.reserve(c.gates.size() + 8);
+  for (const Gate& g : c.gates) lower_gate_into(g, ref, v);
   return v;
 }
 
@@ -959,6 +963,37 @@ static std::vector<HostOp> diagonal_ops(const FusedItem& it) {
   return v;
 }
 
+// Matrix-core block choices of register tile passes, by pass structure.  The
+// choice (which pending ops form which block) depends only on the ops' kinds,
+// tile bits, controls and qubits outside the tile, never on their angles, and
+// the same structures recur across the circuits of a variational batch and the
+// iterations of an optimisation loop.  An entry lists the program's items in
+// order: {op} for an op applied alone, {-1, op, op, ...} for a block.
+namespace {
+using BlockChoice = std::vector<std::vector<int>>;
+std::mutex g_block_mu;
+std::unordered_map<std::string, std::shared_ptr<const BlockChoice>> g_block_cache;
+constexpr size_t kBlockCacheMax = 1 << 16;  // entries; the cache starts over when full
+
+std::shared_ptr<const BlockChoice> block_choice_find(const std::string& key) {
+  std::lock_guard<std::mutex> lk(g_block_mu);
+  auto it = g_block_cache.find(key);
+  return it == g_block_cache.end() ? nullptr : it->second;
+}
+
+void block_choice_store(const std::string& key, BlockChoice&& c) {
+  auto v = std::make_shared<const BlockChoice>(std::move(c));
+  std::lock_guard<std::mutex> lk(g_block_mu);
+  if (g_block_cache.size() >= kBlockCacheMax) g_block_cache.clear();
+  g_block_cache.emplace(key, std::move(v));
+}
+
+template <typename T>
+void key_put(std::string& k, T x) {
+  k.append(reinterpret_cast<const char*>(&x), sizeof x);
+}
+}  // namespace
+
 // Program of one register-resident tile pass (k_tile_reg).  Slots: thread
 // slots 0..7 (code i) and register slots 0..eb-1 (code 16+j); map[] gives the
 // tile-local bit each slot holds.  Ops are issued in order; an op runs in the
@@ -1492,7 +1527,40 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
   std::vector<Item> items;
   std::vector<Blk> blks;
   const bool use_mfma = mfma && eb == 4 && tb == 8;
+  std::string choice_key;
+  std::shared_ptr<const BlockChoice> choice;
   if (use_mfma) {
+    key_put(choice_key, K);
+    key_put(choice_key, max_blobs);
+    for (const L& o : ops) {
+      key_put(choice_key, o.kind);
+      key_put(choice_key, o.a);
+      key_put(choice_key, o.b);
+      key_put(choice_key, o.cm);
+      key_put(choice_key, o.gc);
+      if (o.kind == kQPhaseOp) {
+        key_put(choice_key, qgroups[o.grp].tile);
+        key_put(choice_key, qgroups[o.grp].gmask);
+      }
+    }
+    choice = block_choice_find(choice_key);
+  }
+  if (choice) {
+    for (const std::vector<int>& rec : *choice) {
+      if (rec[0] >= 0) {
+        items.push_back(Item{rec[0], -1, qmask(ops[rec[0]]), false});
+        continue;
+      }
+      Blk b;
+      b.ops.assign(rec.begin() + 1, rec.end());
+      for (int k : b.ops) {
+        b.q |= qmask(ops[k]);
+        b.g |= gmask(ops[k]);
+      }
+      blks.push_back(b);
+      items.push_back(Item{-1, (int)blks.size() - 1, b.q, false});
+    }
+  } else if (use_mfma) {
     auto fusable = [&](const L& o) {
       if (__builtin_popcountll(gmask(o)) > kMfmaMaxSel) return false;
       return o.kind == OP_U1 || o.kind == OP_FLIP || o.kind == OP_D1 || o.kind == OP_D2 || o.kind == OP_U2 ||
@@ -1610,6 +1678,17 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       }
       rem.swap(left);
     }
+    BlockChoice rec;
+    for (const Item& it : items) {
+      if (it.op >= 0) {
+        rec.push_back({it.op});
+        continue;
+      }
+      std::vector<int> r{-1};
+      r.insert(r.end(), blks[it.blk].ops.begin(), blks[it.blk].ops.end());
+      rec.push_back(std::move(r));
+    }
+    block_choice_store(choice_key, std::move(rec));
   } else {
     for (int k = 0; k < (int)ops.size(); ++k) items.push_back(Item{k, -1, qmask(ops[k]), false});
   }
@@ -1662,52 +1741,91 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       auto bitat = [&](int x, int i) -> int {
         return x >= kGlobalBit ? (int)((gset >> (x - kGlobalBit)) & 1) : (i >> pos[x]) & 1;
       };
-      std::array<cd, 256> M{};
-      for (int c = 0; c < 16; ++c) {  // column c = the block applied to basis vector c
-        cd v[16];
-        for (int i = 0; i < 16; ++i) v[i] = (i == c) ? cd(1, 0) : cd(0, 0);
-        for (int k : b.ops) {
-          const L& o = ops[k];
-          if ((o.gc & ~gset) != 0) continue;  // a control outside the tile is 0 in this variant
-          const HostOp& h = *o.h;
-          uint32_t cmk = 0;
-          for (int q = 0; q < K; ++q)
-            if ((o.cm >> q) & 1) cmk |= 1u << pos[q];
-          const int pa = o.a < kGlobalBit ? pos[o.a] : 0, pb = (two(o.kind) && o.b < kGlobalBit) ? pos[o.b] : 0;
-          for (int i = 0; i < 16; ++i) {
-            if ((i & cmk) != cmk) continue;
-            switch (o.kind) {
-              case OP_U1:
-                if (!((i >> pa) & 1)) {
-                  const int j = i | (1 << pa);
-                  const cd x = v[i], y = v[j];
-                  v[i] = h.m[0] * x + h.m[1] * y;
-                  v[j] = h.m[2] * x + h.m[3] * y;
+      // M = op_k ... op_1: each op is a row operation on M (row: block index
+      // of the output, column: of the input), applied to all 16 columns at
+      // once, with separate real and imaginary planes
+      double Mr[16][16], Mi[16][16];
+      for (int i = 0; i < 16; ++i)
+        for (int c = 0; c < 16; ++c) {
+          Mr[i][c] = i == c ? 1.0 : 0.0;
+          Mi[i][c] = 0.0;
+        }
+      auto scale_row = [&](int i, cd z) {
+        const double zr = z.real(), zi = z.imag();
+        for (int c = 0; c < 16; ++c) {
+          const double xr = Mr[i][c], xi = Mi[i][c];
+          Mr[i][c] = zr * xr - zi * xi;
+          Mi[i][c] = zr * xi + zi * xr;
+        }
+      };
+      auto swap_rows = [&](int i, int j) {
+        for (int c = 0; c < 16; ++c) {
+          std::swap(Mr[i][c], Mr[j][c]);
+          std::swap(Mi[i][c], Mi[j][c]);
+        }
+      };
+      for (int k : b.ops) {
+        const L& o = ops[k];
+        if ((o.gc & ~gset) != 0) continue;  // a control outside the tile is 0 in this variant
+        const HostOp& h = *o.h;
+        uint32_t cmk = 0;
+        for (int q = 0; q < K; ++q)
+          if ((o.cm >> q) & 1) cmk |= 1u << pos[q];
+        const int pa = o.a < kGlobalBit ? pos[o.a] : 0, pb = (two(o.kind) && o.b < kGlobalBit) ? pos[o.b] : 0;
+        for (int i = 0; i < 16; ++i) {
+          if ((i & cmk) != cmk) continue;
+          switch (o.kind) {
+            case OP_U1:
+              if (!((i >> pa) & 1)) {
+                const int j = i | (1 << pa);
+                const double m0r = h.m[0].real(), m0i = h.m[0].imag(), m1r = h.m[1].real(), m1i = h.m[1].imag();
+                const double m2r = h.m[2].real(), m2i = h.m[2].imag(), m3r = h.m[3].real(), m3i = h.m[3].imag();
+                for (int c = 0; c < 16; ++c) {
+                  const double xr = Mr[i][c], xi = Mi[i][c], yr = Mr[j][c], yi = Mi[j][c];
+                  Mr[i][c] = m0r * xr - m0i * xi + m1r * yr - m1i * yi;
+                  Mi[i][c] = m0r * xi + m0i * xr + m1r * yi + m1i * yr;
+                  Mr[j][c] = m2r * xr - m2i * xi + m3r * yr - m3i * yi;
+                  Mi[j][c] = m2r * xi + m2i * xr + m3r * yi + m3i * yr;
                 }
-                break;
-              case OP_FLIP:
-                if (!((i >> pa) & 1)) std::swap(v[i], v[i | (1 << pa)]);
-                break;
-              case OP_D1: v[i] *= h.m[bitat(o.a, i)]; break;
-              case OP_D2: v[i] *= h.m[bitat(o.a, i) | (bitat(o.b, i) << 1)]; break;
-              case OP_U2:
-                if (!((i >> pa) & 1) && !((i >> pb) & 1)) {
-                  const int idx[4] = {i, i | (1 << pa), i | (1 << pb), i | (1 << pa) | (1 << pb)};
-                  const cd x[4] = {v[idx[0]], v[idx[1]], v[idx[2]], v[idx[3]]};
-                  for (int r = 0; r < 4; ++r)
-                    v[idx[r]] = h.m[4 * r] * x[0] + h.m[4 * r + 1] * x[1] + h.m[4 * r + 2] * x[2] +
-                                h.m[4 * r + 3] * x[3];
+              }
+              break;
+            case OP_FLIP:
+              if (!((i >> pa) & 1)) swap_rows(i, i | (1 << pa));
+              break;
+            case OP_D1: scale_row(i, h.m[bitat(o.a, i)]); break;
+            case OP_D2: scale_row(i, h.m[bitat(o.a, i) | (bitat(o.b, i) << 1)]); break;
+            case OP_U2:
+              if (!((i >> pa) & 1) && !((i >> pb) & 1)) {
+                const int idx[4] = {i, i | (1 << pa), i | (1 << pb), i | (1 << pa) | (1 << pb)};
+                for (int c = 0; c < 16; ++c) {
+                  double xr[4], xi[4];
+                  for (int t = 0; t < 4; ++t) {
+                    xr[t] = Mr[idx[t]][c];
+                    xi[t] = Mi[idx[t]][c];
+                  }
+                  for (int r = 0; r < 4; ++r) {
+                    double sr = 0, si = 0;
+                    for (int t = 0; t < 4; ++t) {
+                      const double mr = h.m[4 * r + t].real(), mi = h.m[4 * r + t].imag();
+                      sr += mr * xr[t] - mi * xi[t];
+                      si += mr * xi[t] + mi * xr[t];
+                    }
+                    Mr[idx[r]][c] = sr;
+                    Mi[idx[r]][c] = si;
+                  }
                 }
-                break;
-              case OP_SWAP:
-                if (((i >> pa) & 1) && !((i >> pb) & 1)) std::swap(v[i], v[(i ^ (1 << pa)) | (1 << pb)]);
-                break;
-              default: break;
-            }
+              }
+              break;
+            case OP_SWAP:
+              if (((i >> pa) & 1) && !((i >> pb) & 1)) swap_rows(i, (i ^ (1 << pa)) | (1 << pb));
+              break;
+            default: break;
           }
         }
-        for (int i = 0; i < 16; ++i) M[16 * i + c] = v[i];
       }
+      std::array<cd, 256> M;
+      for (int i = 0; i < 16; ++i)
+        for (int c = 0; c < 16; ++c) M[16 * i + c] = cd(Mr[i][c], Mi[i][c]);
       out.blocks.push_back(M);
     }
     out.fused_ops += b.ops.size();
@@ -1925,8 +2043,15 @@ static double from_half(uint16_t h) {
   return f;
 }
 
+bool mfma_blob_fast_available();                // blob.cpp (F16C)
+void mfma_blob_fast(const cd* M, uint16_t* out);  // the same blob, 8 values per conversion
+
 // Operand blob of one R_MFMA op from its 16x16 block matrix (block-index order).
 static void mfma_blob(const std::array<cd, 256>& M, uint16_t* out /* kMfmaBlobBytes / 2 */) {
+  if (mfma_blob_fast_available()) {
+    mfma_blob_fast(M.data(), out);
+    return;
+  }
   auto A = [&](int m, int k) {  // 32x32 real form
     const cd u = M[16 * (m >> 1) + (k >> 1)];
     const int cr = m & 1, cc = k & 1;
@@ -2976,7 +3101,10 @@ struct CircuitPlan {
   std::vector<BatchPass> bp;
   std::vector<RegOp<float>> prog;
   std::vector<BatchTerm> bt;
-  std::vector<uint16_t> blob;  // operand units of the HBM passes' blocks and phase tables
+  // HBM passes' programs: their blocks (nblk, numbered in pass order) and
+  // phase tables (qunits operand units of kMfmaBlobBytes, in pass order)
+  std::vector<RegProgram<float>> progs;
+  size_t nblk = 0, qunits = 0;
   double cost = 0;
   uint64_t amps = 0;  // HBM amplitudes (0 when resident)
 };
@@ -3046,24 +3174,20 @@ CircuitPlan plan_batch_circuit(const Circuit& c, const std::vector<PauliTerm>& t
     // LDS remaps only (the kernel loads and stores tiles in the canonical
     // slot order)
     RegProgram<float> pr = build_reg_program<float>(p, EB, 8, true, false, 1 << 20, true);
-    for (RegOp<float>& r : pr.ops)  // phase tables follow the blocks' operand blobs
-      if (r.kind == R_QPHASE) r.s0 += (int32_t)pr.blocks.size();
-    P.blob_unit = (int32_t)(cp.blob.size() / (kMfmaBlobBytes / 2));
+    for (RegOp<float>& r : pr.ops)  // the circuit's phase-table unit (plan_batch rebases it)
+      if (r.kind == R_QPHASE) r.s0 += (int32_t)cp.qunits;
+    // the circuit's first block of this pass (plan_batch makes it global);
+    // block matrices and phase tables go into the upload image straight from
+    // these programs
+    P.blob_unit = (int32_t)cp.nblk;
     P.scaled = pr.blocks.empty() ? 0 : 1;
-    for (const auto& M : pr.blocks) {
-      const size_t at = cp.blob.size();
-      cp.blob.resize(at + kMfmaBlobBytes / 2);
-      mfma_blob(M, cp.blob.data() + at);
-    }
-    if (!pr.qtab.empty()) {
-      const size_t at = cp.blob.size();
-      cp.blob.resize(at + pr.qtab.size() * 2);
-      std::memcpy(cp.blob.data() + at, pr.qtab.data(), pr.qtab.size() * sizeof(float));
-    }
+    cp.nblk += pr.blocks.size();
+    cp.qunits += pr.qtab.size() * sizeof(float) / kMfmaBlobBytes;
     P.op0 = (int)cp.prog.size();
     P.nops = (int)pr.ops.size();
     cp.prog.insert(cp.prog.end(), pr.ops.begin(), pr.ops.end());
     cp.bp.push_back(P);
+    cp.progs.push_back(std::move(pr));
   }
   // observable passes: each term on a tile set that holds its X bits
   std::vector<std::vector<int>> sets(2);
@@ -3135,7 +3259,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   }
   const size_t nc = cps.size();
   // offsets of each circuit's passes, program ops and terms in the tables
-  std::vector<size_t> bp_at(nc + 1, 0), pg_at(nc + 1, 0), bt_at(nc + 1, 0), bl_at(nc + 1, 0);
+  std::vector<size_t> bp_at(nc + 1, 0), pg_at(nc + 1, 0), bt_at(nc + 1, 0), bl_at(nc + 1, 0), qt_at(nc + 1, 0);
   std::vector<BatchCircuit> bc(nc);
   uint64_t slot_amps = 0;
   NqpuBatchStats st{};
@@ -3148,7 +3272,8 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
     bp_at[k + 1] = bp_at[k] + cp.bp.size();
     pg_at[k + 1] = pg_at[k] + cp.prog.size();
     bt_at[k + 1] = bt_at[k] + cp.bt.size();
-    bl_at[k + 1] = bl_at[k] + cp.blob.size() / (kMfmaBlobBytes / 2);  // operand units
+    bl_at[k + 1] = bl_at[k] + cp.nblk;    // matrix-core blocks
+    qt_at[k + 1] = qt_at[k] + cp.qunits;  // phase-table units
     BatchCircuit C = cp.C;
     C.pass0 += (int32_t)bp_at[k];
     C.red0 += (int32_t)bp_at[k];
@@ -3175,8 +3300,14 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   const size_t og = up(op_ + bp_at[nc] * sizeof(BatchPass));
   const size_t ot = up(og + pg_at[nc] * sizeof(RegOp<float>));
   const size_t ocnt = up(ot + bt_at[nc] * sizeof(BatchTerm));
-  const size_t obl = ocnt + 256;  // operand units, 256-byte aligned
-  const size_t total = obl + bl_at[nc] * kMfmaBlobBytes;
+  const size_t head = ocnt + 256;  // operand units follow, 256-byte aligned
+  const size_t nblk = bl_at[nc], nq = qt_at[nc];
+  const size_t hq = head, hm = hq + nq * kMfmaBlobBytes;  // host image: phase tables, matrices
+  const size_t total = hm + nblk * kMfmaMatrixBytes;
+  if (nblk + nq > (size_t)INT32_MAX) {
+    set_error("batch operand units exceed the 32-bit unit index");
+    return NQPU_INVALID_PARAMETER;
+  }
   char* host = buffer(total);
   if (!host) {
     set_error("no host memory for the batch tables");
@@ -3191,18 +3322,30 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
       for (size_t k; (k = next.fetch_add(1)) < nc;) {
         const CircuitPlan& cp = cps[k];
         BatchPass* dp = reinterpret_cast<BatchPass*>(host + op_) + bp_at[k];
+        RegOp<float>* dg = reinterpret_cast<RegOp<float>*>(host + og) + pg_at[k];
+        if (!cp.prog.empty()) std::memcpy(dg, cp.prog.data(), cp.prog.size() * sizeof(RegOp<float>));
         for (size_t i = 0; i < cp.bp.size(); ++i) {
           BatchPass P = cp.bp[i];
           P.op0 += (int32_t)pg_at[k];
           P.term0 += (int32_t)bt_at[k];
           P.blob_unit += (int32_t)bl_at[k];
+          // operand units count from the blob area: blocks, then phase tables
+          for (int j = 0; j < cp.bp[i].nops && !cp.C.resident; ++j) {
+            RegOp<float>& r = dg[cp.bp[i].op0 + j];
+            if (r.kind == R_QPHASE) r.s0 += (int32_t)(nblk + qt_at[k]) - P.blob_unit;
+          }
           std::memcpy(dp + i, &P, sizeof(BatchPass));
         }
-        if (!cp.blob.empty())
-          std::memcpy(host + obl + bl_at[k] * kMfmaBlobBytes, cp.blob.data(), cp.blob.size() * 2);
-        if (!cp.prog.empty())
-          std::memcpy(reinterpret_cast<RegOp<float>*>(host + og) + pg_at[k], cp.prog.data(),
-                      cp.prog.size() * sizeof(RegOp<float>));
+        char* u = host + hq + qt_at[k] * kMfmaBlobBytes;
+        float2* mm = reinterpret_cast<float2*>(host + hm) + bl_at[k] * 256;
+        for (const RegProgram<float>& pr : cp.progs) {
+          for (const auto& M : pr.blocks)
+            for (int e = 0; e < 256; ++e) *mm++ = make_float2((float)M[e].real(), (float)M[e].imag());
+          if (!pr.qtab.empty()) {
+            std::memcpy(u, pr.qtab.data(), pr.qtab.size() * sizeof(float));
+            u += pr.qtab.size() * sizeof(float);
+          }
+        }
         if (!cp.bt.empty())
           std::memcpy(reinterpret_cast<BatchTerm*>(host + ot) + bt_at[k], cp.bt.data(), cp.bt.size() * sizeof(BatchTerm));
       }
@@ -3220,7 +3363,12 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   plan.og = og;
   plan.ot = ot;
   plan.ocnt = ocnt;
-  plan.obl = obl;
+  plan.head = head;
+  plan.nblk = nblk;
+  plan.qunits = nq;
+  plan.dev_q = head + nblk * kMfmaBlobBytes;
+  plan.dev_m = plan.dev_q + nq * kMfmaBlobBytes;
+  plan.dev_total = plan.dev_m + nblk * kMfmaMatrixBytes;
   plan.total = total;
   plan.slot_amps = slot_amps;
   plan.ncirc = nc;
@@ -3262,13 +3410,23 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   // another, so the HBM it needs is that of the largest circuit
   const uint64_t state_amps = plan.slot_amps * (uint64_t)std::max(grid, 1);
   HIPTRY(hipStreamSynchronize(b->stream), "sync");
-  if (!b->tables.ensure(total) || !b->out.ensure(circuits.size() * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  if (!b->tables.ensure(plan.dev_total) || !b->out.ensure(circuits.size() * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
   if (state_amps && !b->states.ensure(state_amps * 8)) {
     set_error("batch states do not fit in device memory");
     return NQPU_OUT_OF_DEVICE_MEMORY;
   }
-  HIPTRY(copy_on(b->stream, b->tables.p, plan.host, total, hipMemcpyHostToDevice), "batch table upload");
   char* base = (char*)b->tables.p;
+  // the head, then the phase tables and block matrices; the blocks' operand
+  // blobs are built on the device between them
+  HIPTRY(hipMemcpyAsync(base, plan.host, plan.head, hipMemcpyHostToDevice, b->stream), "batch table upload");
+  if (total > plan.head)
+    HIPTRY(hipMemcpyAsync(base + plan.dev_q, plan.host + plan.head, total - plan.head, hipMemcpyHostToDevice,
+                          b->stream), "batch operand upload");
+  if (plan.nblk) {
+    launch_blob_split((const float2*)(base + plan.dev_m), (unsigned char*)(base + plan.head), plan.nblk, b->stream);
+    HIPTRY(hipGetLastError(), "operand blob launch");
+  }
+  HIPTRY(hipStreamSynchronize(b->stream), "batch table upload");
   const auto t1 = clock::now();
   hipEvent_t e0, e1;
   HIPTRY(hipEventCreate(&e0), "event");
@@ -3277,7 +3435,7 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   if (grid > 0)
     launch_batch(b->states.p, plan.slot_amps, (const BatchCircuit*)(base + oc), (int)plan.ncirc, (const BatchPass*)(base + op_),
                  (const RegOp<float>*)(base + og), (const BatchTerm*)(base + ot),
-                 (const unsigned char*)(base + plan.obl), (unsigned*)(base + ocnt), (double*)b->out.p, grid,
+                 (const unsigned char*)(base + plan.head), (unsigned*)(base + ocnt), (double*)b->out.p, grid,
                  b->stream);
   NqpuError le = hip_check(hipGetLastError(), "batch launch");
   (void)hipEventRecord(e1, b->stream);

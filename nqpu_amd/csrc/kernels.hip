@@ -607,6 +607,47 @@ void launch_batch(void* states, uint64_t slot_amps, const BatchCircuit* circs, i
                      passes, prog, terms, blob, counter, out);
 }
 
+// Operand blob of block blockIdx.x from its 16x16 complex64 matrix M: lane l
+// of the blob holds A(l % 32, 16 s + 8 (l / 32) + i) for K-steps s = 0, 1 and
+// i = 0..7 of the 32x32 real form [[re, -im], [im, re]], as 8 fp16 high
+// parts then 8 fp16 low parts (x - hi, rounded to nearest even again; the
+// difference is exact in float).  One wavefront per block, one lane per blob
+// lane, 64 bytes written per lane.
+__global__ __launch_bounds__(64) void k_blob_split(const float2* __restrict__ mats, unsigned char* __restrict__ blobs) {
+  const int l = threadIdx.x, m = l % 32;
+  const float2* M = mats + (size_t)blockIdx.x * 256 + 16 * (m >> 1);
+  uint32_t w[16];
+#pragma unroll
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      uint16_t hb[2], lb[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int k = 16 * s + 8 * (l / 32) + i + t;
+        const float2 u = M[k >> 1];
+        const float x = (m & 1) == 0 ? ((k & 1) == 0 ? u.x : -u.y) : ((k & 1) == 0 ? u.y : u.x);
+        const _Float16 h = (_Float16)x;
+        const _Float16 lo = (_Float16)(x - (float)h);
+        hb[t] = __builtin_bit_cast(uint16_t, h);
+        lb[t] = __builtin_bit_cast(uint16_t, lo);
+      }
+      w[8 * s + i / 2] = hb[0] | ((uint32_t)hb[1] << 16);
+      w[8 * s + 4 + i / 2] = lb[0] | ((uint32_t)lb[1] << 16);
+    }
+  uint4* out = reinterpret_cast<uint4*>(blobs + (size_t)blockIdx.x * kMfmaBlobBytes + l * 64);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+}
+
+void launch_blob_split(const float2* mats, unsigned char* blobs, size_t nblk, hipStream_t s) {
+  for (size_t b0 = 0; b0 < nblk; b0 += (1u << 30)) {  // grid.x stays far below its limit
+    const size_t nb = std::min<size_t>(nblk - b0, 1u << 30);
+    hipLaunchKernelGGL(k_blob_split, dim3((unsigned)nb), dim3(64), 0, s, mats + b0 * 256,
+                       blobs + b0 * kMfmaBlobBytes);
+  }
+}
+
 // out[c] = sum |a_i|^2 over chunk c of 2^cb amplitudes.
 template <typename R, typename T>
 __global__ __launch_bounds__(kBlock) void k_sumsq(const T* st, int cb, uint64_t dim, double* out) {

@@ -263,9 +263,17 @@ struct Batch {
   ~Batch();
 };
 // Host half of a batch: the tables k_batch reads, in one upload image.
+// Host image: [circuits][passes][programs][terms][counter] (head bytes), the
+// phase-table units, then one complex64 16x16 matrix per matrix-core block.
+// Device tables: the head, the blocks' operand blobs (built on the device
+// from the matrices), the phase-table units, the matrices.  Programs address
+// both kinds of operand unit from the start of the blob area.
 struct BatchPlan {
   char* host = nullptr;  // from the buffer callback of plan_batch (not owned)
-  size_t oc = 0, op = 0, og = 0, ot = 0, ocnt = 0, obl = 0, total = 0;  // obl: operand blobs
+  size_t oc = 0, op = 0, og = 0, ot = 0, ocnt = 0, total = 0;  // total: host image bytes
+  size_t head = 0;                            // host = device offset of the blob area
+  size_t nblk = 0, qunits = 0;                // matrix-core blocks, phase-table units
+  size_t dev_q = 0, dev_m = 0, dev_total = 0;  // device offsets: phase tables, matrices; size
   uint64_t slot_amps = 0;  // largest HBM-resident circuit: each workgroup's state slot
   size_t ncirc = 0, program_ops = 0;
   NqpuBatchStats st{};
