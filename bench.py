@@ -118,24 +118,26 @@ def bench_batch(args):
     vcs = variational_batch(args.circuits, seed=103)
     circs = [v.circuit.native() for v in vcs]
     obs = [v.hamiltonian() for v in vcs]
+    obs_native = E.Observables(obs)  # the term table, built once like the native circuits
     log(f"built {len(circs)} circuits in {time.perf_counter() - t0:.1f} s "
         f"({sum(len(v.circuit.gates) for v in vcs)} gates)")
     b = E.Batch()
     steps, warmup = args.batch_steps or args.steps, args.batch_warmup if args.batch_warmup >= 0 else args.warmup
     for w in range(warmup):
-        b.expectation(circs, obs)
+        b.expectation(circs, obs_native)
         log(f"warmup {w + 1}/{warmup} done")
-    kernel_ms = plan_ms = 0.0
+    # batches are queued without waiting: the host plans batch k+1 while the
+    # device runs batch k (two in flight); the timed region ends when the
+    # last batch's energies are on the host
     t1 = time.perf_counter()
-    for _ in range(steps):
-        energies = b.expectation(circs, obs)
-        st = b.last_stats()
-        kernel_ms += st["kernel_ms"]
-        plan_ms += st["plan_ms"]
+    outs = [b.expectation(circs, obs_native, async_=True) for _ in range(steps)]
+    b.synchronize()
     elapsed = time.perf_counter() - t1
+    energies = outs[-1]
+    st = b.last_stats()
     value = len(circs) * steps / elapsed
-    log(f"batched: {elapsed / steps * 1e3:.1f} ms per batch (kernel {kernel_ms / steps:.1f} ms)")
-    kms = kernel_ms / steps
+    kms, pms = st["kernel_ms"], st["plan_ms"]
+    log(f"batched: {elapsed / steps * 1e3:.1f} ms per batch (last batch: kernel {kms:.1f} ms, host {pms:.1f} ms)")
     achieved = st["hbm_bytes"] / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     # plain per-circuit launches: one state, one nqpu_run and one Pauli sum per circuit
     sample = list(range(0, len(circs), max(1, len(circs) // args.per_circuit_sample)))
@@ -156,7 +158,8 @@ def bench_batch(args):
         "config": {"workload": "4096 hardware-efficient ansatz circuits, n ~ U{8..20}, L ~ U{4..12}, "
                                "RY.RZ layers + CZ ladder, TFIM energy with g ~ U[0.2,1.8], seed 103, "
                                "one persistent launch", "circuits": len(circs)},
-        "kernel_ms_per_batch": kms, "plan_ms_per_batch": plan_ms / steps,
+        "kernel_ms_per_batch": kms, "plan_ms_per_batch": pms,
+        "pipelined": "two batches in flight: host planning of batch k+1 overlaps the launch of batch k",
         "batch_stats": st,
         # the persistent launch: algorithmic state bytes of its HBM-resident
         # circuits (2 x gate passes - 1 + observable passes) x 2^n x 8 over its
@@ -496,7 +499,19 @@ def bench_c1(args):
                 ts.append(time.perf_counter() - t)
             amps[prec] = sv.amplitudes()
             z = sv.expectation_z_all()
-        out[prec] = {"ms": min(ts) * 1e3, "amp_updates_per_s": gates * float(1 << n) / min(ts)}
+            # the same run recorded once as a HIP graph (|0...0> fill + passes) and replayed
+            with E.RunGraph(sv, nc, from_zero=True) as g:
+                g.launch()
+                tg = []
+                for _ in range(5):
+                    t = time.perf_counter()
+                    g.launch()
+                    tg.append(time.perf_counter() - t)
+                graph_equal = bool(np.array_equal(sv.amplitudes(), amps[prec]))
+                ginfo = g.info()
+        out[prec] = {"ms": min(ts) * 1e3, "amp_updates_per_s": gates * float(1 << n) / min(ts),
+                     "graph_replay_ms": min(tg) * 1e3, "graph_launches": ginfo["launches"],
+                     "graph_replay_equals_run": graph_equal}
         amps[prec + "_z"] = z
     if not args.no_cpu:
         from oracle import cpu_ref
@@ -513,6 +528,63 @@ def bench_c1(args):
         out["c64_infidelity_vs_oracle"] = float(1 - abs(ov) ** 2 / np.vdot(amps["c64"], amps["c64"]).real)
         out["c64_max_abs_z_err_vs_oracle"] = float(np.max(np.abs(amps["c64_z"] - zref)))
         out["speedup_c128_vs_cpu"] = dt / (out["c128"]["ms"] * 1e-3)
+    return out
+
+
+def bench_variational_loop(args, iters=40):
+    """A variational loop on one state: 18-qubit hardware-efficient ansatz
+    (RY.RZ on every qubit per layer + CZ ladder, 8 layers), each iteration new
+    angles and the transverse-field Ising energy (g = 1).  Plain: nqpu_run per
+    iteration (plan, upload, launches).  Graph: the run recorded once as a
+    HIP graph; each iteration copies the new angles' register programs into
+    it (nqpu_graph_update) and replays it.  complex64, energies in FP64."""
+    import numpy as np
+
+    from nqpu_amd import engine as E
+    n, layers = 18, 8
+    rng = np.random.default_rng(11)
+
+    def ansatz():
+        c = E.Circuit(n)
+        for _ in range(layers):
+            for q in range(n):
+                c.ry(q, float(rng.uniform(-np.pi, np.pi)))
+                c.rz(q, float(rng.uniform(-np.pi, np.pi)))
+            for q in range(n - 1):
+                c.cz(q, q + 1)
+        return c.native()
+
+    circs = [ansatz() for _ in range(iters)]
+    ham = [(-1.0, "I" * i + "ZZ" + "I" * (n - i - 2)) for i in range(n - 1)] + \
+          [(-1.0, "I" * i + "X" + "I" * (n - i - 1)) for i in range(n)]
+    out = {"workload": f"{n}-qubit ansatz, {layers} layers RY.RZ + CZ ladder, TFIM energy, {iters} iterations",
+           "gates": (2 * n + n - 1) * layers}
+    with E.StateVector(n, "c64") as sv:
+        sv.run(circs[0])
+        sv.expectation_pauli_sum(ham)
+        t = time.perf_counter()
+        e_plain = []
+        for c in circs:
+            sv.reset()
+            sv.run(c)
+            e_plain.append(sv.expectation_pauli_sum(ham))
+        dt_plain = time.perf_counter() - t
+        with E.RunGraph(sv, circs[0], from_zero=True) as g:
+            g.launch()
+            t = time.perf_counter()
+            e_graph = []
+            for c in circs:
+                g.update(c)
+                g.launch(async_=True)
+                e_graph.append(sv.expectation_pauli_sum(ham))
+            dt_graph = time.perf_counter() - t
+            info = g.info()
+    out["plain_ms_per_iteration"] = dt_plain * 1e3 / iters
+    out["graph_ms_per_iteration"] = dt_graph * 1e3 / iters
+    out["speedup_graph_vs_plain"] = dt_plain / dt_graph
+    out["graph_recaptures"] = info["recaptures"]
+    out["graph_plan_ms_last_update"] = info["plan_ms"]
+    out["energy_max_abs_diff_graph_vs_plain"] = float(np.max(np.abs(np.array(e_plain) - np.array(e_graph))))
     return out
 
 
@@ -770,6 +842,8 @@ def main():
         args.cpu_seconds = min(args.cpu_seconds, 8.0)
         log("C1: 20-qubit parity circuit")
         result["c1"] = bench_c1(args)
+        log("variational loop: plain runs and graph replays")
+        result["variational_loop"] = bench_variational_loop(args)
         log("reductions and sampling")
         result["reductions"] = bench_reductions(args)
         log("C3: batched variational circuits")

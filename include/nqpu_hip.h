@@ -416,7 +416,7 @@ typedef struct {
   uint64_t resident;        /* circuits held entirely in registers/LDS */
   uint64_t gate_passes;     /* tile passes over HBM-resident states */
   uint64_t observable_passes;
-  double plan_ms;           /* host planning and upload */
+  double plan_ms;           /* host planning and queueing (the upload is asynchronous) */
   double kernel_ms;         /* the persistent launch */
   double state_bytes;       /* HBM for the states of circuits above 12 qubits: one slot of the
                                largest such circuit per workgroup (nqpu_debug_batch_plan: one slot) */
@@ -433,10 +433,18 @@ void nqpu_batch_destroy(NqpuBatch* batch);
  * to out[k].  All circuits run in one persistent launch; every result is
  * independent of which other circuits share the batch.  Circuits need at
  * least 4 qubits; the jax_bridge vmap_simulate map (serial) is what this
- * batches (sdk/rust/src/quantum_ml/bridges/jax_bridge.rs:447). */
+ * batches (sdk/rust/src/quantum_ml/bridges/jax_bridge.rs:447).
+ * With NQPU_RUN_ASYNC in opts->flags the call returns once the batch is
+ * queued: the host can plan the next batch while this one runs (two batches
+ * are kept in flight; a third call first completes the oldest).  `out` is
+ * written when nqpu_batch_synchronize returns (or a later call completes the
+ * batch) and must stay valid until then; circuits and terms may be released
+ * as soon as the call returns. */
 NqpuError nqpu_batch_expectation(NqpuBatch* batch, const NqpuCircuit* const* circuits, uint64_t count,
                                  const NqpuPauliTerm* terms, const uint64_t* offsets,
                                  const NqpuRunOptions* opts, double* out);
+/* Wait for the batches queued with NQPU_RUN_ASYNC and write their results. */
+NqpuError nqpu_batch_synchronize(NqpuBatch* batch);
 NqpuError nqpu_batch_last_stats(const NqpuBatch* batch, NqpuBatchStats* out);
 /* Host half of nqpu_batch_expectation only (planning, no device): fills the
  * pass counts of `out` and the number of register-program operations. */

@@ -202,6 +202,7 @@ SIGNATURES = {
     "nqpu_batch_destroy": (None, [_P]),
     "nqpu_batch_expectation": (_E, [_P, _P, ct.c_uint64, _P, _P, ct.POINTER(NqpuRunOptions), _P]),
     "nqpu_batch_last_stats": (_E, [_P, ct.POINTER(NqpuBatchStats)]),
+    "nqpu_batch_synchronize": (_E, [_P]),
     "nqpu_debug_batch_plan": (_E, [_P, ct.c_uint64, _P, _P, ct.POINTER(NqpuRunOptions),
                                    ct.POINTER(NqpuBatchStats), ct.POINTER(ct.c_uint64)]),
     "nqpu_server_create": (_P, [ct.POINTER(NqpuServerConfig)]),

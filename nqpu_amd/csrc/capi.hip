@@ -674,6 +674,11 @@ NqpuError nqpu_batch_expectation(NqpuBatch* b, const NqpuCircuit* const* circuit
   return batch_expectation(reinterpret_cast<Batch*>(b), cs, ts, o, out);
 }
 
+NqpuError nqpu_batch_synchronize(NqpuBatch* b) {
+  if (!b) return NQPU_NULL_POINTER;
+  return batch_synchronize(reinterpret_cast<Batch*>(b));
+}
+
 NqpuError nqpu_debug_batch_plan(const NqpuCircuit* const* circuits, uint64_t count, const NqpuPauliTerm* terms,
                                 const uint64_t* offsets, const NqpuRunOptions* opts, NqpuBatchStats* out,
                                 uint64_t* program_ops) {

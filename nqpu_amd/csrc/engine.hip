@@ -3052,11 +3052,17 @@ NqpuError collapse(State* s, int q, uint32_t outcome, double f) {
 // ----------------------------------------------------------------- batches
 
 Batch::~Batch() {
-  if (stream) {
-    (void)hipStreamSynchronize(stream);
-    (void)hipStreamDestroy(stream);
+  for (hipStream_t st : {stream, copy_stream})
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+  for (BatchSlot& sl : slot) {
+    if (sl.pinned) (void)hipHostFree(sl.pinned);
+    if (sl.results) (void)hipHostFree(sl.results);
+    for (hipEvent_t e : {sl.uploaded, sl.k0, sl.k1, sl.done})
+      if (e) (void)hipEventDestroy(e);
   }
-  if (pinned) (void)hipHostFree(pinned);
 }
 
 NqpuError create_batch(Batch** out) {
@@ -3068,6 +3074,7 @@ NqpuError create_batch(Batch** out) {
   std::unique_ptr<Batch> b(new Batch());
   HIPTRY(hipSetDevice(0), "hipSetDevice");
   HIPTRY(new_stream(&b->stream), "hipStreamCreate");
+  HIPTRY(new_stream(&b->copy_stream), "hipStreamCreate");
   *out = b.release();
   return NQPU_OK;
 }
@@ -3379,29 +3386,56 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   return NQPU_OK;
 }
 
+// Complete a queued batch: wait for it, copy its results to the caller's
+// buffer, and record its statistics.
+static NqpuError retire(Batch* b, BatchSlot& sl) {
+  if (!sl.busy) return NQPU_OK;
+  sl.busy = false;
+  HIPTRY(hipEventSynchronize(sl.done), "batch");
+  float ms = 0;
+  if (sl.count) (void)hipEventElapsedTime(&ms, sl.k0, sl.k1);
+  if (sl.count) std::memcpy(sl.dest, sl.results, sl.count * sizeof(double));
+  sl.st.kernel_ms = ms;
+  b->stats = sl.st;
+  return NQPU_OK;
+}
+
+NqpuError batch_synchronize(Batch* b) {
+  // in the order the batches were queued
+  for (int i = 0; i < 2; ++i) {
+    BatchSlot& sl = b->slot[(b->next + i) & 1];
+    const NqpuError e = retire(b, sl);
+    if (e != NQPU_OK) return e;
+  }
+  return NQPU_OK;
+}
+
 NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuits,
                             const std::vector<std::vector<PauliTerm>>& terms, const NqpuRunOptions& o,
                             double* out) {
   using clock = std::chrono::steady_clock;
+  BatchSlot& sl = b->slot[b->next];
+  // the slot's previous batch (two batches ago) must be complete before its
+  // host image, device tables and results are reused
+  NqpuError e = retire(b, sl);
+  if (e != NQPU_OK) return e;
   const auto t0 = clock::now();
   BatchPlan plan;
-  // the tables are written straight into a pinned host buffer the batch keeps
-  auto pinned = [b](size_t bytes) -> char* {
-    if (bytes > b->pinned_bytes) {
-      if (b->pinned) (void)hipHostFree(b->pinned);
-      b->pinned = nullptr;
-      b->pinned_bytes = 0;
+  // the tables are written straight into the slot's pinned host buffer
+  auto pinned = [&sl](size_t bytes) -> char* {
+    if (bytes > sl.pinned_bytes) {
+      if (sl.pinned) (void)hipHostFree(sl.pinned);
+      sl.pinned = nullptr;
+      sl.pinned_bytes = 0;
       void* p = nullptr;
       if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
-      b->pinned = (char*)p;
-      b->pinned_bytes = bytes;
+      sl.pinned = (char*)p;
+      sl.pinned_bytes = bytes;
     }
-    return b->pinned;
+    return sl.pinned;
   };
-  HIPTRY(hipStreamSynchronize(b->stream), "sync before planning");  // the previous upload may still read the buffer
   NqpuError pe = plan_batch(circuits, terms, o, plan, pinned);
   if (pe != NQPU_OK) return pe;
-  const size_t oc = plan.oc, op_ = plan.op, og = plan.og, ot = plan.ot, ocnt = plan.ocnt, total = plan.total;
   NqpuBatchStats st = plan.st;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
@@ -3409,51 +3443,66 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   // one state slot per workgroup: a workgroup runs its circuits one after
   // another, so the HBM it needs is that of the largest circuit
   const uint64_t state_amps = plan.slot_amps * (uint64_t)std::max(grid, 1);
-  HIPTRY(hipStreamSynchronize(b->stream), "sync");
-  if (!b->tables.ensure(plan.dev_total) || !b->out.ensure(circuits.size() * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  const size_t nres = circuits.size();
+  // growing a device buffer frees memory a queued batch may still use
+  if ((state_amps * 8 > b->states.bytes || plan.dev_total > sl.tables.bytes || nres * 8 + 8 > sl.out.bytes)) {
+    if ((e = batch_synchronize(b)) != NQPU_OK) return e;
+    HIPTRY(hipStreamSynchronize(b->stream), "sync before batch buffer growth");
+    HIPTRY(hipStreamSynchronize(b->copy_stream), "sync before batch buffer growth");
+  }
+  if (!sl.tables.ensure(plan.dev_total) || !sl.out.ensure(nres * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
   if (state_amps && !b->states.ensure(state_amps * 8)) {
     set_error("batch states do not fit in device memory");
     return NQPU_OUT_OF_DEVICE_MEMORY;
   }
-  char* base = (char*)b->tables.p;
+  if (nres > sl.results_cap) {
+    if (sl.results) (void)hipHostFree(sl.results);
+    sl.results = nullptr;
+    sl.results_cap = 0;
+    HIPTRY(hipHostMalloc((void**)&sl.results, nres * sizeof(double)), "pinned batch results");
+    sl.results_cap = nres;
+  }
+  for (hipEvent_t* ev : {&sl.uploaded, &sl.done})
+    if (!*ev) HIPTRY(hipEventCreateWithFlags(ev, hipEventDisableTiming), "batch event");
+  for (hipEvent_t* ev : {&sl.k0, &sl.k1})
+    if (!*ev) HIPTRY(hipEventCreate(ev), "batch event");
+  char* base = (char*)sl.tables.p;
   // the head, then the phase tables and block matrices; the blocks' operand
-  // blobs are built on the device between them
-  HIPTRY(hipMemcpyAsync(base, plan.host, plan.head, hipMemcpyHostToDevice, b->stream), "batch table upload");
-  if (total > plan.head)
-    HIPTRY(hipMemcpyAsync(base + plan.dev_q, plan.host + plan.head, total - plan.head, hipMemcpyHostToDevice,
-                          b->stream), "batch operand upload");
+  // blobs are built on the device between them.  The copies run on their own
+  // stream, overlapping the previous batch's launch (the slot's own previous
+  // batch is complete: retired above); the launch waits for them.
+  HIPTRY(hipMemcpyAsync(base, plan.host, plan.head, hipMemcpyHostToDevice, b->copy_stream), "batch table upload");
+  if (plan.total > plan.head)
+    HIPTRY(hipMemcpyAsync(base + plan.dev_q, plan.host + plan.head, plan.total - plan.head, hipMemcpyHostToDevice,
+                          b->copy_stream), "batch operand upload");
+  HIPTRY(hipEventRecord(sl.uploaded, b->copy_stream), "batch upload event");
+  HIPTRY(hipStreamWaitEvent(b->stream, sl.uploaded, 0), "batch upload wait");
   if (plan.nblk) {
     launch_blob_split((const float2*)(base + plan.dev_m), (unsigned char*)(base + plan.head), plan.nblk, b->stream);
     HIPTRY(hipGetLastError(), "operand blob launch");
   }
-  HIPTRY(hipStreamSynchronize(b->stream), "batch table upload");
-  const auto t1 = clock::now();
-  hipEvent_t e0, e1;
-  HIPTRY(hipEventCreate(&e0), "event");
-  HIPTRY(hipEventCreate(&e1), "event");
-  (void)hipEventRecord(e0, b->stream);
+  HIPTRY(hipEventRecord(sl.k0, b->stream), "batch event");
   if (grid > 0)
-    launch_batch(b->states.p, plan.slot_amps, (const BatchCircuit*)(base + oc), (int)plan.ncirc, (const BatchPass*)(base + op_),
-                 (const RegOp<float>*)(base + og), (const BatchTerm*)(base + ot),
-                 (const unsigned char*)(base + plan.head), (unsigned*)(base + ocnt), (double*)b->out.p, grid,
-                 b->stream);
-  NqpuError le = hip_check(hipGetLastError(), "batch launch");
-  (void)hipEventRecord(e1, b->stream);
-  NqpuError se = hip_check(hipStreamSynchronize(b->stream), "batch");
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (le != NQPU_OK) return le;
-  if (se != NQPU_OK) return se;
-  if (!circuits.empty())
-    HIPTRY(copy_on(b->stream, out, b->out.p, circuits.size() * 8, hipMemcpyDeviceToHost), "batch results");
+    launch_batch(b->states.p, plan.slot_amps, (const BatchCircuit*)(base + plan.oc), (int)plan.ncirc,
+                 (const BatchPass*)(base + plan.op), (const RegOp<float>*)(base + plan.og),
+                 (const BatchTerm*)(base + plan.ot), (const unsigned char*)(base + plan.head),
+                 (unsigned*)(base + plan.ocnt), (double*)sl.out.p, grid, b->stream);
+  HIPTRY(hipGetLastError(), "batch launch");
+  HIPTRY(hipEventRecord(sl.k1, b->stream), "batch event");
+  if (nres)
+    HIPTRY(hipMemcpyAsync(sl.results, sl.out.p, nres * sizeof(double), hipMemcpyDeviceToHost, b->stream),
+           "batch results");
+  HIPTRY(hipEventRecord(sl.done, b->stream), "batch done event");
   st.circuits = circuits.size();
-  st.plan_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  st.kernel_ms = ms;
+  st.plan_ms = std::chrono::duration<double, std::milli>(clock::now() - t0).count();
   st.state_bytes = (double)state_amps * 8.0;
-  b->stats = st;
-  return NQPU_OK;
+  sl.st = st;
+  sl.dest = out;
+  sl.count = nres;
+  sl.busy = true;
+  b->next ^= 1;
+  if (o.flags & NQPU_RUN_ASYNC) return NQPU_OK;
+  return batch_synchronize(b);
 }
 
 // ------------------------------------------------------------------- RNG

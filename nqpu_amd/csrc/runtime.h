@@ -254,11 +254,28 @@ struct PauliTerm {
   uint64_t x, z;
   double coeff;
 };
-struct Batch {
-  hipStream_t stream = nullptr;
-  DeviceScratch states, tables, out;
+// One batch in flight: its tables on the host (pinned image) and device,
+// its results, and the events that mark its upload and completion.
+struct BatchSlot {
+  DeviceScratch tables, out;
   char* pinned = nullptr;  // pinned host image of the tables, reused across batches
   size_t pinned_bytes = 0;
+  double* results = nullptr;  // pinned host copy of the results
+  size_t results_cap = 0;
+  double* dest = nullptr;     // the caller's buffer, written when the batch is retired
+  size_t count = 0;
+  hipEvent_t uploaded = nullptr, k0 = nullptr, k1 = nullptr, done = nullptr;
+  bool busy = false;
+  NqpuBatchStats st{};
+};
+// Batched execution context: one stream, one state region, and two slots so
+// that the host plans batch k+1 while the device runs batch k.
+struct Batch {
+  hipStream_t stream = nullptr;
+  hipStream_t copy_stream = nullptr;  // table uploads: batch k+1's overlaps batch k's launch
+  DeviceScratch states;
+  BatchSlot slot[2];
+  int next = 0;  // slot of the next batch
   NqpuBatchStats stats{};
   ~Batch();
 };
@@ -282,9 +299,14 @@ struct BatchPlan {
 NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vector<std::vector<PauliTerm>>& terms,
                      const NqpuRunOptions& o, BatchPlan& plan, const std::function<char*(size_t)>& buffer);
 NqpuError create_batch(Batch** out);
+// With NQPU_RUN_ASYNC in o.flags the call returns once the batch is queued;
+// `out` is written when batch_synchronize returns (or a later batch needs
+// the slot), and must stay valid until then.
 NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuits,
                             const std::vector<std::vector<PauliTerm>>& terms, const NqpuRunOptions& o,
                             double* out);
+// Wait for every queued batch and write its results.
+NqpuError batch_synchronize(Batch* b);
 
 // The C ABI's thread-local xorshift64* stream (c_ffi.rs:48-67).
 // The C ABI's xorshift64* generator (sdk/rust/src/infra/bindings/c_ffi.rs:48-67):

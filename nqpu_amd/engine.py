@@ -533,6 +533,17 @@ def _term_table(observables):
     return table, np.array(offsets, dtype=np.uint64)
 
 
+class Observables:
+    """The Pauli sums of a batch in the library's term-table form
+    (NqpuPauliTerm rows and per-circuit offsets), built once and reused by
+    every Batch.expectation call that evaluates the same observables (the
+    observables' counterpart of NativeCircuit)."""
+
+    def __init__(self, observables):
+        self.table, self.offsets = _term_table(observables)
+        self.count = len(self.offsets) - 1
+
+
 class Batch:
     """Many complex64 circuits in one persistent GPU launch, each returning
     the FP64 expectation of its own Pauli sum (the batched counterpart of
@@ -540,20 +551,35 @@ class Batch:
 
     def __init__(self):
         lib = L.load()
+        self._pending = []
         self.ptr = lib.nqpu_batch_create()
         if not self.ptr:
             raise L.NqpuError(9, "nqpu_batch_create")
 
-    def expectation(self, circuits, observables, semantics="standard") -> np.ndarray:
+    def expectation(self, circuits, observables, semantics="standard", async_=False) -> np.ndarray:
+        """async_=True returns once the batch is queued: the returned array is
+        filled when synchronize() returns (the host plans the next batch
+        meanwhile)."""
         lib = L.load()
         natives = [c if isinstance(c, NativeCircuit) else c.native() for c in circuits]
         arr = (ct.c_void_p * max(1, len(natives)))(*[nc.ptr for nc in natives])
-        table, off = _term_table(observables)
+        if not isinstance(observables, Observables):
+            observables = Observables(observables)
+        if observables.count != len(natives):
+            raise ValueError(f"{observables.count} observables for {len(natives)} circuits")
+        table, off = observables.table, observables.offsets
         out = np.empty(max(1, len(natives)), dtype=np.float64)
-        o = run_options("tiled", semantics)
+        o = run_options("tiled", semantics, async_=async_)
         L.check(lib.nqpu_batch_expectation(self.ptr, arr, len(natives), table.ctypes.data, off.ctypes.data,
                                            ct.byref(o), out.ctypes.data), "nqpu_batch_expectation")
+        if async_:
+            self._pending.append(out)  # written by the library until synchronize()
+            self._pending = self._pending[-2:]
         return out[: len(natives)]
+
+    def synchronize(self) -> None:
+        L.check(L.load().nqpu_batch_synchronize(self.ptr), "nqpu_batch_synchronize")
+        self._pending = []
 
     def last_stats(self) -> dict:
         st = L.NqpuBatchStats()

@@ -402,6 +402,7 @@ extern "C" {
         opts: *const NqpuRunOptions,
         out: *mut f64,
     ) -> NqpuError;
+    pub fn nqpu_batch_synchronize(batch: *mut NqpuBatch) -> NqpuError;
     pub fn nqpu_batch_last_stats(batch: *const NqpuBatch, out: *mut NqpuBatchStats) -> NqpuError;
     pub fn nqpu_debug_batch_plan(
         circuits: *const *const NqpuCircuit,

@@ -102,3 +102,21 @@ def test_batch_argument_errors(batch):
         batch.expectation([Circuit(3).h(0)], [[(1.0, "ZII")]])
     with pytest.raises(RuntimeError):
         batch.expectation([Circuit(5).h(0)], [[(1.0, 0, 1 << 7)]])
+
+
+def test_queued_batches_equal_synchronous_batches(batch):
+    """Three batches queued with async_=True (the third waits for the first's
+    slot) give, after synchronize(), exactly the energies of the same batches
+    run one at a time; a larger batch in between grows the device buffers."""
+    sets = [variational_batch(k, seed=s, nmin=8, nmax=nmax) for k, s, nmax in ((30, 11, 16), (200, 12, 20), (17, 13, 14))]
+    want = [batch.expectation([v.circuit for v in vs], [v.hamiltonian() for v in vs]) for vs in sets]
+    got = [batch.expectation([v.circuit for v in vs], [v.hamiltonian() for v in vs], async_=True) for vs in sets]
+    batch.synchronize()
+    for k in range(3):
+        assert np.array_equal(got[k], want[k]), f"batch {k}"
+    # and again, now with every buffer already large enough
+    got = [batch.expectation([v.circuit for v in vs], [v.hamiltonian() for v in vs], async_=True) for vs in sets]
+    batch.synchronize()
+    for k in range(3):
+        assert np.array_equal(got[k], want[k]), f"batch {k} (second round)"
+    assert batch.last_stats()["circuits"] == 17
