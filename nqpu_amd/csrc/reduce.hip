@@ -137,6 +137,96 @@ __global__ __launch_bounds__(kPauliThreads) void k_pauli_group(const T* __restri
   }
 }
 
+// ------------------------------------------------ Pauli strings on tiles
+//
+// Strings with different X masks need a pass each in k_pauli_group.  When
+// their X bits all fit one tile of K <= 12 qubits (the transverse-field
+// terms X_i of a chain, for one), one pass over the state evaluates them all:
+// a workgroup loads a tile of 2^K amplitudes into LDS (the state's lowest
+// qubits are tile bits, so the loads are contiguous runs), and every pair
+// (i, i ^ x) of a string lies inside one tile.  The raw sum A_t is the same
+// as k_pauli_group's: the sign (-1)^popc(i & z) splits into the tile-local Z
+// bits and the Z bits outside the tile (one parity per tile).
+
+constexpr int kPauliTileMaxBits = 12;
+
+struct PauliTileArgs {
+  int32_t K, nt;
+  int32_t q[kPauliTileMaxBits];  // tile qubits, ascending
+  uint32_t xl[kPauliGroup];      // X/Y bits of each string, tile-local
+  uint32_t zl[kPauliGroup];      // Z/Y bits inside the tile, tile-local
+  uint64_t zg[kPauliGroup];      // Z/Y bits outside the tile
+  uint32_t odd;                  // bit t: string t has an odd count of bits in x & z
+  uint32_t pad;
+};
+
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_pauli_tile(const T* __restrict__ st, uint64_t ntiles, PauliTileArgs g,
+                                                    double* __restrict__ part) {
+  extern __shared__ unsigned char smraw[];
+  T* sm = reinterpret_cast<T*>(smraw);
+  const int tid = threadIdx.x, E = 1 << g.K;
+  // tile element e = tid + 256 r sits at global index base | dlo | dhi(r)
+  uint64_t dlo = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (j < g.K) dlo |= (uint64_t)((tid >> j) & 1) << g.q[j];
+  double acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint64_t base = tile;
+    for (int j = 0; j < g.K; ++j) base = ins0(base, g.q[j]);
+    for (int e0 = 0; e0 < E; e0 += 256) {
+      if (e0 + tid >= E) break;
+      uint64_t d = base | dlo;
+      for (int j = 8; j < g.K; ++j) d |= (uint64_t)((e0 >> j) & 1) << g.q[j];
+      sm[e0 + tid] = st[d];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (t >= g.nt) break;
+      const uint32_t x = g.xl[t], z = g.zl[t];
+      const int h = 31 - __clz(x);
+      const bool im = (g.odd >> t) & 1;
+      double s = 0.0;
+      for (int p = tid; p < E / 2; p += 256) {  // pairs (e, e ^ x) with bit h of e clear
+        const uint32_t e = (uint32_t)ins0((uint64_t)p, h), j = e ^ x;
+        const T ai = sm[e], aj = sm[j];
+        const double w = im ? (double)aj.x * ai.y - (double)aj.y * ai.x   // Im(conj(a_j) a_i)
+                            : (double)aj.x * ai.x + (double)aj.y * ai.y;  // Re(conj(a_j) a_i)
+        s += (__popc(e & z) & 1) ? -w : w;
+      }
+      acc[t] += (__popcll(base & g.zg[t]) & 1) ? -s : s;
+    }
+    __syncthreads();
+  }
+  __shared__ double red[NT][4];
+  const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const double w = wsum(acc[t]);
+    if (lane == 0) red[t][wave] = w;
+  }
+  __syncthreads();
+  if (tid < NT) part[(uint64_t)blockIdx.x * NT + tid] = (red[tid][0] + red[tid][1]) + (red[tid][2] + red[tid][3]);
+}
+
+template <typename T>
+void launch_pauli_tile(const T* st, uint64_t ntiles, const PauliTileArgs& g, int grid, int width, double* part,
+                       hipStream_t s) {
+  const dim3 gr(grid), bl(256);
+  const size_t lds = sizeof(T) << g.K;
+  switch (width) {
+    case 2: hipLaunchKernelGGL((k_pauli_tile<T, 2>), gr, bl, lds, s, st, ntiles, g, part); break;
+    case 4: hipLaunchKernelGGL((k_pauli_tile<T, 4>), gr, bl, lds, s, st, ntiles, g, part); break;
+    case 8: hipLaunchKernelGGL((k_pauli_tile<T, 8>), gr, bl, lds, s, st, ntiles, g, part); break;
+    case 16: hipLaunchKernelGGL((k_pauli_tile<T, 16>), gr, bl, lds, s, st, ntiles, g, part); break;
+    default: hipLaunchKernelGGL((k_pauli_tile<T, 32>), gr, bl, lds, s, st, ntiles, g, part); break;
+  }
+}
+
 // out[c] = sum over rows r of in[r * width + c] for c < cols: one workgroup
 // per column; thread t adds rows t, t + 256, ... in order, then a fixed tree.
 __global__ __launch_bounds__(256) void k_sum_columns(const double* __restrict__ in, int rows, int width, int cols,
@@ -184,7 +274,64 @@ int pow2_at_least(int v) {
 // kPauliMaxGrid * kPauliGroup doubles.
 NqpuError pauli_launch(const State* s, const uint64_t* xm, const uint64_t* zm, uint64_t count, double* part,
                        double* res) {
+  // tiles of 32 KiB of LDS: 2^12 complex64 or 2^11 complex128 amplitudes
+  const int K = std::min(s->n, s->prec == NQPU_PRECISION_C64 ? kPauliTileMaxBits : kPauliTileMaxBits - 1);
+  const uint64_t low = (1ull << std::min(s->n, 5)) - 1;  // the state's lowest qubits: contiguous tile loads
   for (uint64_t a = 0; a < count;) {
+    // strings with at least two different X masks whose X bits fit one tile
+    // (with the lowest qubits): one tile pass for all of them
+    if (xm[a] != 0) {
+      uint64_t qm = low, b = a, last = 0;
+      int distinct = 0;
+      while (b < count && b - a < (uint64_t)kPauliGroup) {
+        if (xm[b] != last) {
+          if (__builtin_popcountll(qm | xm[b]) > K) break;
+          qm |= xm[b];
+          last = xm[b];
+          ++distinct;
+        }
+        ++b;
+      }
+      if (distinct >= 2) {
+        for (int q = 0; __builtin_popcountll(qm) < K; ++q) qm |= 1ull << q;
+        PauliTileArgs g{};
+        g.K = K;
+        g.nt = (int32_t)(b - a);
+        int loc[64];
+        for (int q = 0, j = 0; q < 64; ++q) {
+          loc[q] = -1;
+          if ((qm >> q) & 1) {
+            g.q[j] = q;
+            loc[q] = j++;
+          }
+        }
+        auto local = [&](uint64_t m) {
+          uint32_t r = 0;
+          for (int q = 0; q < 64; ++q)
+            if (((m >> q) & 1) && loc[q] >= 0) r |= 1u << loc[q];
+          return r;
+        };
+        for (uint64_t t = a; t < b; ++t) {
+          g.xl[t - a] = local(xm[t]);
+          g.zl[t - a] = local(zm[t]);
+          g.zg[t - a] = zm[t] & ~qm;
+          if (__builtin_popcountll(xm[t] & zm[t]) & 1) g.odd |= 1u << (t - a);
+        }
+        const uint64_t ntiles = s->dim() >> K;
+        const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(ntiles, kPauliMaxGrid));
+        const int width = std::max(2, pow2_at_least(g.nt));
+        if (s->prec == NQPU_PRECISION_C64)
+          launch_pauli_tile((const float2*)s->amps, ntiles, g, grid, width, part, s->stream);
+        else
+          launch_pauli_tile((const double2*)s->amps, ntiles, g, grid, width, part, s->stream);
+        HIPTRY(hipGetLastError(), "pauli tile launch");
+        hipLaunchKernelGGL(k_sum_columns, dim3(g.nt), dim3(256), 0, s->stream, (const double*)part, grid, width,
+                           g.nt, res + a);
+        HIPTRY(hipGetLastError(), "pauli column sums");
+        a = b;
+        continue;
+      }
+    }
     uint64_t b = a;
     const uint64_t x = xm[a];
     while (b < count && xm[b] == x && b - a < (uint64_t)kPauliGroup) ++b;

@@ -3,7 +3,8 @@ oracle and numpy.
 
 * Overlap <a|b> for all four precision pairings against np.vdot (and
   |<a|b>|^2 against orc_fidelity), device copies bit for bit.
-* Grouped Pauli sums (one pass per distinct X mask) term by term against
+* Grouped Pauli sums (one pass per distinct X mask, or one tile pass for
+  strings whose X bits fit a tile) term by term against
   orc_expectation_pauli: complex128 within 1e-12, complex64 within 1e-6.
 * Pauli strings in both qubit orders and the Estimator's MSB terms against
   orc_pauli_expectation_msb (primitives.rs:683-728), repeated qubits too.
@@ -14,6 +15,8 @@ oracle and numpy.
 * A measurement whose drawn outcome has no measured mass (complex64 drift,
   here forced through an unnormalised state) collapses onto the other one.
 """
+import ctypes as ct
+
 import numpy as np
 import pytest
 
@@ -241,3 +244,42 @@ def test_measure_never_collapses_onto_a_massless_outcome(oracle):
         assert abs(p - 0.25) < 1e-7
         hits += r >= 0.25
     assert hits > 10  # the massless outcome was drawn, and avoided, many times
+
+
+@pytest.mark.parametrize("n,prec,tol", [(6, "c128", 1e-12), (11, "c64", 1e-6), (14, "c128", 1e-12),
+                                        (20, "c64", 1e-6), (20, "c128", 1e-12)])
+def test_strings_with_different_x_masks_in_one_tile(oracle, n, prec, tol):
+    """Strings whose X/Y bits fit one tile with the lowest qubits (every X_i of
+    a chain, neighbouring XX / XY / YZ pairs, Z factors inside and outside the
+    tile) share tile passes; each value must still be the oracle's, term by
+    term, and the no-allocation sum must agree."""
+    c = random_u3_brickwork(n, 8, 900 + n)
+    exp = oracle.run(n, c.gates)
+    terms = []
+    for i in range(n):
+        s = ["I"] * n
+        s[i] = "X"
+        terms.append((-0.7, "".join(s)))
+    for i in range(n - 1):
+        for a, b in (("X", "X"), ("X", "Y"), ("Y", "Z"), ("Y", "Y")):
+            s = ["I"] * n
+            s[i], s[i + 1] = a, b
+            if i + 3 < n:
+                s[n - 1 - (i % 2)] = "Z" if s[n - 1 - (i % 2)] == "I" else s[n - 1 - (i % 2)]
+            terms.append((0.3, "".join(s)))
+    with StateVector(n, prec) as sv:
+        sv.run(c)
+        got = sv.expectation_pauli_values([p for _, p in terms])
+        total = sv.expectation_pauli_sum(terms)
+        xs = np.array([E.pauli_masks(p)[0] for _, p in terms], dtype=np.uint64)
+        zs = np.array([E.pauli_masks(p)[1] for _, p in terms], dtype=np.uint64)
+        cs = np.array([cf for cf, _ in terms], dtype=np.float64)
+        v = ct.c_double()
+        lib = L.load()
+        L.check(lib.nqpu_expectation_pauli_sum_noalloc(sv.ptr, xs.ctypes.data, zs.ctypes.data, cs.ctypes.data,
+                                                       len(terms), ct.byref(v)), "noalloc")
+    want = np.array([oracle.expectation_pauli(exp, n, *E.pauli_masks(p)) for _, p in terms])
+    assert np.max(np.abs(got - want)) < tol
+    ref_total = float(np.dot(cs, want))
+    assert abs(total - ref_total) < 50 * tol
+    assert abs(v.value - ref_total) < 50 * tol
