@@ -220,9 +220,11 @@ typedef struct {
 /* register tiles (complex64: 12-qubit tiles; complex128: any width): run each
  * pass as its own straight-line kernel,
  * generated from the pass's register program and compiled with hipRTC on
- * first use (about a second per eight passes, on parallel threads); the
- * kernels depend only on the program's structure, not on gate angles, and
- * stay cached for the life of the process */
+ * first use (complex64: about a second per eight passes, complex128 several
+ * times that, on parallel threads); the kernels depend only on the program's
+ * structure, not on gate angles, and stay cached for the life of the process.
+ * Compiled complex128 passes move bits with register/lane exchanges instead
+ * of LDS remaps. */
 #define NQPU_RUN_COMPILE 16u
 /* complex64 register tiles: apply runs of diagonal gates one by one instead
  * of merging each run into one quadratic phase evaluated per amplitude */
@@ -539,8 +541,8 @@ NqpuError nqpu_server_stats(const NqpuServer* server, NqpuServerStats* out);
 /* ---- Plan autotuning ----
  * The reference's tuner targets Apple GPUs (backends/runtime/auto_tuning.rs);
  * this one times the engine's plan choices on this device: tile qubits, the
- * contiguous low qubits of every tile, compiled (complex64) or interpreted
- * pass kernels, fixed or moving low positions, and tiles per workgroup (the
+ * contiguous low qubits of every tile, compiled or interpreted pass
+ * kernels, fixed or moving low positions, and tiles per workgroup (the
  * grid).  The fastest becomes the default plan for states of that qubit
  * count and precision in this process (runs whose options leave tile_qubits,
  * low_qubits and those flags at zero, without NQPU_RUN_UNTUNED), and is

@@ -2395,7 +2395,9 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
   for (const Pass& p : passes) {
     if (reg) {
       emulate_reg_pass(a, c.n, p,
-                       build_reg_program<double>(p, K - 8, 8, po.mfma && c64, c64, 1 << 20, po.qphase && c64), K - 8);
+                       build_reg_program<double>(p, K - 8, 8, po.mfma && c64, c64 || po.compile, 1 << 20,
+                                                 po.qphase && c64),
+                       K - 8);
     } else {
       std::vector<DevOp<double>> dops;
       append_dev_ops<double>(p, dops);
@@ -2418,13 +2420,16 @@ static NqpuError prepare_reg(const std::vector<Pass>& passes, int n, int eb, boo
   std::vector<PassArgs> args;
   std::vector<size_t> blob_at;  // byte offset of each pass's operand blobs
   constexpr bool c64 = std::is_same<R, float>::value;
+  // lane exchanges: complex64, and complex128 compiled passes (the
+  // interpreting kernel keeps complex128 exchanges out)
+  const bool xchg = c64 || compile;
   std::vector<RegProgram<R>> progs(passes.size());
   {
     const size_t nt = std::min<size_t>(passes.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
     std::atomic<size_t> next{0};
     auto work = [&] {
       for (size_t i; (i = next.fetch_add(1)) < passes.size();)
-        progs[i] = build_reg_program<R>(passes[i], eb, 8, mfma && c64, c64, 1 << 20, qphase && c64);
+        progs[i] = build_reg_program<R>(passes[i], eb, 8, mfma && c64, xchg, 1 << 20, qphase && c64);
     };
     std::vector<std::thread> th;
     for (size_t t = 1; t < nt; ++t) th.emplace_back(work);
@@ -2654,7 +2659,7 @@ static NqpuError jit_check_circuit_t(const Circuit& c, const NqpuRunOptions& o) 
   std::vector<RegOp<R>> prog;
   std::vector<PassArgs> args;
   for (const Pass& p : plan_tiles(ops, c.n, po)) {
-    const RegProgram<R> pr = build_reg_program<R>(p, K - 8, 8, po.mfma && c64, c64, 1 << 20, po.qphase && c64);
+    const RegProgram<R> pr = build_reg_program<R>(p, K - 8, 8, po.mfma && c64, true, 1 << 20, po.qphase && c64);
     PassArgs pa{};
     pa.op0 = (int)prog.size();
     pa.nops = (int)pr.ops.size();
@@ -2743,7 +2748,7 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
     if (po.registers && K >= 11 && K <= 13)
       for (const Pass& p : passes) {
         const RegProgram<double> rp =
-            build_reg_program<double>(p, K - 8, 8, po.mfma && c64, c64, 1 << 20, po.qphase && c64);
+            build_reg_program<double>(p, K - 8, 8, po.mfma && c64, c64 || po.compile, 1 << 20, po.qphase && c64);
         for (const RegOp<double>& r : rp.ops) {
           st.remaps += r.kind == R_REMAP;
           st.exchanges += r.kind == R_XCHG;

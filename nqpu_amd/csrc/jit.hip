@@ -25,6 +25,7 @@
 #include <mutex>
 #include <sstream>
 #include <thread>
+#include <type_traits>
 
 #include "runtime.h"
 #include "jit_src.inc"  // kJitPrelude: integer typedefs + devtypes.h + regtile.h (generated by build.py)
@@ -75,7 +76,7 @@ void emit_op(std::ostringstream& o, const RegOp<R>& r, int k, int eb) {
   const char* TT = JitNames<R>::t;
   switch (r.kind) {
     case R_REMAP:
-      o << "    nqpu::reg_remap<" << RT << ", " << TT << ", " << eb << ">(v, sm, ops[" << k << "], tid, true);\n";
+      o << "    nqpu::reg_remap<" << RT << ", " << TT << ", " << eb << ">(v, sm, opv[" << k << "], tid, true);\n";
       return;
     case R_MFMA:
       o << "    { int var = 0;";
@@ -85,14 +86,14 @@ void emit_op(std::ostringstream& o, const RegOp<R>& r, int k, int eb) {
       return;
     case R_XCHG:
       if (r.s1 < 16)
-        o << "    nqpu::reg_xchg_lane<float2, " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, lane);\n";
+        o << "    nqpu::reg_xchg_lane<" << TT << ", " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, lane);\n";
       else
         o << "    nqpu::reg_xchg_reg<" << TT << ", " << eb << ", " << r.s0 << ", " << (r.s1 - 16) << ">(v);\n";
       return;
     case R_QPHASE: {  // chunk masks are structure: compile-time constants
       o << "    { const uint64_t qm[4] = {";
       for (int c = 0; c < 4; ++c) o << (c ? ", " : "") << (c < r.pad ? qphase_chunk(r, c) : 0) << "ull";
-      o << "}; nqpu::reg_qphase_n<float, float2, " << eb << ", " << r.pad << ">(v, ops[" << k
+      o << "}; nqpu::reg_qphase_n<float, float2, " << eb << ", " << r.pad << ">(v, opv[" << k
         << "], blob, tid, base, qm); }\n";
       return;
     }
@@ -102,15 +103,15 @@ void emit_op(std::ostringstream& o, const RegOp<R>& r, int k, int eb) {
   o << "    if ((base & " << r.gc << "ull) == " << r.gc << "ull) { const bool on = (tid & " << r.ct << "u) == "
     << r.ct << "u; (void)on; ";
   switch (r.kind) {
-    case R_U1: o << "nqpu::reg_u1<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ">(v, ops[" << k << "], on);"; break;
-    case R_FLIP: o << "nqpu::reg_flip<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ">(v, ops[" << k << "], on);"; break;
+    case R_U1: o << "nqpu::reg_u1<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ">(v, opv[" << k << "], on);"; break;
+    case R_FLIP: o << "nqpu::reg_flip<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ">(v, opv[" << k << "], on);"; break;
     case R_D1:
-    case R_D2: o << "nqpu::reg_diag<" << RT << ", " << TT << ", " << eb << ">(v, ops[" << k << "], tid, on, base);"; break;
+    case R_D2: o << "nqpu::reg_diag<" << RT << ", " << TT << ", " << eb << ">(v, opv[" << k << "], tid, on, base);"; break;
     case R_U2:
-      o << "nqpu::reg_u2<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, ops[" << k << "], on);";
+      o << "nqpu::reg_u2<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, opv[" << k << "], on);";
       break;
     case R_SWAP:
-      o << "nqpu::reg_swap<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, ops[" << k << "], on);";
+      o << "nqpu::reg_swap<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ", " << r.s1 << ">(v, opv[" << k << "], on);";
       break;
     default: break;
   }
@@ -130,6 +131,14 @@ std::string kernel_source(int idx, const std::vector<RegOp<R>>& prog, const Pass
   o << "  nqpu::reg_tile_pass<" << RT << ", " << TT << ", " << eb << ">(st, pa, ntiles, [&](" << TT << " (&v)["
     << (1 << eb) << "], " << TT << "* sm, uint64_t base, int tid) __attribute__((always_inline)) {\n";
   o << "    const int lane = tid & 63; (void)lane; (void)sm; (void)ops; (void)blob;\n";
+  if (std::is_same<R, double>::value)
+    // the ops' operands (FP64 matrices) are read where they are used: an op
+    // pointer that depends on the tile (base >> 63 is 0) keeps the compiler
+    // from hoisting every op's scalar loads out of the tile loop, which
+    // overflows the scalar registers into vector-register lanes
+    o << "    const nqpu::RegOp<double>* opv = ops + (base >> 63);\n";
+  else
+    o << "    const nqpu::RegOp<float>* opv = ops;\n";
   for (int k = 0; k < nops; ++k) emit_op<R>(o, prog[op0 + k], k, eb);
   o << "  });\n}\n";
   return o.str();

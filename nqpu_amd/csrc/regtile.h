@@ -487,8 +487,8 @@ __device__ __forceinline__ void reg_mfma(T (&v)[1 << EB], const unsigned char* b
 // partner lane l ^ 2^B with DPP (quad_perm xor for B <= 1, row_shr/row_shl 4
 // for B = 2, row_ror 8 for B = 3).
 template <int B>
-__device__ __forceinline__ void xchg_component(float& lo, float& hi, bool y) {
-  unsigned a = __float_as_uint(lo), b = __float_as_uint(hi);
+__device__ __forceinline__ void xchg_bits(unsigned& lo, unsigned& hi, bool y) {
+  unsigned a = lo, b = hi;
   if constexpr (B == 5) {
     const auto p = __builtin_amdgcn_permlane32_swap(a, b, false, false);
     a = p[0];
@@ -521,13 +521,34 @@ __device__ __forceinline__ void xchg_component(float& lo, float& hi, bool y) {
     a = na;
     b = nb;
   }
+  lo = a;
+  hi = b;
+}
+
+template <int B>
+__device__ __forceinline__ void xchg_component(float& lo, float& hi, bool y) {
+  unsigned a = __float_as_uint(lo), b = __float_as_uint(hi);
+  xchg_bits<B>(a, b, y);
   lo = __uint_as_float(a);
   hi = __uint_as_float(b);
 }
 
+// A double component moves as its two 32-bit halves.
+template <int B>
+__device__ __forceinline__ void xchg_component(double& lo, double& hi, bool y) {
+  const uint64_t a = (uint64_t)__double_as_longlong(lo), b = (uint64_t)__double_as_longlong(hi);
+  unsigned a0 = (unsigned)a, a1 = (unsigned)(a >> 32), b0 = (unsigned)b, b1 = (unsigned)(b >> 32);
+  xchg_bits<B>(a0, b0, y);
+  xchg_bits<B>(a1, b1, y);
+  lo = __longlong_as_double((long long)(((uint64_t)a1 << 32) | a0));
+  hi = __longlong_as_double((long long)(((uint64_t)b1 << 32) | b0));
+}
+
+// complex64 and complex128 (compiled passes: the interpreting kernel keeps
+// complex128 exchanges out, see reg_program)
 template <typename T, int EB, int J, int B>
 __device__ __forceinline__ void reg_xchg_lane(T (&v)[1 << EB], int lane) {
-  if constexpr (nqpu_is_float2<T>::value && J < EB && B < 6) {
+  if constexpr (J < EB && B < 6) {
     const bool y = (lane >> B) & 1;
 #pragma unroll
     for (int r0 = 0; r0 < (1 << EB); ++r0) {

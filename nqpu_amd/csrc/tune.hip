@@ -131,10 +131,12 @@ std::vector<Candidate> candidates(int n, int prec) {
     if (small) v.push_back({10, 5, 0, 1});
   } else {
     const bool small = n <= 20;
-    // (compiled complex128 passes need about 200 VGPRs, two waves per SIMD,
-    // and measured 40 ms per 30-qubit pass against 24 ms interpreted on
-    // MI355X: not candidates)
     v.push_back({small ? 10 : 11, 4, 0, 1});
+    // compiled complex128 passes move bits with register/lane exchanges
+    // instead of LDS remaps (30-qubit brickwork on MI355X: 548 ms against
+    // 781 ms interpreted, after 22 s of run-time compilation)
+    v.push_back({11, 4, NQPU_RUN_COMPILE, 1});
+    v.push_back({12, 4, NQPU_RUN_COMPILE, 1});
     v.push_back({11, 4, 0, 2});
     v.push_back({11, 5, 0, 1});
     v.push_back({12, 4, 0, 1});

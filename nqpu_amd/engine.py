@@ -133,13 +133,13 @@ class NativeCircuit:
         return _stats(st)
 
     def emulate(self, amps, precision="c64", semantics="standard", tile_qubits=0, low_qubits=0,
-                lds_tiles=False, batch_layout=False, mfma=True, qphase=True, fixed_low=False):
+                lds_tiles=False, batch_layout=False, mfma=True, qphase=True, fixed_low=False, compile=False):
         """Host emulation of the tiled GPU execution (plan, per-pass programs
         and kernel semantics) in FP64; returns the new amplitudes.  With
         batch_layout, emulates the layout the batched kernel uses."""
         a = np.ascontiguousarray(amps, dtype=np.complex128).copy()
         o = run_options("tiled", semantics, tile_qubits, low_qubits, lds_tiles=lds_tiles, mfma=mfma,
-                        qphase=qphase, fixed_low=fixed_low)
+                        qphase=qphase, fixed_low=fixed_low, compile=compile)
         if batch_layout:
             o.flags |= L.RUN_BATCH_LAYOUT
         L.check(L.load().nqpu_debug_emulate(self.ptr, ct.byref(o), _prec(precision), a.ctypes.data, len(a)),

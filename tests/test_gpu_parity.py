@@ -360,15 +360,17 @@ def test_async_runs_queue_and_match_sync_runs(oracle):
 
 @pytest.mark.parametrize("tile", [11, 12])
 def test_compiled_complex128_passes_match_oracle(oracle, c1, tile):
-    """complex128 register tile passes as compiled straight-line kernels,
-    against the oracle to 1e-12 and bit for bit against the interpreting
-    kernel (the same plan and the same arithmetic)."""
+    """complex128 register tile passes as compiled straight-line kernels
+    (which move bits with register/lane exchanges; the interpreting kernel
+    uses LDS remaps) against the oracle to 1e-12, and against the
+    interpreting kernel's result."""
     c, exp = c1
     with StateVector(20, "c128") as sv:
-        sv.run(c, tile_qubits=tile, compile=True)
+        st = sv.run(c, tile_qubits=tile, compile=True)
         out = sv.amplitudes()
         sv.reset()
         sv.run(c, tile_qubits=tile, compile=False)
         interp = sv.amplitudes()
+    assert st["exchanges"] > 0
     assert np.max(np.abs(out - exp)) < 1e-12
-    assert np.array_equal(out, interp)
+    assert np.max(np.abs(out - interp)) < 1e-13

@@ -62,6 +62,23 @@ def test_emulated_c128_tiles_match_oracle(oracle, name):
     assert np.max(np.abs(got - exp)) < 1e-12
 
 
+@pytest.mark.parametrize("name", ["mixed13", "brick14", "mixed16", "qft_trotter15"])
+@pytest.mark.parametrize("tile", [11, 12])
+def test_emulated_c128_compiled_exchanges_match_oracle(oracle, name, tile):
+    """complex128 compiled passes move bits with register/lane exchanges
+    instead of LDS remaps (and store in the layout the program ends in)."""
+    c = CIRCUITS[name]()
+    n = c.num_qubits
+    a0 = random_state(n, 13)
+    exp = a0.copy()
+    oracle.apply_circuit(exp, n, c.gates)
+    nc = c.native()
+    got = nc.emulate(a0, "c128", tile_qubits=tile, compile=True)
+    assert np.max(np.abs(got - exp)) < 1e-12
+    st = nc.plan("c128", tile_qubits=tile, compile=True)
+    assert st["exchanges"] > 0 and st["remaps"] < nc.plan("c128", tile_qubits=tile)["remaps"]
+
+
 def test_diagonal_gates_do_not_need_tile_bits():
     """A QFT's controlled phases and a Trotter step's RZZ are diagonal, so a
     pass needs only the Hadamard/RX targets as tile bits: a 30-qubit QFT fits
