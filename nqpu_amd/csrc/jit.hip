@@ -103,8 +103,14 @@ void emit_op(std::ostringstream& o, const RegOp<R>& r, int k, int eb) {
   o << "    if ((base & " << r.gc << "ull) == " << r.gc << "ull) { const bool on = (tid & " << r.ct << "u) == "
     << r.ct << "u; (void)on; ";
   switch (r.kind) {
-    case R_U1: o << "nqpu::reg_u1<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ">(v, opv[" << k << "], on);"; break;
-    case R_FLIP: o << "nqpu::reg_flip<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ">(v, opv[" << k << "], on);"; break;
+    case R_U1:
+      o << "nqpu::reg_u1<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ", "
+        << ((r.ct == 0 && r.cr == 0) ? 1 : 0) << ">(v, opv[" << k << "], on);";
+      break;
+    case R_FLIP:
+      o << "nqpu::reg_flip<" << RT << ", " << TT << ", " << eb << ", " << r.s0 << ", " << (r.ct == 0 ? 1 : 0)
+        << ">(v, opv[" << k << "], on);";
+      break;
     case R_D1:
     case R_D2: o << "nqpu::reg_diag<" << RT << ", " << TT << ", " << eb << ">(v, opv[" << k << "], tid, on, base);"; break;
     case R_U2:

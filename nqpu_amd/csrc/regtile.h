@@ -86,12 +86,14 @@ __device__ __forceinline__ void store_stream(T* p, const T& v) {
 
 __device__ __forceinline__ uint32_t lds_swz(uint32_t t) { return t ^ (((t >> 5) ^ (t >> 10)) & 31u); }
 
-template <typename R, typename T, int EB, int J>
+// PLAIN: 1 = the op has no controls inside the tile, 0 = it has some, -1 =
+// read from the op (the interpreting kernel); compiled passes know it.
+template <typename R, typename T, int EB, int J, int PLAIN = -1>
 __device__ __forceinline__ void reg_u1(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
   const R m0r = op.m[0], m0i = op.m[1], m1r = op.m[2], m1i = op.m[3];
   const R m2r = op.m[4], m2i = op.m[5], m3r = op.m[6], m3i = op.m[7];
-  const uint32_t cr = op.cr;
-  if (op.ct == 0 && cr == 0) {  // uncontrolled: straight-line math
+  const uint32_t cr = PLAIN == 1 ? 0u : op.cr;
+  if (PLAIN < 0 ? (op.ct == 0 && cr == 0) : PLAIN == 1) {  // uncontrolled: straight-line math
 #pragma unroll
     for (int r0 = 0; r0 < (1 << EB); ++r0) {
       if (r0 & (1 << J)) continue;
@@ -115,10 +117,11 @@ __device__ __forceinline__ void reg_u1(T (&v)[1 << EB], const RegOp<R>& op, bool
   }
 }
 
-template <typename R, typename T, int EB, int J>
+// PLAIN: 1 = no controls on thread slots, 0 = some, -1 = read from the op.
+template <typename R, typename T, int EB, int J, int PLAIN = -1>
 __device__ __forceinline__ void reg_flip(T (&v)[1 << EB], const RegOp<R>& op, bool on) {
   const uint32_t cr = op.cr;
-  if (op.ct == 0) {  // controls only on register slots: a uniform permutation
+  if (PLAIN < 0 ? op.ct == 0 : PLAIN == 1) {  // controls only on register slots: a uniform permutation
 #pragma unroll
     for (int r0 = 0; r0 < (1 << EB); ++r0) {
       if (r0 & (1 << J)) continue;
