@@ -792,8 +792,20 @@ def main():
         log("accuracy run: complex128 state of the same circuit")
         ref = E.StateVector(n, "c128")
         t128 = time.perf_counter()
-        ref.run(nc, **dict(run_kw, compile=False))  # interpreted complex128 passes (faster than compiled)
+        ref.run(nc, **dict(run_kw, compile=False))  # interpreted: no run-time compilation for a single run
         result["c128_ms_per_step"] = (time.perf_counter() - t128) * 1e3
+        # compiled complex128 passes (register/lane exchanges): a run that
+        # compiles the pass kernels, then a timed run
+        tcc = time.perf_counter()
+        ref.reset()
+        ref.run(nc, fusion="tiled", compile=True, tile_qubits=11)
+        ccompile = time.perf_counter() - tcc
+        ref.reset()
+        tcc = time.perf_counter()
+        stc = ref.run(nc, fusion="tiled", compile=True, tile_qubits=11, time_passes=True)
+        result["c128_compiled"] = {"ms_per_step": (time.perf_counter() - tcc) * 1e3, "first_run_s": ccompile,
+                                   "passes": stc["passes"], "avg_pass_ms": stc["pass_ms"] / max(stc["timed_passes"], 1),
+                                   "exchanges": stc["exchanges"], "remaps": stc["remaps"]}
         sv.reset()
         sv.run(nc, **run_kw)
         ov = sv.overlap(ref)
