@@ -419,9 +419,12 @@ typedef struct {
   uint64_t gate_passes;     /* tile passes over HBM-resident states */
   uint64_t observable_passes;
   double plan_ms;           /* host planning and queueing (the upload is asynchronous) */
-  double kernel_ms;         /* the persistent launch */
+  double kernel_ms;         /* the persistent launch and the launches of split circuits */
   double state_bytes;       /* HBM for the states of circuits above 12 qubits: one slot of the
-                               largest such circuit per workgroup (nqpu_debug_batch_plan: one slot) */
+                               largest such circuit below 18 qubits per workgroup of the persistent
+                               launch (nqpu_debug_batch_plan: one slot), plus one state for each
+                               circuit of 18 qubits or more, which runs its passes as grids of
+                               work items beside the persistent launch */
   double hbm_bytes;         /* algorithmic state traffic of the launch: sum over circuits above 12
                                qubits of (2 x gate passes - 1 + observable passes) x 2^n x 8 bytes
                                (the first gate pass only writes) */

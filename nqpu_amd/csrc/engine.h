@@ -69,6 +69,21 @@ struct BatchCircuit {
   int32_t out;           // output slot
   int32_t pad;
   uint64_t amps;         // amplitudes of the HBM state (not resident)
+  uint64_t off;          // split circuits: offset of its own state in the split-state region
+};
+
+// Circuits of at least kBatchSplitQubits qubits are "split": their passes
+// run as ordinary grids, one launch per pass index over every split
+// circuit, each workgroup taking kBatchSplitTiles consecutive tiles of one
+// circuit's pass (instead of one workgroup streaming a whole circuit, which
+// made the largest circuits the tail of the persistent launch).
+constexpr int kBatchSplitQubits = 18;
+constexpr int kBatchSplitTiles = 8;
+
+// A unit of a split circuit's work: `ntile` consecutive tiles of gate pass
+// or observable pass `pass` of circuit `circ` (table index).
+struct BatchItem {
+  int32_t circ, pass, tile0, ntile;
 };
 
 struct BatchPass {
@@ -94,6 +109,17 @@ struct BatchTerm {
 void launch_batch(void* states, uint64_t slot_amps, const BatchCircuit* circs, int ncirc, const BatchPass* passes,
                   const RegOp<float>* prog, const BatchTerm* terms, const unsigned char* blob, unsigned* counter,
                   double* out, int grid, hipStream_t s);
+// Split circuits: one gate pass index over `nitems` items (states: the
+// split-state region); all observable passes into part[item]; out[C.out] =
+// the sum of each split circuit's parts, in item order (first[k], count[k]
+// for split circuit k of the table's split circuits circ0, circ0 + 1, ...).
+void launch_batch_split_gate(void* states, const BatchCircuit* circs, const BatchPass* passes,
+                             const RegOp<float>* prog, const unsigned char* blob, const BatchItem* items,
+                             int nitems, hipStream_t s);
+void launch_batch_split_obs(const void* states, const BatchCircuit* circs, const BatchPass* passes,
+                            const BatchTerm* terms, const BatchItem* items, int nitems, double* part, hipStream_t s);
+void launch_batch_split_out(const BatchCircuit* circs, int circ0, int nsplit, const int32_t* first,
+                            const int32_t* count, const double* part, double* out, hipStream_t s);
 
 
 // Operand blobs of nblk matrix-core blocks from their complex64 matrices

@@ -3291,7 +3291,8 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
     C.red0 += (int32_t)bp_at[k];
     if (!C.resident) {
       C.amps = cp.amps;
-      slot_amps = std::max(slot_amps, cp.amps);
+      // split circuits have their own state (offset assigned below)
+      if (C.n < kBatchSplitQubits) slot_amps = std::max(slot_amps, cp.amps);
       st.gate_passes += C.npass;
       st.observable_passes += C.nred;
       // gate passes write the state and read it (all but the first, which
@@ -3302,17 +3303,62 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
     }
     bc[k] = C;
   }
-  // longest circuits first: the queue hands them out before the short ones
+  // The persistent kernel's queue: circuits below kBatchSplitQubits, longest
+  // first (handed out before the short ones); then the split circuits, whose
+  // passes run as grids of work items.
+  auto split = [&](size_t k) { return !bc[k].resident && bc[k].n >= kBatchSplitQubits; };
   std::vector<size_t> order(nc);
   for (size_t i = 0; i < nc; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cps[x].cost > cps[y].cost; });
-  // one table image: [circuits][passes][program][terms][counter]
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    if (split(x) != split(y)) return !split(x);
+    return cps[x].cost > cps[y].cost;
+  });
+  size_t nqueue = 0;
+  while (nqueue < nc && !split(order[nqueue])) ++nqueue;
+  // split circuits: own states, gate items per pass index, observable items
+  uint64_t split_amps = 0;
+  int max_pass = 0;
+  for (size_t i = nqueue; i < nc; ++i) {
+    BatchCircuit& C = bc[order[i]];
+    C.off = split_amps;
+    split_amps += C.amps;
+    max_pass = std::max(max_pass, (int)C.npass);
+  }
+  auto chunks = [](int n, int K, int32_t circ, int32_t pass, std::vector<BatchItem>& v) {
+    const int64_t ntiles = 1ll << (n - K);
+    for (int64_t t = 0; t < ntiles; t += kBatchSplitTiles)
+      v.push_back(BatchItem{circ, pass, (int32_t)t, (int32_t)std::min<int64_t>(kBatchSplitTiles, ntiles - t)});
+  };
+  std::vector<BatchItem> items;
+  std::vector<std::pair<size_t, size_t>> gate_items;  // per pass index: (first item, count)
+  for (int p = 0; p < max_pass; ++p) {
+    const size_t at = items.size();
+    for (size_t i = nqueue; i < nc; ++i) {
+      const size_t k = order[i];
+      if (p < bc[k].npass) chunks(bc[k].n, cps[k].bp[p].K, (int32_t)i, p, items);
+    }
+    gate_items.push_back({at, items.size() - at});
+  }
+  const size_t obs_at = items.size();
+  std::vector<int32_t> ofirst, ocount;
+  for (size_t i = nqueue; i < nc; ++i) {
+    const size_t k = order[i];
+    const size_t at = items.size();
+    for (int rp = 0; rp < bc[k].nred; ++rp) chunks(bc[k].n, cps[k].bp[bc[k].npass + rp].K, (int32_t)i, rp, items);
+    ofirst.push_back((int32_t)(at - obs_at));
+    ocount.push_back((int32_t)(items.size() - at));
+  }
+  const size_t nsplit = nc - nqueue;
+  // one table image: [circuits][passes][program][terms][counter][items][first][count]
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
   const size_t oc = 0, op_ = up(oc + nc * sizeof(BatchCircuit));
   const size_t og = up(op_ + bp_at[nc] * sizeof(BatchPass));
   const size_t ot = up(og + pg_at[nc] * sizeof(RegOp<float>));
   const size_t ocnt = up(ot + bt_at[nc] * sizeof(BatchTerm));
-  const size_t head = ocnt + 256;  // operand units follow, 256-byte aligned
+  const size_t oit = ocnt + 256;
+  const size_t ofi = up(oit + items.size() * sizeof(BatchItem));
+  const size_t ocn = up(ofi + nsplit * sizeof(int32_t));
+  const size_t head = up(ocn + nsplit * sizeof(int32_t));  // operand units follow, 256-byte aligned
   const size_t nblk = bl_at[nc], nq = qt_at[nc];
   const size_t hq = head, hm = hq + nq * kMfmaBlobBytes;  // host image: phase tables, matrices
   const size_t total = hm + nblk * kMfmaMatrixBytes;
@@ -3328,6 +3374,11 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   for (size_t i = 0; i < nc; ++i)
     std::memcpy(host + oc + i * sizeof(BatchCircuit), &bc[order[i]], sizeof(BatchCircuit));
   std::memset(host + ocnt, 0, 256);  // the work-queue counter starts at zero
+  if (!items.empty()) std::memcpy(host + oit, items.data(), items.size() * sizeof(BatchItem));
+  if (nsplit) {
+    std::memcpy(host + ofi, ofirst.data(), nsplit * sizeof(int32_t));
+    std::memcpy(host + ocn, ocount.data(), nsplit * sizeof(int32_t));
+  }
   {
     std::atomic<size_t> next{0};
     auto fill = [&] {
@@ -3384,9 +3435,16 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   plan.total = total;
   plan.slot_amps = slot_amps;
   plan.ncirc = nc;
+  plan.nqueue = nqueue;
+  plan.split_amps = split_amps;
+  plan.oit = oit;
+  plan.ofirst = ofi;
+  plan.ocount = ocn;
+  plan.gate_items = gate_items;
+  plan.obs_items = {obs_at, items.size() - obs_at};
   plan.program_ops = pg_at[nc];
   st.circuits = circuits.size();
-  st.state_bytes = (double)slot_amps * 8.0;
+  st.state_bytes = (double)(slot_amps + split_amps) * 8.0;
   plan.st = st;
   return NQPU_OK;
 }
@@ -3444,18 +3502,22 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   NqpuBatchStats st = plan.st;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
-  const int grid = (int)std::min<size_t>(circuits.size(), (size_t)cus * 3);  // 3 workgroups fit per CU
-  // one state slot per workgroup: a workgroup runs its circuits one after
-  // another, so the HBM it needs is that of the largest circuit
-  const uint64_t state_amps = plan.slot_amps * (uint64_t)std::max(grid, 1);
+  const int grid = (int)std::min<size_t>(plan.nqueue, (size_t)cus * 3);  // 3 workgroups fit per CU
+  // one state slot per workgroup: a workgroup runs its queued circuits one
+  // after another, so the HBM it needs is that of the largest one; then the
+  // split circuits' own states
+  const uint64_t slots_amps = plan.slot_amps * (uint64_t)std::max(grid, 1);
+  const uint64_t state_amps = slots_amps + plan.split_amps;
   const size_t nres = circuits.size();
+  const size_t nobs = plan.obs_items.second;
+  const size_t out_doubles = ((nres + 31) & ~size_t(31)) + nobs;  // results, then observable item parts
   // growing a device buffer frees memory a queued batch may still use
-  if ((state_amps * 8 > b->states.bytes || plan.dev_total > sl.tables.bytes || nres * 8 + 8 > sl.out.bytes)) {
+  if ((state_amps * 8 > b->states.bytes || plan.dev_total > sl.tables.bytes || out_doubles * 8 + 8 > sl.out.bytes)) {
     if ((e = batch_synchronize(b)) != NQPU_OK) return e;
     HIPTRY(hipStreamSynchronize(b->stream), "sync before batch buffer growth");
     HIPTRY(hipStreamSynchronize(b->copy_stream), "sync before batch buffer growth");
   }
-  if (!sl.tables.ensure(plan.dev_total) || !sl.out.ensure(nres * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
+  if (!sl.tables.ensure(plan.dev_total) || !sl.out.ensure(out_doubles * 8 + 8)) return NQPU_OUT_OF_DEVICE_MEMORY;
   if (state_amps && !b->states.ensure(state_amps * 8)) {
     set_error("batch states do not fit in device memory");
     return NQPU_OUT_OF_DEVICE_MEMORY;
@@ -3487,12 +3549,37 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
     HIPTRY(hipGetLastError(), "operand blob launch");
   }
   HIPTRY(hipEventRecord(sl.k0, b->stream), "batch event");
+  const BatchCircuit* dcirc = (const BatchCircuit*)(base + plan.oc);
+  const BatchPass* dpass = (const BatchPass*)(base + plan.op);
+  const RegOp<float>* dprog = (const RegOp<float>*)(base + plan.og);
+  const BatchTerm* dterm = (const BatchTerm*)(base + plan.ot);
+  const unsigned char* dblob = (const unsigned char*)(base + plan.head);
+  const BatchItem* ditems = (const BatchItem*)(base + plan.oit);
   if (grid > 0)
-    launch_batch(b->states.p, plan.slot_amps, (const BatchCircuit*)(base + plan.oc), (int)plan.ncirc,
-                 (const BatchPass*)(base + plan.op), (const RegOp<float>*)(base + plan.og),
-                 (const BatchTerm*)(base + plan.ot), (const unsigned char*)(base + plan.head),
+    launch_batch(b->states.p, plan.slot_amps, dcirc, (int)plan.nqueue, dpass, dprog, dterm, dblob,
                  (unsigned*)(base + plan.ocnt), (double*)sl.out.p, grid, b->stream);
   HIPTRY(hipGetLastError(), "batch launch");
+  // split circuits: one launch per gate pass index, then every observable
+  // pass, then their sums
+  float2* split_states = (float2*)b->states.p + slots_amps;
+  for (const auto& gi : plan.gate_items)
+    if (gi.second) {
+      launch_batch_split_gate(split_states, dcirc, dpass, dprog, dblob, ditems + gi.first, (int)gi.second,
+                              b->stream);
+      HIPTRY(hipGetLastError(), "split circuit pass launch");
+    }
+  const int nsplit = (int)(plan.ncirc - plan.nqueue);
+  if (nsplit) {
+    double* part = (double*)sl.out.p + ((nres + 31) & ~size_t(31));
+    if (nobs) {
+      launch_batch_split_obs(split_states, dcirc, dpass, dterm, ditems + plan.obs_items.first, (int)nobs, part,
+                             b->stream);
+      HIPTRY(hipGetLastError(), "split circuit observable launch");
+    }
+    launch_batch_split_out(dcirc, (int)plan.nqueue, nsplit, (const int32_t*)(base + plan.ofirst),
+                           (const int32_t*)(base + plan.ocount), part, (double*)sl.out.p, b->stream);
+    HIPTRY(hipGetLastError(), "split circuit result launch");
+  }
   HIPTRY(hipEventRecord(sl.k1, b->stream), "batch event");
   if (nres)
     HIPTRY(hipMemcpyAsync(sl.results, sl.out.p, nres * sizeof(double), hipMemcpyDeviceToHost, b->stream),

@@ -607,6 +607,140 @@ void launch_batch(void* states, uint64_t slot_amps, const BatchCircuit* circs, i
                      passes, prog, terms, blob, counter, out);
 }
 
+// Tiles [tile0, tile0 + ntile) of gate pass p of a batched circuit over its
+// state at st: load (pass 0 starts from |0...0>), run the pass's register
+// program, store.  The same steps as the persistent kernel's passes.
+template <int EB>
+__device__ __forceinline__ void batch_gate_tiles(float2* __restrict__ st, const BatchPass& P, bool first,
+                                                 uint64_t tile0, uint64_t ntile, const RegOp<float>* prog,
+                                                 const unsigned char* blob, float2* sm, int tid) {
+  using T = float2;
+  constexpr int E = 1 << EB;
+  uint64_t dlo = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) dlo |= (uint64_t)((tid >> j) & 1) << P.q[j];
+  for (uint64_t tile = tile0; tile < tile0 + ntile; ++tile) {
+    uint64_t base = tile;
+#pragma unroll
+    for (int j = 0; j < 8 + EB; ++j) base = ins0(base, P.q[j]);
+    T v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      uint64_t d = base | dlo;
+#pragma unroll
+      for (int j = 0; j < EB; ++j)
+        if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
+      if (first) {
+        v[r].x = d == 0 ? 1.f : 0.f;
+        v[r].y = 0.f;
+      } else {
+        v[r] = st[d];
+      }
+      if (P.scaled) {
+        v[r].x *= kMfmaScale;
+        v[r].y *= kMfmaScale;
+      }
+    }
+    reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true, base,
+                              blob + (size_t)P.blob_unit * kMfmaBlobBytes);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      uint64_t d = base | dlo;
+#pragma unroll
+      for (int j = 0; j < EB; ++j)
+        if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
+      if (P.scaled) {
+        v[r].x *= 1.0f / kMfmaScale;
+        v[r].y *= 1.0f / kMfmaScale;
+      }
+      st[d] = v[r];
+    }
+  }
+}
+
+template <int EB>
+__global__ __launch_bounds__(256, 1) void k_batch_split_gate(float2* __restrict__ states,
+                                                             const BatchCircuit* __restrict__ circs,
+                                                             const BatchPass* __restrict__ passes,
+                                                             const RegOp<float>* __restrict__ prog,
+                                                             const unsigned char* __restrict__ blob,
+                                                             const BatchItem* __restrict__ items) {
+  extern __shared__ unsigned char smraw[];
+  const BatchItem it = items[blockIdx.x];
+  const BatchCircuit C = circs[it.circ];
+  const BatchPass P = passes[C.pass0 + it.pass];
+  batch_gate_tiles<EB>(states + C.off, P, it.pass == 0, (uint64_t)it.tile0, (uint64_t)it.ntile, prog, blob,
+                       reinterpret_cast<float2*>(smraw), threadIdx.x);
+}
+
+// Observable pass it.pass of a split circuit over tiles [tile0, tile0 + ntile):
+// part[blockIdx.x] = the workgroup's sum of coeff * <term> contributions.
+__global__ __launch_bounds__(256) void k_batch_split_obs(const float2* __restrict__ states,
+                                                         const BatchCircuit* __restrict__ circs,
+                                                         const BatchPass* __restrict__ passes,
+                                                         const BatchTerm* __restrict__ terms,
+                                                         const BatchItem* __restrict__ items,
+                                                         double* __restrict__ part) {
+  extern __shared__ unsigned char smraw[];
+  float2* sm = reinterpret_cast<float2*>(smraw);
+  __shared__ double red[4];
+  const int tid = threadIdx.x;
+  const BatchItem it = items[blockIdx.x];
+  const BatchCircuit C = circs[it.circ];
+  const BatchPass* P = passes + C.red0 + it.pass;
+  const float2* st = states + C.off;
+  const int K = P->K;
+  const int32_t* q = P->q;
+  const uint32_t TT = 1u << K;
+  double acc = 0;
+  for (uint64_t tile = (uint64_t)it.tile0; tile < (uint64_t)(it.tile0 + it.ntile); ++tile) {
+    uint64_t base = tile;
+    for (int j = 0; j < K; ++j) base = ins0(base, q[j]);
+    for (uint32_t t = tid; t < TT; t += 256) {
+      uint64_t d = base;
+      for (int j = 0; j < K; ++j) d |= (uint64_t)((t >> j) & 1) << q[j];
+      sm[t] = st[d];
+    }
+    __syncthreads();
+    acc += tile_terms(sm, TT, terms + P->term0, P->nterm, base, tid);
+    __syncthreads();
+  }
+  const double s = block_sum<256>(acc, red);
+  if (tid == 0) part[blockIdx.x] = s;
+}
+
+// out[C.out] of split circuit k = the sum of its observable items' parts, in order.
+__global__ void k_batch_split_out(const BatchCircuit* __restrict__ circs, int circ0, int nsplit,
+                                  const int32_t* __restrict__ first, const int32_t* __restrict__ count,
+                                  const double* __restrict__ part, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsplit) return;
+  double s = 0;
+  for (int i = 0; i < count[k]; ++i) s += part[first[k] + i];
+  out[circs[circ0 + k].out] = s;
+}
+
+void launch_batch_split_gate(void* states, const BatchCircuit* circs, const BatchPass* passes,
+                             const RegOp<float>* prog, const unsigned char* blob, const BatchItem* items,
+                             int nitems, hipStream_t s) {
+  const size_t lds = sizeof(float2) << (8 + kBatchEB);
+  hipLaunchKernelGGL(k_batch_split_gate<kBatchEB>, dim3(nitems), dim3(256), lds, s, (float2*)states, circs, passes,
+                     prog, blob, items);
+}
+
+void launch_batch_split_obs(const void* states, const BatchCircuit* circs, const BatchPass* passes,
+                            const BatchTerm* terms, const BatchItem* items, int nitems, double* part, hipStream_t s) {
+  const size_t lds = sizeof(float2) << (8 + kBatchEB);
+  hipLaunchKernelGGL(k_batch_split_obs, dim3(nitems), dim3(256), lds, s, (const float2*)states, circs, passes, terms,
+                     items, part);
+}
+
+void launch_batch_split_out(const BatchCircuit* circs, int circ0, int nsplit, const int32_t* first,
+                            const int32_t* count, const double* part, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_split_out, dim3((nsplit + 255) / 256), dim3(256), 0, s, circs, circ0, nsplit, first,
+                     count, part, out);
+}
+
 // Operand blob of block blockIdx.x from its 16x16 complex64 matrix M: lane l
 // of the blob holds A(l % 32, 16 s + 8 (l / 32) + i) for K-steps s = 0, 1 and
 // i = 0..7 of the 32x32 real form [[re, -im], [im, re]], as 8 fp16 high

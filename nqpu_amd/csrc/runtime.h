@@ -291,8 +291,18 @@ struct BatchPlan {
   size_t head = 0;                            // host = device offset of the blob area
   size_t nblk = 0, qunits = 0;                // matrix-core blocks, phase-table units
   size_t dev_q = 0, dev_m = 0, dev_total = 0;  // device offsets: phase tables, matrices; size
-  uint64_t slot_amps = 0;  // largest HBM-resident circuit: each workgroup's state slot
+  uint64_t slot_amps = 0;  // largest queued HBM-resident circuit: each workgroup's state slot
   size_t ncirc = 0, program_ops = 0;
+  // split circuits (kBatchSplitQubits and more) follow the nqueue queued ones
+  // in the circuit table; their states take split_amps amplitudes after the
+  // workgroup slots; work items at oit (gate passes per pass index, then the
+  // observable passes), per split circuit its first observable item and
+  // count at ofirst / ocount
+  size_t nqueue = 0;
+  uint64_t split_amps = 0;
+  size_t oit = 0, ofirst = 0, ocount = 0;
+  std::vector<std::pair<size_t, size_t>> gate_items;  // (first item, count) per pass index
+  std::pair<size_t, size_t> obs_items{0, 0};
   NqpuBatchStats st{};
 };
 // buffer(bytes) returns at least `bytes` of host memory for the table image

@@ -20,8 +20,13 @@ def test_plan_is_deterministic_and_counts_passes():
     small = sum(1 for v in vcs if v.circuit.num_qubits <= 12)
     assert a["resident"] == small
     assert a["gate_passes"] > 0 and a["observable_passes"] > 0
-    # the plan's state bytes are one workgroup slot: the largest HBM-resident circuit
-    assert a["state_bytes"] == 8.0 * max(1 << v.circuit.num_qubits for v in vcs if v.circuit.num_qubits > 12)
+    # the plan's state bytes are one workgroup slot (the largest HBM-resident
+    # circuit below 18 qubits) plus a state of its own for each circuit of 18
+    # qubits or more, which runs as grids of work items
+    slot = max(1 << v.circuit.num_qubits for v in vcs if 12 < v.circuit.num_qubits < 18)
+    split = sum(1 << v.circuit.num_qubits for v in vcs if v.circuit.num_qubits >= 18)
+    assert split > 0
+    assert a["state_bytes"] == 8.0 * (slot + split)
 
 
 def test_circuits_without_gate_passes_still_get_one():

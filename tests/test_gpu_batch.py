@@ -120,3 +120,24 @@ def test_queued_batches_equal_synchronous_batches(batch):
     for k in range(3):
         assert np.array_equal(got[k], want[k]), f"batch {k} (second round)"
     assert batch.last_stats()["circuits"] == 17
+
+
+def test_split_circuits_alone_equal_mixed_and_oracle(batch, oracle):
+    """Circuits of 18 qubits and more run their passes as grids of work items
+    (several workgroups per circuit) next to the persistent launch: each
+    energy is the same alone and in a mix, and the oracle's."""
+    vcs = variational_batch(40, seed=21, nmin=16, nmax=20, lmin=2, lmax=5)
+    vcs += [v for v in variational_batch(60, seed=22, nmin=8, nmax=12)]
+    assert sum(1 for v in vcs if v.circuit.num_qubits >= 18) >= 10
+    circs = [v.circuit for v in vcs]
+    obs = [v.hamiltonian() for v in vcs]
+    mixed = batch.expectation(circs, obs)
+    for k in range(0, len(vcs), 7):
+        alone = batch.expectation([circs[k]], [obs[k]])
+        assert alone[0] == mixed[k], k
+    big = [k for k, v in enumerate(vcs) if v.circuit.num_qubits >= 18][:4]
+    for k in big:
+        assert abs(mixed[k] - oracle_energy(oracle, vcs[k].circuit, obs[k])) < 2e-5
+    # a split circuit without observables gets 0
+    e = batch.expectation([circs[big[0]], circs[0]], [[], obs[0]])
+    assert e[0] == 0.0 and e[1] == mixed[0]
