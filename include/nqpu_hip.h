@@ -241,6 +241,14 @@ typedef struct {
  * NQPU_RUN_TIME_PASSES, pass_ms then covers all runs queued since the last
  * synchronisation (read it after nqpu_state_synchronize). */
 #define NQPU_RUN_ASYNC 256u
+/* run the circuit from |0...0> instead of from the state's current contents
+ * (the same as nqpu_state_reset first, without its separate fill pass and
+ * synchronisation): the first tile pass generates the start state instead of
+ * reading it.  A tiled complex64 run may leave the state's qubits in the
+ * layout its last pass stored them in; every call that reads or writes the
+ * state by index restores the canonical layout first, so the layout is never
+ * visible through this interface. */
+#define NQPU_RUN_FROM_ZERO 512u
 
 /* Statistics of the last run on a state. */
 typedef struct {
@@ -473,6 +481,11 @@ NqpuError nqpu_debug_compile_passes(const NqpuCircuit* circuit, const NqpuRunOpt
 /* The same for a state of the given precision (complex128: register tiles
  * of 11-13 qubits). */
 NqpuError nqpu_debug_compile_passes_ex(const NqpuCircuit* circuit, const NqpuRunOptions* opts, uint32_t precision);
+
+/* The state's current qubit layout: positions[q] = the bit of an
+ * amplitude's position in device memory that holds qubit q (q itself in the
+ * canonical layout; see NQPU_RUN_FROM_ZERO).  len >= the state's qubits. */
+NqpuError nqpu_debug_state_layout(const NqpuState* state, int32_t* positions, uint32_t len);
 
 /* Detail string of the last error on the calling thread. */
 const char* nqpu_last_error_detail(void);

@@ -79,6 +79,7 @@ RUN_NO_QPHASE = 32
 RUN_FIXED_LOW = 64
 RUN_UNTUNED = 128
 RUN_ASYNC = 256
+RUN_FROM_ZERO = 512
 
 
 class NqpuFusionCounts(ct.Structure):
@@ -198,6 +199,7 @@ SIGNATURES = {
     "nqpu_debug_emulate": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32, _P, ct.c_uint64]),
     "nqpu_debug_compile_passes": (_E, [_P, ct.POINTER(NqpuRunOptions)]),
     "nqpu_debug_compile_passes_ex": (_E, [_P, ct.POINTER(NqpuRunOptions), ct.c_uint32]),
+    "nqpu_debug_state_layout": (_E, [_P, _P, ct.c_uint32]),
     "nqpu_batch_create": (_P, []),
     "nqpu_batch_destroy": (None, [_P]),
     "nqpu_batch_expectation": (_E, [_P, _P, ct.c_uint64, _P, _P, ct.POINTER(NqpuRunOptions), _P]),

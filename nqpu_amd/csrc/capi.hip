@@ -213,6 +213,8 @@ NqpuError nqpu_state_copy(NqpuState* dst, const NqpuState* src) {
   launch_copy(d->amps, s->amps, d->dim(), d->prec == NQPU_PRECISION_C64, d->stream);
   e = hip_check(hipGetLastError(), "copy launch");
   if (e != NQPU_OK) return e;
+  d->perm = s->perm;  // the copy keeps the layout
+  d->perm_k = s->perm_k;
   return hip_check(hipStreamSynchronize(d->stream), "copy");
 }
 
@@ -578,6 +580,14 @@ NqpuError nqpu_debug_emulate(const NqpuCircuit* c, const NqpuRunOptions* opts, u
   NqpuError e = emulate_tiled(*Cc(c), o, (int)precision, a);
   for (size_t i = 0; i < a.size(); ++i) amps[i] = NqpuComplex{a[i].real(), a[i].imag()};
   return e;
+}
+
+NqpuError nqpu_debug_state_layout(const NqpuState* state, int32_t* positions, uint32_t len) {
+  if (!state || !positions) return NQPU_NULL_POINTER;
+  const State* s = S(state);
+  if (len < (uint32_t)s->n) return NQPU_INVALID_PARAMETER;
+  for (int q = 0; q < s->n; ++q) positions[q] = s->perm.empty() ? q : s->perm[q];
+  return NQPU_OK;
 }
 
 NqpuError nqpu_circuit_fusion_counts(const NqpuCircuit* c, NqpuFusionCounts* out) {

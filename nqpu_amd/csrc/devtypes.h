@@ -109,6 +109,10 @@ struct PassArgs {
   int32_t sync_store;
   // tiles each workgroup processes (grid = tiles / tiles_per_group); 0 or 1: one
   int32_t tiles_per_group;
+  // register tile pass: nonzero: the pass starts from |0...0> (amplitude 1 at
+  // position 0) instead of loading the state; the first pass of a run from
+  // the start state reads nothing
+  int32_t from_zero;
 };
 constexpr float kMfmaScale = 32768.0f;
 

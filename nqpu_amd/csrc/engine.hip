@@ -302,7 +302,8 @@ std::vector<HostOp> merge_single_qubit(const std::vector<HostOp>& ops) {
 // its qubits fit in S; otherwise its qubits become blocked.  Ops that join
 // commute with every op they overtake (disjoint qubits), so the order of
 // application is preserved.
-static std::vector<Pass> plan_tiles_once(const std::vector<HostOp>& ops, int n, const PlanOptions& po, bool move) {
+static std::vector<Pass> plan_tiles_once(const std::vector<HostOp>& ops, int n, const PlanOptions& po, bool move,
+                                         std::vector<int>* final_pi) {
   std::vector<Pass> passes;
   const int K = std::min(po.tile_qubits, n);
   const int L = std::min(po.low_qubits, K);
@@ -504,12 +505,27 @@ static std::vector<Pass> plan_tiles_once(const std::vector<HostOp>& ops, int n, 
     if (move) commit(np);
     rem.swap(next);
   }
-  // back to the identity: each pass holds the low positions and, cycle by
-  // cycle of the remaining permutation (shortest first, those through the
-  // low positions first), as many further positions as fit; every qubit whose
-  // position is in the pass goes home if its home is too, the others fill
-  // the positions left
-  for (int guard = 0; move && guard < 4 * n; ++guard) {
+  if (move) {
+    std::vector<Pass> close;
+    if (!closing_passes(pi, n, K, L, &close)) return {};  // the caller keeps the fixed-low plan
+    if (po.keep_perm) {
+      if (final_pi) *final_pi = pi;
+    } else {
+      for (Pass& p : close) passes.push_back(std::move(p));
+    }
+  }
+  return passes;
+}
+
+// Back to the identity: each pass holds the low positions and, cycle by
+// cycle of the remaining permutation (shortest first, those through the low
+// positions first), as many further positions as fit; every qubit whose
+// position is in the pass goes home if its home is too, the others fill the
+// positions left.  False if the passes do not reach the identity.
+bool closing_passes(std::vector<int> pi, int n, int K, int L, std::vector<Pass>* out) {
+  std::vector<int> inv(n);
+  for (int q = 0; q < n; ++q) inv[pi[q]] = q;
+  for (int guard = 0; guard < 4 * n; ++guard) {
     bool done = true;
     for (int q = 0; q < n; ++q) done = done && pi[q] == q;
     if (done) break;
@@ -546,25 +562,44 @@ static std::vector<Pass> plan_tiles_once(const std::vector<HostOp>& ops, int n, 
       const int q = __builtin_ctzll(m);
       if (np[q] < 0) used |= bit(np[q] = __builtin_ctzll(P & ~used));
     }
-    emit_pass(S, {}, np);
-    commit(np);
+    // a pass without ops over physical positions P that stores the qubit at
+    // position x at np[inv[x]]
+    Pass p;
+    bool ident = true;
+    for (int x = 0; x < n; ++x)
+      if ((P >> x) & 1) {
+        p.qubits.push_back(x);
+        p.out.push_back(np[inv[x]]);
+        ident = ident && p.out.back() == x;
+      }
+    if (ident) p.out.clear();
+    out->push_back(std::move(p));
+    pi = np;
+    for (int q = 0; q < n; ++q) inv[pi[q]] = q;
   }
-  for (int q = 0; move && q < n; ++q)
-    if (pi[q] != q) return {};  // not back at the identity: the caller keeps the fixed-low plan
-  return passes;
+  for (int q = 0; q < n; ++q)
+    if (pi[q] != q) return false;
+  return true;
 }
 
 // With a moving low set (complex64 register passes), both plans are made and
 // the one with fewer passes kept: a QFT's final qubit reversal, for one, is
 // free inside fixed-low tiles but leaves a permutation to undo otherwise.
-std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po) {
+std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po,
+                             std::vector<int>* final_pi) {
+  if (final_pi) final_pi->clear();
   const int K = std::min(po.tile_qubits, n), L = std::min(po.low_qubits, K);
-  if (!(po.move_low && n > K && K >= 11 && K <= 13 && L == 5)) return plan_tiles_once(ops, n, po, false);
+  if (!(po.move_low && n > K && K >= 11 && K <= 13 && L == 5)) return plan_tiles_once(ops, n, po, false, nullptr);
   std::vector<Pass> fixed;
-  std::thread th([&] { fixed = plan_tiles_once(ops, n, po, false); });
-  std::vector<Pass> moving = plan_tiles_once(ops, n, po, true);
+  std::thread th([&] { fixed = plan_tiles_once(ops, n, po, false, nullptr); });
+  std::vector<int> pm;
+  std::vector<Pass> moving = plan_tiles_once(ops, n, po, true, &pm);
   th.join();
-  return !moving.empty() && moving.size() < fixed.size() ? moving : fixed;
+  if (moving.empty() || moving.size() >= fixed.size()) return fixed;
+  bool ident = true;
+  for (int q = 0; q < (int)pm.size(); ++q) ident = ident && pm[q] == q;
+  if (final_pi && !ident) *final_pi = pm;
+  return moving;
 }
 
 // ------------------------------------------------------ reference fusion
@@ -838,6 +873,7 @@ NqpuError create_state(int n, int prec, State** out) {
 }
 
 NqpuError reset_state(State* s) {
+  s->perm.clear();  // |0...0> is the same in every layout
   dispatch(s, [&](auto* tag) {
     using R = std::remove_pointer_t<decltype(tag)>;
     Launch<R>::init_zero(s->amps, s->n, s->stream);
@@ -859,6 +895,8 @@ NqpuError clone_state(const State* s, State** out) {
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
   HIPTRY(hipMemcpyAsync(c->amps, s->amps, bytes, hipMemcpyDeviceToDevice, c->stream), "clone copy");
   HIPTRY(hipStreamSynchronize(c->stream), "clone sync");
+  c->perm = s->perm;  // the copy keeps the layout
+  c->perm_k = s->perm_k;
   *out = c.release();
   return NQPU_OK;
 }
@@ -2387,8 +2425,12 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
     }
     return NQPU_OK;
   }
+  // as a run does: the plan may leave the qubits in another layout, which
+  // the state's next read by index restores (settle_state's passes)
   PlanOptions po = plan_options(o, prec, c.n);
-  std::vector<Pass> passes = plan_tiles(ops, c.n, po);
+  po.keep_perm = true;
+  std::vector<int> fpi;
+  std::vector<Pass> passes = plan_tiles(ops, c.n, po, &fpi);
   const int K = std::min(po.tile_qubits, c.n);
   const bool reg = po.registers && K >= 11 && K <= 13;
   const bool c64 = prec == NQPU_PRECISION_C64;
@@ -2403,6 +2445,14 @@ NqpuError emulate_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, std
       append_dev_ops<double>(p, dops);
       emulate_lds_pass(a, c.n, p, dops);
     }
+  }
+  if (!fpi.empty()) {
+    std::vector<Pass> close;
+    if (!closing_passes(fpi, c.n, K, 5, &close)) {
+      set_error("the state's qubit layout cannot be restored");
+      return NQPU_SIMULATION_FAILED;
+    }
+    for (const Pass& p : close) emulate_reg_pass(a, c.n, p, build_reg_program<double>(p, K - 8, 8, false, c64), K - 8);
   }
   return NQPU_OK;
 }
@@ -2531,8 +2581,9 @@ static NqpuError prepare_tiled_ops(const std::vector<HostOp>& ops, int n, const 
   *out = PreparedRun{};
   out->prec = std::is_same<R, float>::value ? NQPU_PRECISION_C64 : NQPU_PRECISION_C128;
   out->n = n;
-  std::vector<Pass> passes = plan_tiles(ops, n, po);
+  std::vector<Pass> passes = plan_tiles(ops, n, po, &out->perm);
   const int K = std::min(po.tile_qubits, n);
+  out->perm_k = K;
   NqpuError e = NQPU_OK;
   if (po.registers && K >= 11 && K <= 13) {
     out->stats.device_ops += ops.size();
@@ -2631,16 +2682,50 @@ static void add_stats(NqpuRunStats& a, const NqpuRunStats& b) {
   a.qphase_gates += b.qphase_gates;
 }
 
+// A run from |0...0>: its first register pass generates the start state
+// instead of loading it (no read, and no separate fill pass before it).
+// False when the run's first launch cannot (LDS tile passes, or no passes).
+bool start_from_zero(PreparedRun* r) {
+  if (r->launches.empty() || r->launches[0].kind == RunLaunch::LDS) return false;
+  r->launches[0].pa.from_zero = 1;
+  return true;
+}
+
 template <typename R>
-static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanOptions& po) {
+static NqpuError run_tiled(State* s, const std::vector<HostOp>& ops, const PlanOptions& po, bool from_zero) {
   PreparedRun r;
   NqpuError e = prepare_tiled_ops<R>(ops, s->n, po, &r);
   if (e != NQPU_OK) return e;
+  if (from_zero && !start_from_zero(&r)) Launch<R>::init_zero(s->amps, s->n, s->stream);
   if ((e = upload_image(s, r.image)) != NQPU_OK) return e;
   if ((e = launch_prepared_t<R>(r, s->amps, s->opbuf.p, s->stream, &s->timer)) != NQPU_OK) return e;
+  s->perm = r.perm;
+  s->perm_k = r.perm_k;
   // the run's compiled modules stay loaded until the stream is synchronised
   s->jit_hold.insert(s->jit_hold.end(), r.hold.begin(), r.hold.end());
   add_stats(s->stats, r.stats);
+  return NQPU_OK;
+}
+
+NqpuError settle_state(const State* s0) {
+  if (s0->perm.empty()) return NQPU_OK;
+  // the amplitudes move, the state they represent does not
+  State* s = const_cast<State*>(s0);
+  std::vector<Pass> passes;
+  if (!closing_passes(s->perm, s->n, s->perm_k, 5, &passes)) {
+    set_error("the state's qubit layout cannot be restored");
+    return NQPU_SIMULATION_FAILED;
+  }
+  PreparedRun r;
+  r.prec = s->prec;
+  r.n = s->n;
+  const int eb = s->perm_k - 8;
+  NqpuError e = s->prec == NQPU_PRECISION_C64 ? prepare_reg<float>(passes, s->n, eb, false, false, false, 1, &r)
+                                              : prepare_reg<double>(passes, s->n, eb, false, false, false, 1, &r);
+  if (e != NQPU_OK) return e;
+  if ((e = upload_image(s, r.image)) != NQPU_OK) return e;
+  if ((e = launch_prepared(r, s->amps, s->opbuf.p, s->stream, nullptr)) != NQPU_OK) return e;
+  s->perm.clear();
   return NQPU_OK;
 }
 
@@ -2648,7 +2733,8 @@ template <typename R>
 static NqpuError jit_check_circuit_t(const Circuit& c, const NqpuRunOptions& o) {
   constexpr bool c64 = std::is_same<R, float>::value;
   const int prec = c64 ? NQPU_PRECISION_C64 : NQPU_PRECISION_C128;
-  const PlanOptions po = plan_options(o, prec, c.n);
+  PlanOptions po = plan_options(o, prec, c.n);
+  po.keep_perm = true;  // the passes a run compiles
   const int K = std::min(po.tile_qubits, c.n);
   if (!po.registers || (c64 ? K != 12 : (K < 11 || K > 13))) {
     set_error(c64 ? "compiled passes need complex64 register tiles of 12 qubits"
@@ -2739,6 +2825,7 @@ NqpuRunStats plan_stats(const Circuit& c, const NqpuRunOptions& o, int prec) {
   } else {
     std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
     PlanOptions po = plan_options(o, prec, c.n);
+    po.keep_perm = true;  // as runs plan
     std::vector<Pass> passes = plan_tiles(ops, c.n, po);
     st.passes = passes.size();
     st.device_ops = ops.size();
@@ -2765,17 +2852,26 @@ NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o) {
   return run_circuit_with(s, c, o, plan_options(o, s->prec, s->n));
 }
 
-NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, const PlanOptions& po) {
+NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, const PlanOptions& po0) {
   s->stats = NqpuRunStats{};
   s->stats.gates = c.gates.size();
   const bool ref = o.semantics == NQPU_SEMANTICS_REFERENCE;
+  const bool from_zero = (o.flags & NQPU_RUN_FROM_ZERO) != 0;
   NqpuError e = NQPU_OK;
+  // runs are planned on the canonical layout; |0...0> is the same in every
+  // layout
+  if (from_zero) s->perm.clear();
+  else if ((e = settle_state(s)) != NQPU_OK) return e;
+  // a tiled run leaves the qubits where its last pass put them
+  PlanOptions po = po0;
+  po.keep_perm = true;
   // runs queued without waiting (NQPU_RUN_ASYNC) keep adding to the pass
   // timer until the state is synchronised
   if (!s->async_open) s->timer.begin((o.flags & NQPU_RUN_TIME_PASSES) != 0);
   else if (o.flags & NQPU_RUN_TIME_PASSES) s->timer.on = true;
   dispatch(s, [&](auto* tag) {
     using R = std::remove_pointer_t<decltype(tag)>;
+    if (from_zero && o.fusion != NQPU_FUSION_TILED) Launch<R>::init_zero(s->amps, s->n, s->stream);
     if (o.fusion == NQPU_FUSION_NONE) {
       for (const Gate& g : c.gates) {
         for (const HostOp& op : lower_gate(g, ref)) {
@@ -2801,7 +2897,7 @@ NqpuError run_circuit_with(State* s, const Circuit& c, const NqpuRunOptions& o, 
       }
     } else {
       std::vector<HostOp> ops = merge_single_qubit(lower_circuit(c, ref));
-      e = run_tiled<R>(s, ops, po);
+      e = run_tiled<R>(s, ops, po, from_zero);
     }
   });
   if (e != NQPU_OK) {
@@ -2861,6 +2957,12 @@ PassTimer::~PassTimer() {
 NqpuError overlap(const State* a0, const State* b0, NqpuComplex* out) {
   State* a = const_cast<State*>(a0);
   State* b = const_cast<State*>(b0);
+  // <a|b> does not depend on the layout when both states share it
+  if (a->perm != b->perm) {
+    NqpuError e = settle_state(a);
+    if (e == NQPU_OK) e = settle_state(b);
+    if (e != NQPU_OK) return e;
+  }
   HIPTRY(hipStreamSynchronize(b->stream), "sync");
   const int cb = std::min(a->n, 16);
   const uint64_t nch = a->dim() >> cb;
@@ -2882,7 +2984,8 @@ NqpuError overlap(const State* a0, const State* b0, NqpuComplex* out) {
 }
 
 NqpuError apply_gate_now(State* s, const Gate& g) {
-  NqpuError e = NQPU_OK;
+  NqpuError e = settle_state(s);
+  if (e != NQPU_OK) return e;
   dispatch(s, [&](auto* tag) {
     using R = std::remove_pointer_t<decltype(tag)>;
     for (const HostOp& op : lower_gate(g, false))
@@ -2899,6 +3002,8 @@ static const uint64_t kXferChunk = 1ull << 24;  // amplitudes per staging copy
 
 NqpuError get_amplitudes(const State* s0, uint64_t off, uint64_t count, NqpuComplex* out) {
   State* s = const_cast<State*>(s0);
+  NqpuError e = settle_state(s);
+  if (e != NQPU_OK) return e;
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
   if (s->prec == NQPU_PRECISION_C128) {
     HIPTRY(copy_on(s->stream, out, (char*)s->amps + off * 16, count * 16, hipMemcpyDeviceToHost), "amplitude copy");
@@ -2916,6 +3021,8 @@ NqpuError get_amplitudes(const State* s0, uint64_t off, uint64_t count, NqpuComp
 }
 
 NqpuError set_amplitudes(State* s, uint64_t off, uint64_t count, const NqpuComplex* in) {
+  NqpuError e = settle_state(s);
+  if (e != NQPU_OK) return e;
   HIPTRY(hipStreamSynchronize(s->stream), "sync");
   if (s->prec == NQPU_PRECISION_C128) {
     HIPTRY(copy_on(s->stream, (char*)s->amps + off * 16, in, count * 16, hipMemcpyHostToDevice), "amplitude copy");
@@ -3003,6 +3110,10 @@ NqpuError zsum_all(const State* s0, double* tot, std::vector<double>* s1) {
       if ((c >> (q - cb)) & 1) (*s1)[q] += r[0];
   }
   *tot = t;
+  if (!s->perm.empty()) {  // position sums -> qubit sums
+    std::vector<double> by_pos(*s1);
+    for (int q = 0; q < n; ++q) (*s1)[q] = by_pos[s->perm[q]];
+  }
   return NQPU_OK;
 }
 
@@ -3045,9 +3156,10 @@ NqpuError measure_qubit(State* s, int q, double r, uint32_t* outcome, double* pr
 }
 
 NqpuError collapse(State* s, int q, uint32_t outcome, double f) {
+  const int pos = s->perm.empty() ? q : s->perm[q];  // the qubit's position in the layout
   dispatch(s, [&](auto* tag) {
     using R = std::remove_pointer_t<decltype(tag)>;
-    Launch<R>::collapse(s->amps, s->n, q, outcome, f, s->stream);
+    Launch<R>::collapse(s->amps, s->n, pos, outcome, f, s->stream);
   });
   HIPTRY(hipGetLastError(), "collapse");
   HIPTRY(hipStreamSynchronize(s->stream), "collapse");

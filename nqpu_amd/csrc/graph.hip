@@ -11,8 +11,11 @@
 // plans it on the host, checks that it launches the same kernels with the
 // same arguments, and copies the new image in stream order (after every replay
 // queued before it).  Another structure records the graph again.
-// Nothing in a graph writes memory with hipMemsetAsync: the |0...0> fill is
-// a kernel.
+// Nothing in a graph writes memory with hipMemsetAsync: a graph that starts
+// from |0...0> has its first register pass generate that state (or, for LDS
+// tile passes, a fill kernel before them).  A replay leaves the state in the
+// qubit layout of the run's last pass (State::perm); a graph that does not
+// start from |0...0> restores the canonical layout before its replay.
 #include "runtime.h"
 
 #include <chrono>
@@ -71,8 +74,10 @@ bool same_launches(const PreparedRun& a, const PreparedRun& b) {
 
 NqpuError plan(RunGraph* g, const Circuit& c, PreparedRun* out) {
   const auto t0 = std::chrono::steady_clock::now();
-  const PlanOptions po = plan_options(g->o, g->s->prec, g->s->n);
+  PlanOptions po = plan_options(g->o, g->s->prec, g->s->n);
+  po.keep_perm = true;
   NqpuError e = prepare_tiled(c, g->o, g->s->prec, g->s->n, po, out);
+  if (e == NQPU_OK && (g->gflags & NQPU_GRAPH_FROM_ZERO)) (void)start_from_zero(out);
   g->info.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return e;
 }
@@ -103,7 +108,9 @@ NqpuError capture(RunGraph* g) {
   State* s = g->s;
   HIPTRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin graph capture");
   NqpuError e = NQPU_OK;
-  if (g->gflags & NQPU_GRAPH_FROM_ZERO) {
+  // from |0...0>: a fill kernel unless the first pass generates the state
+  const bool fill = (g->gflags & NQPU_GRAPH_FROM_ZERO) && (g->run.launches.empty() || !g->run.launches[0].pa.from_zero);
+  if (fill) {
     if (s->prec == NQPU_PRECISION_C64) Launch<float>::init_zero(s->amps, s->n, s->stream);
     else Launch<double>::init_zero(s->amps, s->n, s->stream);
     e = hip_check(hipGetLastError(), "|0...0> fill launch");
@@ -118,7 +125,7 @@ NqpuError capture(RunGraph* g) {
   HIPTRY(end, "end graph capture");
   g->graph = graph;
   HIPTRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "instantiate graph");
-  g->info.launches = g->run.launches.size() + ((g->gflags & NQPU_GRAPH_FROM_ZERO) ? 1 : 0);
+  g->info.launches = g->run.launches.size() + (fill ? 1 : 0);
   g->info.passes = g->run.launches.size();
   return NQPU_OK;
 }
@@ -219,7 +226,13 @@ NqpuError nqpu_graph_launch(NqpuRunGraph* graph, uint32_t flags) {
     set_error("run graph has no recorded launches (a failed update)");
     return NQPU_INVALID_PARAMETER;
   }
+  // the recorded passes start from the canonical layout (or from |0...0>,
+  // which is the same in every layout)
+  NqpuError e = NQPU_OK;
+  if (!(g->gflags & NQPU_GRAPH_FROM_ZERO) && (e = settle_state(g->s)) != NQPU_OK) return e;
   HIPTRY(hipGraphLaunch(g->exec, g->s->stream), "graph launch");
+  g->s->perm = g->run.perm;
+  g->s->perm_k = g->run.perm_k;
   g->info.replays++;
   g->s->stats = g->run.stats;
   if (flags & NQPU_RUN_ASYNC) return NQPU_OK;

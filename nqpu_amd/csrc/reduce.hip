@@ -578,10 +578,12 @@ NqpuError pauli_values(const State* s0, const std::vector<PauliTerm>& terms, std
   const uint64_t count = terms.size();
   vals->assign(count, 0.0);
   if (!count) return NQPU_OK;
+  // a Pauli string's value does not depend on the layout: its masks are
+  // taken to the positions that hold its qubits
   std::vector<uint64_t> xm(count), zm(count);
   for (uint64_t t = 0; t < count; ++t) {
-    xm[t] = terms[t].x;
-    zm[t] = terms[t].z;
+    xm[t] = layout_mask(s, terms[t].x);
+    zm[t] = layout_mask(s, terms[t].z);
   }
   std::vector<uint32_t> idx(count);
   group_order(xm.data(), count, idx.data());
@@ -689,14 +691,20 @@ NqpuError pauli_sum_noalloc(const State* s, const uint64_t* xm, const uint64_t* 
   double* sp = nullptr;
   NqpuError e = scratch_ptr(s->device, &sp);
   if (e != NQPU_OK) return e;
-  // host arrays on the stack: nothing here allocates
+  // host arrays on the stack: nothing here allocates.  The masks are taken
+  // to the positions of the state's layout, as in pauli_values.
   uint32_t idx[kPauliSumNoAllocMaxTerms];
+  uint64_t xl[kPauliSumNoAllocMaxTerms], zl[kPauliSumNoAllocMaxTerms];
   uint64_t xs[kPauliSumNoAllocMaxTerms], zs[kPauliSumNoAllocMaxTerms];
   double raw[kPauliSumNoAllocMaxTerms];
-  group_order(xm, count, idx, true);
   for (uint64_t t = 0; t < count; ++t) {
-    xs[t] = xm[idx[t]];
-    zs[t] = zm[idx[t]];
+    xl[t] = layout_mask(s, xm[t]);
+    zl[t] = layout_mask(s, zm[t]);
+  }
+  group_order(xl, count, idx, true);
+  for (uint64_t t = 0; t < count; ++t) {
+    xs[t] = xl[idx[t]];
+    zs[t] = zl[idx[t]];
   }
   double* dpart = sp;
   double* dres = sp + kStaticPart;
@@ -717,6 +725,9 @@ NqpuError pauli_sum_noalloc(const State* s, const uint64_t* xm, const uint64_t* 
 NqpuError sample(const State* s0, const std::vector<double>& r, uint64_t* out) {
   State* s = const_cast<State*>(s0);
   if (r.empty()) return NQPU_OK;
+  // shots follow the cumulative distribution in index order
+  NqpuError e = settle_state(s);
+  if (e != NQPU_OK) return e;
   const int n = s->n;
   const int cb = std::min(n, kChunkBits), sb = std::min(cb, kSubBits);
   const uint64_t nch = s->dim() >> cb, nsub = s->dim() >> sb;
@@ -791,6 +802,7 @@ NqpuError initialize_amplitudes(State* s, const NqpuComplex* in, uint64_t len) {
     return NQPU_INVALID_PARAMETER;
   }
   const double inv = 1.0 / nrm;
+  s->perm.clear();  // every amplitude is written, in the canonical layout
   for (uint64_t off = 0; off < dim; off += slice) {
     const uint64_t c = std::min(slice, dim - off);
     if (slice < dim)  // the single-slice case still holds the upload
@@ -825,6 +837,8 @@ NqpuError reset_qubit(State* s, int q) {
 
 NqpuError probabilities(const State* s0, double* out) {
   State* s = const_cast<State*>(s0);
+  NqpuError e = settle_state(s);
+  if (e != NQPU_OK) return e;
   const uint64_t dim = s->dim();
   const uint64_t slice = std::min<uint64_t>(dim, 1ull << 24);
   if (!s->scratch.ensure(slice * 8)) return NQPU_OUT_OF_DEVICE_MEMORY;

@@ -654,13 +654,25 @@ __device__ __forceinline__ void reg_tile_pass(T* __restrict__ st, const PassArgs
 #pragma unroll
     for (int j = 0; j < 8 + EB; ++j) base = ins0(base, pa.q[j]);
     T v[E];
+    if (pa.from_zero) {
 #pragma unroll
-    for (int r = 0; r < E; ++r) {
-      uint64_t d = base | dlo;
+      for (int r = 0; r < E; ++r) {
+        uint64_t d = base | dlo;
 #pragma unroll
-      for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) d |= 1ull << pa.ql[8 + j];
-      v[r] = load_stream(st + d);
+        for (int j = 0; j < EB; ++j)
+          if ((r >> j) & 1) d |= 1ull << pa.ql[8 + j];
+        v[r].x = d == 0 ? (R)1 : (R)0;
+        v[r].y = (R)0;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < E; ++r) {
+        uint64_t d = base | dlo;
+#pragma unroll
+        for (int j = 0; j < EB; ++j)
+          if ((r >> j) & 1) d |= 1ull << pa.ql[8 + j];
+        v[r] = load_stream(st + d);
+      }
     }
     if (pa.scaled) {
 #pragma unroll

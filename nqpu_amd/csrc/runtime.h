@@ -75,6 +75,9 @@ struct PlanOptions {
   // register tile passes: tiles per workgroup (the launch grid is the tile
   // count divided by this)
   int tiles_per_group = 1;
+  // with move_low: end the run with the qubits where its last pass left
+  // them (no closing permutation passes); plan_tiles reports that map
+  bool keep_perm = false;
 };
 
 // A measured plan choice for states of n qubits at one precision (tune.hip).
@@ -98,7 +101,14 @@ NqpuError tune_load(const char* path);
 std::vector<HostOp> lower_gate(const Gate& g, bool reference_semantics);
 std::vector<HostOp> lower_circuit(const Circuit& c, bool reference_semantics);
 std::vector<HostOp> merge_single_qubit(const std::vector<HostOp>& ops);
-std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po);
+// final_pi (optional): where circuit qubit q is at the end of the plan
+// (empty: at q; only with po.keep_perm can it differ)
+std::vector<Pass> plan_tiles(const std::vector<HostOp>& ops, int n, const PlanOptions& po,
+                             std::vector<int>* final_pi = nullptr);
+// Permutation-only passes of K-qubit tiles (L = 5 low positions) that move
+// circuit qubit q from position pi[q] back to position q (appended to out);
+// false if they do not get there.
+bool closing_passes(std::vector<int> pi, int n, int K, int L, std::vector<Pass>* out);
 FusionResult fuse_gates_reference(const std::vector<Gate>& gates);
 PlanOptions plan_options(const NqpuRunOptions& o, int prec, int n = 0);  // n: state qubits (0: unknown)
 // What run_circuit would launch, computed on the host.
@@ -162,6 +172,12 @@ struct State {
   std::vector<std::shared_ptr<void>> jit_hold;
   // runs were queued with NQPU_RUN_ASYNC and not yet waited for
   bool async_open = false;
+  // Qubit layout: amplitude bit q of the state's logical index is stored at
+  // bit perm[q] of its position in `amps` (empty: at bit q).  A tiled run
+  // leaves the qubits where its last pass put them; perm_k is that run's
+  // tile width, which the passes that restore the layout (settle_state) use.
+  std::vector<int> perm;
+  int perm_k = 0;
   ~State();
   size_t amp_size() const { return prec == NQPU_PRECISION_C64 ? 8 : 16; }
   uint64_t dim() const { return 1ull << n; }
@@ -187,10 +203,17 @@ struct PreparedRun {
   std::vector<RunLaunch> launches;
   std::vector<std::shared_ptr<void>> hold;  // compiled modules the launches use
   NqpuRunStats stats{};
+  // the state's qubit layout after the run (State::perm) and its tile width
+  std::vector<int> perm;
+  int perm_k = 0;
 };
 // Plan circuit c (NQPU_FUSION_TILED; c.n <= n) for an n-qubit state of precision prec.
 NqpuError prepare_tiled(const Circuit& c, const NqpuRunOptions& o, int prec, int n, const PlanOptions& po,
                         PreparedRun* out);
+// Make the run start from |0...0>: its first register pass generates that
+// state instead of loading it.  False (run unchanged) when its first launch
+// is an LDS tile pass or it has none.
+bool start_from_zero(PreparedRun* r);
 // Enqueue the run's launches on st over the state at amps, the image at
 // `image` in device memory (timer: optional HIP events around each launch).
 NqpuError launch_prepared(const PreparedRun& r, void* amps, const void* image, hipStream_t st, PassTimer* timer);
@@ -209,6 +232,18 @@ int device_count();
 NqpuError create_state(int n, int prec, State** out);
 NqpuError clone_state(const State* s, State** out);
 NqpuError reset_state(State* s);
+// Restore the state's canonical layout (State::perm) with permutation-only
+// passes queued on its stream; a no-op when it has it.  Every operation that
+// reads or writes amplitudes by index calls this first; the norm and scaling
+// do not depend on the layout.
+NqpuError settle_state(const State* s);
+// The positions in the state's current layout of the qubits in mask m.
+inline uint64_t layout_mask(const State* s, uint64_t m) {
+  if (s->perm.empty()) return m;
+  uint64_t r = 0;
+  for (; m; m &= m - 1) r |= 1ull << s->perm[__builtin_ctzll(m)];
+  return r;
+}
 NqpuError run_circuit(State* s, const Circuit& c, const NqpuRunOptions& o);
 // Wait for the runs queued on the state; completes their statistics.
 NqpuError synchronize_state(State* s);

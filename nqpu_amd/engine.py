@@ -179,7 +179,7 @@ _SEMANTICS = {"standard": L.SEMANTICS_STANDARD, "reference": L.SEMANTICS_REFEREN
 
 def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=0,
                 time_passes=False, lds_tiles=False, mfma=True, compile=False, qphase=True,
-                fixed_low=False, untuned=False, async_=False) -> L.NqpuRunOptions:
+                fixed_low=False, untuned=False, async_=False, from_zero=False) -> L.NqpuRunOptions:
     """mfma=False keeps every gate of complex64 register tiles on the vector
     ALUs (no fused blocks on the matrix cores); compile=True runs complex64
     register tile passes as straight-line kernels compiled at run time;
@@ -194,7 +194,8 @@ def run_options(fusion="tiled", semantics="standard", tile_qubits=0, low_qubits=
     o.flags = ((L.RUN_TIME_PASSES if time_passes else 0) | (L.RUN_LDS_TILES if lds_tiles else 0)
                | (0 if mfma else L.RUN_NO_MFMA) | (L.RUN_COMPILE if compile else 0)
                | (0 if qphase else L.RUN_NO_QPHASE) | (L.RUN_FIXED_LOW if fixed_low else 0)
-               | (L.RUN_UNTUNED if untuned else 0) | (L.RUN_ASYNC if async_ else 0))
+               | (L.RUN_UNTUNED if untuned else 0) | (L.RUN_ASYNC if async_ else 0)
+               | (L.RUN_FROM_ZERO if from_zero else 0))
     return o
 
 
@@ -287,13 +288,15 @@ class StateVector:
     # --- execution
     def run(self, circuit: Circuit | NativeCircuit, fusion="tiled", semantics="standard",
             tile_qubits=0, low_qubits=0, time_passes=False, lds_tiles=False, mfma=True, compile=False,
-            qphase=True, fixed_low=False, untuned=False, async_=False) -> dict:
+            qphase=True, fixed_low=False, untuned=False, async_=False, from_zero=False) -> dict:
         """async_=True returns once the passes are queued (call synchronize()
-        before reading pass_ms); the state's later calls are ordered after it."""
+        before reading pass_ms); the state's later calls are ordered after it.
+        from_zero=True runs the circuit from |0...0> (reset() and run() in
+        one, without the separate fill pass)."""
         lib = L.load()
         nc = circuit if isinstance(circuit, NativeCircuit) else circuit.native()
         o = run_options(fusion, semantics, tile_qubits, low_qubits, time_passes, lds_tiles, mfma, compile, qphase,
-                        fixed_low, untuned, async_)
+                        fixed_low, untuned, async_, from_zero)
         L.check(lib.nqpu_run(nc.ptr, self.ptr, ct.byref(o)), "nqpu_run")
         return self.last_stats()
 

@@ -150,6 +150,7 @@ pub const NQPU_RUN_NO_QPHASE: u32 = 32;
 pub const NQPU_RUN_FIXED_LOW: u32 = 64;
 pub const NQPU_RUN_UNTUNED: u32 = 128;
 pub const NQPU_RUN_ASYNC: u32 = 256;
+pub const NQPU_RUN_FROM_ZERO: u32 = 512;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default)]
@@ -426,6 +427,7 @@ extern "C" {
         opts: *const NqpuRunOptions,
         precision: u32,
     ) -> NqpuError;
+    pub fn nqpu_debug_state_layout(state: *const NqpuState, positions: *mut i32, len: u32) -> NqpuError;
     pub fn nqpu_last_error_detail() -> *const c_char;
     pub fn nqpu_device_count() -> u32;
     pub fn nqpu_server_create(config: *const NqpuServerConfig) -> *mut NqpuServer;

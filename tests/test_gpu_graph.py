@@ -64,7 +64,10 @@ def test_replays_equal_the_eager_run(oracle, n, prec, kw):
         assert np.max(np.abs(want - exp)) < 1e-12
     with StateVector(n, prec) as sv, RunGraph(sv, c, from_zero=True, **kw) as g:
         info = g.info()
-        assert info["passes"] >= 1 and info["launches"] == info["passes"] + 1
+        # from |0...0>: the first register pass generates the start state;
+        # LDS tile passes (tiles under 11 qubits) get a fill kernel first
+        fill = 1 if min(n, 11 if prec == "c64" else (10 if n <= 20 else 11)) < 11 else 0
+        assert info["passes"] >= 1 and info["launches"] == info["passes"] + fill
         for k in range(3):
             # eager work on the same state between replays
             sv.run(other)
