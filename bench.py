@@ -753,10 +753,13 @@ def main():
     dim = 1 << n
     ms_per_step = elapsed * 1e3 / args.steps
     value = gates * dim * args.steps / elapsed
-    bytes_per_pass = 2.0 * dim * 8
+    # algorithmic state traffic: every pass reads and writes the state, except
+    # the first register pass of a run from |0...0>, which only writes it
+    step_bytes = (2.0 * passes - (0 if args.lds else 1)) * dim * 8
+    bytes_per_pass = step_bytes / max(passes, 1)
     avg_pass_ms = pass_ms / max(timed_passes, 1)
     achieved = bytes_per_pass / (avg_pass_ms * 1e-3) / 1e9
-    hbm_frac_step = passes * bytes_per_pass / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+    hbm_frac_step = step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
 
     result = {
         "metric": "amp-updates/s (30-qubit complex64 random circuit)",

@@ -129,8 +129,11 @@ void Server::execute(State& st, Request& r) {
   st.n = r.circuit.n;
   st.prec = prec;
   st.stats = NqpuRunStats{};
-  NqpuError e = reset_state(&st);
-  if (e == NQPU_OK) e = run_circuit(&st, r.circuit, run);
+  // from |0...0>: the run's first pass generates the start state (no fill
+  // pass, no synchronisation before the run)
+  NqpuRunOptions ro = run;
+  ro.flags |= NQPU_RUN_FROM_ZERO;
+  NqpuError e = run_circuit(&st, r.circuit, ro);
   r.expectations.assign(r.terms.size(), 0.0);
   if (e == NQPU_OK && !r.terms.empty()) {
     std::vector<double> v;
