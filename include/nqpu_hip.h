@@ -430,9 +430,10 @@ typedef struct {
   double kernel_ms;         /* the persistent launch and the launches of split circuits */
   double state_bytes;       /* HBM for the states of circuits above 12 qubits: one slot of the
                                largest such circuit below 18 qubits per workgroup of the persistent
-                               launch (nqpu_debug_batch_plan: one slot), plus one state for each
-                               circuit of 18 qubits or more, which runs its passes as grids of
-                               work items beside the persistent launch */
+                               launch (nqpu_debug_batch_plan: one slot), plus the states of the
+                               circuits of 18 qubits or more, which run their passes as grids of
+                               work items beside the persistent launch, in waves of at most 8 GiB
+                               of states */
   double hbm_bytes;         /* algorithmic state traffic of the launch: sum over circuits above 12
                                qubits of (2 x gate passes - 1 + observable passes) x 2^n x 8 bytes
                                (the first gate pass only writes) */

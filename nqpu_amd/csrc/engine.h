@@ -79,6 +79,9 @@ struct BatchCircuit {
 // made the largest circuits the tail of the persistent launch).
 constexpr int kBatchSplitQubits = 18;
 constexpr int kBatchSplitTiles = 8;
+// split circuits run in waves whose states fit this many complex64
+// amplitudes (8 GiB), so the HBM a batch needs does not grow with its count
+constexpr uint64_t kBatchSplitWaveAmps = 1ull << 30;
 
 // A unit of a split circuit's work: `ntile` consecutive tiles of gate pass
 // or observable pass `pass` of circuit `circ` (table index).

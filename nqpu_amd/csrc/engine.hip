@@ -3427,38 +3427,50 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   });
   size_t nqueue = 0;
   while (nqueue < nc && !split(order[nqueue])) ++nqueue;
-  // split circuits: own states, gate items per pass index, observable items
-  uint64_t split_amps = 0;
-  int max_pass = 0;
-  for (size_t i = nqueue; i < nc; ++i) {
-    BatchCircuit& C = bc[order[i]];
-    C.off = split_amps;
-    split_amps += C.amps;
-    max_pass = std::max(max_pass, (int)C.npass);
-  }
+  // split circuits: waves of circuits whose states fit kBatchSplitWaveAmps
+  // (at least one circuit each); per wave, gate items per pass index, then
+  // its observable items
   auto chunks = [](int n, int K, int32_t circ, int32_t pass, std::vector<BatchItem>& v) {
     const int64_t ntiles = 1ll << (n - K);
     for (int64_t t = 0; t < ntiles; t += kBatchSplitTiles)
       v.push_back(BatchItem{circ, pass, (int32_t)t, (int32_t)std::min<int64_t>(kBatchSplitTiles, ntiles - t)});
   };
+  uint64_t split_amps = 0;
   std::vector<BatchItem> items;
-  std::vector<std::pair<size_t, size_t>> gate_items;  // per pass index: (first item, count)
-  for (int p = 0; p < max_pass; ++p) {
-    const size_t at = items.size();
-    for (size_t i = nqueue; i < nc; ++i) {
-      const size_t k = order[i];
-      if (p < bc[k].npass) chunks(bc[k].n, cps[k].bp[p].K, (int32_t)i, p, items);
+  std::vector<BatchPlan::SplitLaunch> launches;
+  std::vector<int32_t> ofirst(nc - nqueue), ocount(nc - nqueue);
+  size_t parts = 0;
+  for (size_t w0 = nqueue; w0 < nc;) {
+    size_t w1 = w0;
+    uint64_t wave_amps = 0;
+    int max_pass = 0;
+    while (w1 < nc && (w1 == w0 || wave_amps + bc[order[w1]].amps <= kBatchSplitWaveAmps)) {
+      BatchCircuit& C = bc[order[w1]];
+      C.off = wave_amps;  // the wave's states share one region, reused by the next wave
+      wave_amps += C.amps;
+      max_pass = std::max(max_pass, (int)C.npass);
+      ++w1;
     }
-    gate_items.push_back({at, items.size() - at});
-  }
-  const size_t obs_at = items.size();
-  std::vector<int32_t> ofirst, ocount;
-  for (size_t i = nqueue; i < nc; ++i) {
-    const size_t k = order[i];
+    split_amps = std::max(split_amps, wave_amps);
+    for (int p = 0; p < max_pass; ++p) {
+      const size_t at = items.size();
+      for (size_t i = w0; i < w1; ++i) {
+        const size_t k = order[i];
+        if (p < bc[k].npass) chunks(bc[k].n, cps[k].bp[p].K, (int32_t)i, p, items);
+      }
+      if (items.size() > at) launches.push_back({at, items.size() - at, false, 0});
+    }
     const size_t at = items.size();
-    for (int rp = 0; rp < bc[k].nred; ++rp) chunks(bc[k].n, cps[k].bp[bc[k].npass + rp].K, (int32_t)i, rp, items);
-    ofirst.push_back((int32_t)(at - obs_at));
-    ocount.push_back((int32_t)(items.size() - at));
+    for (size_t i = w0; i < w1; ++i) {
+      const size_t k = order[i];
+      const size_t c0 = items.size();
+      for (int rp = 0; rp < bc[k].nred; ++rp) chunks(bc[k].n, cps[k].bp[bc[k].npass + rp].K, (int32_t)i, rp, items);
+      ofirst[i - nqueue] = (int32_t)(parts + (c0 - at));
+      ocount[i - nqueue] = (int32_t)(items.size() - c0);
+    }
+    if (items.size() > at) launches.push_back({at, items.size() - at, true, parts});
+    parts += items.size() - at;
+    w0 = w1;
   }
   const size_t nsplit = nc - nqueue;
   // one table image: [circuits][passes][program][terms][counter][items][first][count]
@@ -3552,8 +3564,8 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   plan.oit = oit;
   plan.ofirst = ofi;
   plan.ocount = ocn;
-  plan.gate_items = gate_items;
-  plan.obs_items = {obs_at, items.size() - obs_at};
+  plan.split_launches = launches;
+  plan.split_parts = parts;
   plan.program_ops = pg_at[nc];
   st.circuits = circuits.size();
   st.state_bytes = (double)(slot_amps + split_amps) * 8.0;
@@ -3621,8 +3633,7 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   const uint64_t slots_amps = plan.slot_amps * (uint64_t)std::max(grid, 1);
   const uint64_t state_amps = slots_amps + plan.split_amps;
   const size_t nres = circuits.size();
-  const size_t nobs = plan.obs_items.second;
-  const size_t out_doubles = ((nres + 31) & ~size_t(31)) + nobs;  // results, then observable item parts
+  const size_t out_doubles = ((nres + 31) & ~size_t(31)) + plan.split_parts;  // results, then item parts
   // growing a device buffer frees memory a queued batch may still use
   if ((state_amps * 8 > b->states.bytes || plan.dev_total > sl.tables.bytes || out_doubles * 8 + 8 > sl.out.bytes)) {
     if ((e = batch_synchronize(b)) != NQPU_OK) return e;
@@ -3671,23 +3682,22 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
     launch_batch(b->states.p, plan.slot_amps, dcirc, (int)plan.nqueue, dpass, dprog, dterm, dblob,
                  (unsigned*)(base + plan.ocnt), (double*)sl.out.p, grid, b->stream);
   HIPTRY(hipGetLastError(), "batch launch");
-  // split circuits: one launch per gate pass index, then every observable
-  // pass, then their sums
+  // split circuits: per wave, one launch per gate pass index, then its
+  // observable items; then every split circuit's sum of its parts
   float2* split_states = (float2*)b->states.p + slots_amps;
-  for (const auto& gi : plan.gate_items)
-    if (gi.second) {
-      launch_batch_split_gate(split_states, dcirc, dpass, dprog, dblob, ditems + gi.first, (int)gi.second,
-                              b->stream);
-      HIPTRY(hipGetLastError(), "split circuit pass launch");
-    }
-  const int nsplit = (int)(plan.ncirc - plan.nqueue);
-  if (nsplit) {
-    double* part = (double*)sl.out.p + ((nres + 31) & ~size_t(31));
-    if (nobs) {
-      launch_batch_split_obs(split_states, dcirc, dpass, dterm, ditems + plan.obs_items.first, (int)nobs, part,
+  double* part = (double*)sl.out.p + ((nres + 31) & ~size_t(31));
+  for (const BatchPlan::SplitLaunch& l : plan.split_launches) {
+    if (l.obs) {
+      launch_batch_split_obs(split_states, dcirc, dpass, dterm, ditems + l.first, (int)l.count, part + l.part0,
                              b->stream);
       HIPTRY(hipGetLastError(), "split circuit observable launch");
+    } else {
+      launch_batch_split_gate(split_states, dcirc, dpass, dprog, dblob, ditems + l.first, (int)l.count, b->stream);
+      HIPTRY(hipGetLastError(), "split circuit pass launch");
     }
+  }
+  const int nsplit = (int)(plan.ncirc - plan.nqueue);
+  if (nsplit) {
     launch_batch_split_out(dcirc, (int)plan.nqueue, nsplit, (const int32_t*)(base + plan.ofirst),
                            (const int32_t*)(base + plan.ocount), part, (double*)sl.out.p, b->stream);
     HIPTRY(hipGetLastError(), "split circuit result launch");

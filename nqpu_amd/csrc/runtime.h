@@ -329,15 +329,22 @@ struct BatchPlan {
   uint64_t slot_amps = 0;  // largest queued HBM-resident circuit: each workgroup's state slot
   size_t ncirc = 0, program_ops = 0;
   // split circuits (kBatchSplitQubits and more) follow the nqueue queued ones
-  // in the circuit table; their states take split_amps amplitudes after the
-  // workgroup slots; work items at oit (gate passes per pass index, then the
-  // observable passes), per split circuit its first observable item and
-  // count at ofirst / ocount
+  // in the circuit table.  They run in waves whose states fit
+  // kBatchSplitWaveAmps amplitudes (split_amps: the largest wave) after the
+  // workgroup slots; each wave's launches run its gate passes (one launch
+  // per pass index) and then its observable passes, which write one partial
+  // sum per item.  Work items at oit; per split circuit its first observable
+  // item (counted over all waves) and count at ofirst / ocount.
+  struct SplitLaunch {
+    size_t first, count;  // work items
+    bool obs;             // observable items (else gate items)
+    size_t part0;         // observable items: index of the first one's partial sum
+  };
   size_t nqueue = 0;
   uint64_t split_amps = 0;
   size_t oit = 0, ofirst = 0, ocount = 0;
-  std::vector<std::pair<size_t, size_t>> gate_items;  // (first item, count) per pass index
-  std::pair<size_t, size_t> obs_items{0, 0};
+  std::vector<SplitLaunch> split_launches;
+  size_t split_parts = 0;  // observable items over all waves
   NqpuBatchStats st{};
 };
 // buffer(bytes) returns at least `bytes` of host memory for the table image

@@ -51,3 +51,16 @@ def test_error_of_one_circuit_is_reported():
     with pytest.raises(L.NqpuError) as e:
         Batch.plan(circs, obs)
     assert e.value.code == 2  # INVALID_PARAMETER: batched circuits need at least 4 qubits
+
+
+def test_split_circuits_run_in_waves_of_bounded_memory():
+    """Circuits of 18 qubits and more get states of their own, in waves that
+    fit 2^30 complex64 amplitudes (8 GiB): 1100 circuits of 20 qubits need two
+    waves, and the plan's state memory is one wave's."""
+    vcs = variational_batch(1100, seed=23, nmin=20, nmax=20, lmin=1, lmax=1)
+    circs = [v.circuit.native() for v in vcs]
+    obs = [v.hamiltonian() for v in vcs]
+    p = Batch.plan(circs, obs)
+    assert p["state_bytes"] == 8.0 * (1 << 30)
+    few = Batch.plan(circs[:100], obs[:100])
+    assert few["state_bytes"] == 8.0 * 100 * (1 << 20)

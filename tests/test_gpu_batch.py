@@ -141,3 +141,17 @@ def test_split_circuits_alone_equal_mixed_and_oracle(batch, oracle):
     # a split circuit without observables gets 0
     e = batch.expectation([circs[big[0]], circs[0]], [[], obs[0]])
     assert e[0] == 0.0 and e[1] == mixed[0]
+
+
+def test_split_circuits_in_two_waves(batch):
+    """1100 circuits of 20 qubits exceed one wave's state memory (1024 of
+    them): the batch runs them in two waves over the same states, and every
+    energy equals the one from a batch of one wave."""
+    vcs = variational_batch(1100, seed=23, nmin=20, nmax=20, lmin=1, lmax=1)
+    circs = [v.circuit for v in vcs]
+    obs = [v.hamiltonian() for v in vcs]
+    both = batch.expectation(circs, obs)
+    assert batch.last_stats()["state_bytes"] == 8.0 * (1 << 30)
+    first = batch.expectation(circs[:600], obs[:600])
+    last = batch.expectation(circs[600:], obs[600:])
+    assert np.array_equal(both, np.concatenate([first, last]))
