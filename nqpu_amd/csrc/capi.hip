@@ -699,10 +699,10 @@ NqpuError nqpu_debug_batch_plan(const NqpuCircuit* const* circuits, uint64_t cou
   NqpuError e = batch_args(circuits, count, terms, offsets, opts, cs, ts, o);
   if (e != NQPU_OK) return e;
   BatchPlan plan;
-  std::vector<char> image;
+  std::unique_ptr<char[]> image;  // left uninitialised, like the pinned image of a real batch
   e = plan_batch(cs, ts, o, plan, [&image](size_t bytes) {
-    image.resize(bytes);
-    return image.data();
+    image.reset(new (std::nothrow) char[bytes]);
+    return image.get();
   });
   if (e != NQPU_OK) return e;
   *out = plan.st;

@@ -260,25 +260,30 @@ static bool is_identity2(const cd* m, double eps) {  // is_identity_matrix, gate
 // Per-qubit accumulation of consecutive uncontrolled single-qubit ops into
 // one 2x2 (the single-qubit part of fuse_gates, gate_fusion.rs:537-666).
 std::vector<HostOp> merge_single_qubit(const std::vector<HostOp>& ops) {
-  struct Pend { bool live = false; cd m[4]; int count = 0; HostOp first; };
-  std::vector<Pend> pend(64);
+  struct Pend { cd m[4]; int count = 0; size_t first = 0; };  // count 0: nothing pending
+  Pend pend[64];
+  uint64_t live = 0;
   std::vector<HostOp> out;
   out.reserve(ops.size());
   auto flush = [&](int q) {
     Pend& p = pend[q];
-    if (!p.live) return;
-    p.live = false;
-    if (p.count == 1) { out.push_back(p.first); return; }
+    live &= ~bit(q);
+    const int count = p.count;
+    p.count = 0;
+    if (count == 1) { out.push_back(ops[p.first]); return; }
     if (is_identity2(p.m, 1e-12)) return;
     if (p.m[1] == cd(0) && p.m[2] == cd(0)) out.push_back(op_d1(q, 0, p.m[0], p.m[3]));
     else out.push_back(op_u1(q, 0, p.m));
   };
-  for (const HostOp& o : ops) {
+  for (size_t k = 0; k < ops.size(); ++k) {
+    const HostOp& o = ops[k];
     cd m[4];
     if (op_matrix_1q(o, m)) {
       Pend& p = pend[o.a];
-      if (!p.live) {
-        p.live = true; p.count = 1; p.first = o;
+      if (!p.count) {
+        live |= bit(o.a);
+        p.count = 1;
+        p.first = k;
         for (int i = 0; i < 4; ++i) p.m[i] = m[i];
       } else {
         mul2(m, p.m, p.m);
@@ -286,12 +291,10 @@ std::vector<HostOp> merge_single_qubit(const std::vector<HostOp>& ops) {
       }
       continue;
     }
-    uint64_t qm = o.qubit_mask();
-    for (int q = 0; q < 64; ++q)
-      if ((qm >> q) & 1) flush(q);
+    for (uint64_t qm = o.qubit_mask() & live; qm; qm &= qm - 1) flush(__builtin_ctzll(qm));
     out.push_back(o);
   }
-  for (int q = 0; q < 64; ++q) flush(q);
+  for (uint64_t qm = live; qm; qm &= qm - 1) flush(__builtin_ctzll(qm));
   return out;
 }
 
@@ -1433,21 +1436,22 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
     const size_t at = out.qtab.size();
     out.qtab.resize(at + (len + unit - 1) / unit * unit, (R)0);
     R* T = out.qtab.data() + at;
+    // per thread: the sum for tid is the one for tid without its lowest set
+    // bit t, plus t's linear term and its couplings to the other set bits
+    // (all higher than t)
+    double tc[256], tr[8][256];
+    tc[0] = g.c0;
+    for (int j = 0; j < eb; ++j) tr[j][0] = lr[j];
+    for (int tid = 1; tid < 256; ++tid) {
+      const int t = __builtin_ctz(tid), rest = tid & (tid - 1);
+      double c = tc[rest] + lt[t];
+      for (int m = rest; m; m &= m - 1) c += att[t][__builtin_ctz(m)];
+      tc[tid] = c;
+      for (int j = 0; j < eb; ++j) tr[j][tid] = tr[j][rest] + atr[t][j];
+    }
     for (int tid = 0; tid < 256; ++tid) {
-      double c = g.c0;
-      for (int t = 0; t < 8; ++t) {
-        if (!((tid >> t) & 1)) continue;
-        c += lt[t];
-        for (int u = t + 1; u < 8; ++u)
-          if ((tid >> u) & 1) c += att[t][u];
-      }
-      T[tid] = fr(c);
-      for (int j = 0; j < eb; ++j) {
-        double l = lr[j];
-        for (int t = 0; t < 8; ++t)
-          if ((tid >> t) & 1) l += atr[t][j];
-        T[256 * (1 + j) + tid] = fr(l);
-      }
+      T[tid] = fr(tc[tid]);
+      for (int j = 0; j < eb; ++j) T[256 * (1 + j) + tid] = fr(tr[j][tid]);
     }
     RegOp<R> r{};
     r.kind = R_QPHASE;
