@@ -51,6 +51,25 @@ enum RegOpKind : int32_t {
 
 constexpr int kMfmaBlobBytes = 64 * 64;  // 64 lanes x (2 K-steps x (hi, lo) x 8 fp16)
 constexpr int kMfmaMatrixBytes = 256 * 8;  // a block's 16x16 complex64 matrix (the blob's source)
+
+// A matrix-core block recorded as its row operations (batched circuits): a
+// stream of 64-bit words per block, composed into the 16x16 matrix on the
+// device in FP64.  Each op is a header word, then `nvals` FP64 values
+// (re, im interleaved).  Header: kind | pa << 8 | pb << 12 | cmk << 16 |
+// nvals << 32.  pa, pb: block index bits (0..3); the op acts on the rows i
+// with (i & cmk) == cmk.
+enum BlockRowKind : uint32_t {
+  BR_U1 = 0,     // 2x2 on bit pa (4 values)
+  BR_U2 = 1,     // 4x4 on bits (pa, pb), sub-index bit_pa | bit_pb << 1 (16 values)
+  BR_FLIP = 2,   // X on bit pa
+  BR_SWAP = 3,   // exchange bits pa and pb
+  BR_SCALE0 = 4, // every row times one factor (1 value)
+  BR_SCALE1 = 5, // row i times f[bit_pa(i)] (2 values)
+  BR_SCALE2 = 6  // row i times f[bit_pa(i) | bit_pb(i) << 1] (4 values)
+};
+constexpr uint64_t block_row_header(uint32_t kind, int pa, int pb, uint32_t cmk, int nvals) {
+  return (uint64_t)kind | (uint64_t)pa << 8 | (uint64_t)pb << 12 | (uint64_t)cmk << 16 | (uint64_t)nvals << 32;
+}
 constexpr int kQPhaseChunkFloats = 256 * 16;  // one chunk table
 
 template <typename R>

@@ -129,6 +129,8 @@ void launch_batch_split_out(const BatchCircuit* circs, int circ0, int nsplit, co
 // (kMfmaMatrixBytes each, row-major in block-index order) into consecutive
 // kMfmaBlobBytes units: the layout and fp16 high/low split reg_mfma reads.
 void launch_blob_split(const float2* mats, unsigned char* blobs, size_t nblk, hipStream_t s);
+// Matrices of nblk blocks from their row-operation words (BR_* records).
+void launch_block_compose(const uint32_t* index, const uint64_t* words, float2* mats, size_t nblk, hipStream_t s);
 
 // ----------------------------------------------------------- kernel launches
 // All launches are asynchronous on `s`; R = float (c64) or double (c128).

@@ -1050,6 +1050,11 @@ template <typename R>
 struct RegProgram {
   std::vector<RegOp<R>> ops;
   std::vector<std::array<cd, 256>> blocks;
+  // with defer_blocks: each block's row operations instead of its matrix
+  // (block k's words start at block_at[k] in block_words; BR_* records)
+  std::vector<uint64_t> block_words;
+  std::vector<uint32_t> block_at;
+  size_t nblocks() const { return blocks.size() + block_at.size(); }
   uint64_t fused_ops = 0;  // device ops inside those blocks
   // R_QPHASE tables, in operand units of kMfmaBlobBytes (an op's s0 counts
   // units from the start of qtab; the device program places them after the
@@ -1074,7 +1079,7 @@ template <typename R>
 // long as thread slots 0..4 hold the tile's five lowest bits again (so the
 // store is as coalesced as the load).
 RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false, bool xchg = false,
-                                int max_blobs = 1 << 20, bool qphase = false) {
+                                int max_blobs = 1 << 20, bool qphase = false, bool defer_blocks = false) {
   const int K = (int)p.qubits.size();  // == tb + eb
   // tile bits stored to the five lowest positions of the tile (the bits
   // loaded from there unless the pass permutes its positions)
@@ -1772,7 +1777,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       if ((b.g >> q) & 1) sel[ns++] = q;
     RegOp<R> r{};
     r.kind = R_MFMA;
-    r.s0 = (int32_t)out.blocks.size();
+    r.s0 = (int32_t)out.nblocks();
     r.pad = ns;
     r.s1 = (ns > 0 ? sel[0] : 0) | (ns > 1 ? sel[1] << 8 : 0);
     for (int var = 0; var < (1 << ns); ++var) {
@@ -1783,6 +1788,53 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
       auto bitat = [&](int x, int i) -> int {
         return x >= kGlobalBit ? (int)((gset >> (x - kGlobalBit)) & 1) : (i >> pos[x]) & 1;
       };
+      if (defer_blocks) {
+        // the variant's row operations, with the qubits outside the tile
+        // resolved: a diagonal factor on them becomes a fixed factor
+        out.block_at.push_back((uint32_t)out.block_words.size());
+        auto put = [&](uint32_t kind, int pa, int pb, uint32_t cmk, std::initializer_list<cd> vals) {
+          out.block_words.push_back(block_row_header(kind, pa, pb, cmk, (int)vals.size() * 2));
+          for (const cd& z : vals) {
+            out.block_words.push_back(__builtin_bit_cast(uint64_t, z.real()));
+            out.block_words.push_back(__builtin_bit_cast(uint64_t, z.imag()));
+          }
+        };
+        for (int k : b.ops) {
+          const L& o = ops[k];
+          if ((o.gc & ~gset) != 0) continue;  // a control outside the tile is 0 in this variant
+          const HostOp& h = *o.h;
+          uint32_t cmk = 0;
+          for (int q = 0; q < K; ++q)
+            if ((o.cm >> q) & 1) cmk |= 1u << pos[q];
+          const bool ina = o.a < kGlobalBit, inb = two(o.kind) && o.b < kGlobalBit;
+          const int pa = ina ? pos[o.a] : 0, pb = inb ? pos[o.b] : 0;
+          switch (o.kind) {
+            case OP_U1: put(BR_U1, pa, 0, cmk, {h.m[0], h.m[1], h.m[2], h.m[3]}); break;
+            case OP_FLIP: put(BR_FLIP, pa, 0, cmk, {}); break;
+            case OP_SWAP: put(BR_SWAP, pa, pb, cmk, {}); break;
+            case OP_U2: {
+              out.block_words.push_back(block_row_header(BR_U2, pa, pb, cmk, 32));
+              for (int e = 0; e < 16; ++e) {
+                out.block_words.push_back(__builtin_bit_cast(uint64_t, h.m[e].real()));
+                out.block_words.push_back(__builtin_bit_cast(uint64_t, h.m[e].imag()));
+              }
+              break;
+            }
+            case OP_D1:
+              if (ina) put(BR_SCALE1, pa, 0, cmk, {h.m[0], h.m[1]});
+              else put(BR_SCALE0, 0, 0, cmk, {h.m[bitat(o.a, 0)]});
+              break;
+            case OP_D2:
+              if (ina && inb) put(BR_SCALE2, pa, pb, cmk, {h.m[0], h.m[1], h.m[2], h.m[3]});
+              else if (ina) put(BR_SCALE1, pa, 0, cmk, {h.m[bitat(o.b, 0) << 1], h.m[1 | bitat(o.b, 0) << 1]});
+              else if (inb) put(BR_SCALE1, pb, 0, cmk, {h.m[bitat(o.a, 0)], h.m[bitat(o.a, 0) | 2]});
+              else put(BR_SCALE0, 0, 0, cmk, {h.m[bitat(o.a, 0) | bitat(o.b, 0) << 1]});
+              break;
+            default: break;
+          }
+        }
+        continue;
+      }
       // M = op_k ... op_1: each op is a row operation on M (row: block index
       // of the output, column: of the input), applied to all 16 columns at
       // once, with separate real and imaginary planes
@@ -3233,6 +3285,7 @@ struct CircuitPlan {
   // phase tables (qunits operand units of kMfmaBlobBytes, in pass order)
   std::vector<RegProgram<float>> progs;
   size_t nblk = 0, qunits = 0;
+  size_t bwords = 0;  // the blocks' row-operation words (composed on the device)
   double cost = 0;
   uint64_t amps = 0;  // HBM amplitudes (0 when resident)
 };
@@ -3301,15 +3354,17 @@ CircuitPlan plan_batch_circuit(const Circuit& c, const std::vector<PauliTerm>& t
     // matrix-core blocks and quadratic phases as in the standalone passes;
     // LDS remaps only (the kernel loads and stores tiles in the canonical
     // slot order)
-    RegProgram<float> pr = build_reg_program<float>(p, EB, 8, true, false, 1 << 20, true);
+    // (blocks as row operations, composed on the device)
+    RegProgram<float> pr = build_reg_program<float>(p, EB, 8, true, false, 1 << 20, true, true);
     for (RegOp<float>& r : pr.ops)  // the circuit's phase-table unit (plan_batch rebases it)
       if (r.kind == R_QPHASE) r.s0 += (int32_t)cp.qunits;
     // the circuit's first block of this pass (plan_batch makes it global);
-    // block matrices and phase tables go into the upload image straight from
-    // these programs
+    // block row operations and phase tables go into the upload image straight
+    // from these programs
     P.blob_unit = (int32_t)cp.nblk;
-    P.scaled = pr.blocks.empty() ? 0 : 1;
-    cp.nblk += pr.blocks.size();
+    P.scaled = pr.nblocks() ? 1 : 0;
+    cp.nblk += pr.nblocks();
+    cp.bwords += pr.block_words.size();
     cp.qunits += pr.qtab.size() * sizeof(float) / kMfmaBlobBytes;
     P.op0 = (int)cp.prog.size();
     P.nops = (int)pr.ops.size();
@@ -3388,6 +3443,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   const size_t nc = cps.size();
   // offsets of each circuit's passes, program ops and terms in the tables
   std::vector<size_t> bp_at(nc + 1, 0), pg_at(nc + 1, 0), bt_at(nc + 1, 0), bl_at(nc + 1, 0), qt_at(nc + 1, 0);
+  std::vector<size_t> bw_at(nc + 1, 0);
   std::vector<BatchCircuit> bc(nc);
   uint64_t slot_amps = 0;
   NqpuBatchStats st{};
@@ -3402,6 +3458,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
     bt_at[k + 1] = bt_at[k] + cp.bt.size();
     bl_at[k + 1] = bl_at[k] + cp.nblk;    // matrix-core blocks
     qt_at[k + 1] = qt_at[k] + cp.qunits;  // phase-table units
+    bw_at[k + 1] = bw_at[k] + cp.bwords;  // block row-operation words
     BatchCircuit C = cp.C;
     C.pass0 += (int32_t)bp_at[k];
     C.red0 += (int32_t)bp_at[k];
@@ -3487,10 +3544,13 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   const size_t ofi = up(oit + items.size() * sizeof(BatchItem));
   const size_t ocn = up(ofi + nsplit * sizeof(int32_t));
   const size_t head = up(ocn + nsplit * sizeof(int32_t));  // operand units follow, 256-byte aligned
-  const size_t nblk = bl_at[nc], nq = qt_at[nc];
-  const size_t hq = head, hm = hq + nq * kMfmaBlobBytes;  // host image: phase tables, matrices
-  const size_t total = hm + nblk * kMfmaMatrixBytes;
-  if (nblk + nq > (size_t)INT32_MAX) {
+  const size_t nblk = bl_at[nc], nq = qt_at[nc], nw = bw_at[nc];
+  // host image after the head: phase tables, the blocks' word index
+  // (nblk + 1 entries), their row-operation words; the device copy of it
+  // follows the blocks' operand blobs
+  const size_t hq = head, hbi = hq + nq * kMfmaBlobBytes, hbw = up(hbi + (nblk + 1) * sizeof(uint32_t));
+  const size_t total = hbw + nw * sizeof(uint64_t);
+  if (nblk + nq > (size_t)INT32_MAX || nw > (size_t)UINT32_MAX) {
     set_error("batch operand units exceed the 32-bit unit index");
     return NQPU_INVALID_PARAMETER;
   }
@@ -3528,10 +3588,14 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
           std::memcpy(dp + i, &P, sizeof(BatchPass));
         }
         char* u = host + hq + qt_at[k] * kMfmaBlobBytes;
-        float2* mm = reinterpret_cast<float2*>(host + hm) + bl_at[k] * 256;
+        uint32_t* bi = reinterpret_cast<uint32_t*>(host + hbi) + bl_at[k];
+        uint64_t* bw = reinterpret_cast<uint64_t*>(host + hbw) + bw_at[k];
+        size_t w = bw_at[k];
         for (const RegProgram<float>& pr : cp.progs) {
-          for (const auto& M : pr.blocks)
-            for (int e = 0; e < 256; ++e) *mm++ = make_float2((float)M[e].real(), (float)M[e].imag());
+          for (uint32_t at : pr.block_at) *bi++ = (uint32_t)(w + at);
+          if (!pr.block_words.empty()) std::memcpy(bw, pr.block_words.data(), pr.block_words.size() * sizeof(uint64_t));
+          bw += pr.block_words.size();
+          w += pr.block_words.size();
           if (!pr.qtab.empty()) {
             std::memcpy(u, pr.qtab.data(), pr.qtab.size() * sizeof(float));
             u += pr.qtab.size() * sizeof(float);
@@ -3548,6 +3612,7 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
     fill();
     for (auto& x : th) x.join();
   }
+  reinterpret_cast<uint32_t*>(host + hbi)[nblk] = (uint32_t)nw;
   plan.host = host;
   plan.oc = oc;
   plan.op = op_;
@@ -3558,7 +3623,9 @@ NqpuError plan_batch(const std::vector<const Circuit*>& circuits, const std::vec
   plan.nblk = nblk;
   plan.qunits = nq;
   plan.dev_q = head + nblk * kMfmaBlobBytes;
-  plan.dev_m = plan.dev_q + nq * kMfmaBlobBytes;
+  plan.dev_bi = plan.dev_q + (hbi - hq);
+  plan.dev_bw = plan.dev_q + (hbw - hq);
+  plan.dev_m = up(plan.dev_q + (total - hq));
   plan.dev_total = plan.dev_m + nblk * kMfmaMatrixBytes;
   plan.total = total;
   plan.slot_amps = slot_amps;
@@ -3672,6 +3739,9 @@ NqpuError batch_expectation(Batch* b, const std::vector<const Circuit*>& circuit
   HIPTRY(hipEventRecord(sl.uploaded, b->copy_stream), "batch upload event");
   HIPTRY(hipStreamWaitEvent(b->stream, sl.uploaded, 0), "batch upload wait");
   if (plan.nblk) {
+    launch_block_compose((const uint32_t*)(base + plan.dev_bi), (const uint64_t*)(base + plan.dev_bw),
+                         (float2*)(base + plan.dev_m), plan.nblk, b->stream);
+    HIPTRY(hipGetLastError(), "block composition launch");
     launch_blob_split((const float2*)(base + plan.dev_m), (unsigned char*)(base + plan.head), plan.nblk, b->stream);
     HIPTRY(hipGetLastError(), "operand blob launch");
   }

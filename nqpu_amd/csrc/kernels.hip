@@ -741,6 +741,111 @@ void launch_batch_split_out(const BatchCircuit* circs, int circ0, int nsplit, co
                      count, part, out);
 }
 
+// Matrix-core blocks of a batch composed from their row operations (BR_*
+// word streams; block b's words are [index[b], index[b + 1])) in FP64, then
+// stored as complex64 16x16 matrices (row-major, row = output index): 16
+// threads per block, one column each, the column in LDS.  Row operations act
+// on each column alone, so the threads never wait for each other.
+constexpr int kComposeBlocks = 8;  // blocks per workgroup
+
+__device__ __forceinline__ double2 cfma2(double2 a, double2 x, double2 b, double2 y) {  // a x + b y
+  return make_double2(a.x * x.x - a.y * x.y + b.x * y.x - b.y * y.y, a.x * x.y + a.y * x.x + b.x * y.y + b.y * y.x);
+}
+
+__global__ __launch_bounds__(16 * kComposeBlocks) void k_block_compose(const uint32_t* __restrict__ index,
+                                                                        const uint64_t* __restrict__ words,
+                                                                        float2* __restrict__ mats, uint64_t nblk) {
+  __shared__ double2 sm[kComposeBlocks * 256];
+  const int lb = threadIdx.x >> 4, c = threadIdx.x & 15;
+  const uint64_t blk = (uint64_t)blockIdx.x * kComposeBlocks + lb;
+  if (blk >= nblk) return;  // (no barriers below)
+  double2* M = sm + lb * 256;  // M[16 i + c]: row i of this thread's column c
+  for (int i = 0; i < 16; ++i) M[16 * i + c] = make_double2(i == c ? 1.0 : 0.0, 0.0);
+  const uint64_t* w = words + index[blk];
+  const uint64_t* const end = words + index[blk + 1];
+  while (w < end) {
+    const uint64_t h = *w++;
+    const uint32_t kind = (uint32_t)(h & 0xFF), cmk = (uint32_t)((h >> 16) & 0xFFFF);
+    const int pa = (int)((h >> 8) & 3), pb = (int)((h >> 12) & 3), nv = (int)((h >> 32) & 0xFF);
+    const double* v = reinterpret_cast<const double*>(w);
+    w += nv;
+    auto val = [&](int k) { return make_double2(v[2 * k], v[2 * k + 1]); };
+    switch (kind) {
+      case BR_U1: {
+        const double2 m0 = val(0), m1 = val(1), m2 = val(2), m3 = val(3);
+        for (int i = 0; i < 16; ++i) {
+          if ((i & cmk) != cmk || ((i >> pa) & 1)) continue;
+          const int j = i | (1 << pa);
+          const double2 x = M[16 * i + c], y = M[16 * j + c];
+          M[16 * i + c] = cfma2(m0, x, m1, y);
+          M[16 * j + c] = cfma2(m2, x, m3, y);
+        }
+        break;
+      }
+      case BR_U2: {
+        for (int i = 0; i < 16; ++i) {
+          if ((i & cmk) != cmk || ((i >> pa) & 1) || ((i >> pb) & 1)) continue;
+          const int idx[4] = {i, i | (1 << pa), i | (1 << pb), i | (1 << pa) | (1 << pb)};
+          double2 x[4];
+          for (int t = 0; t < 4; ++t) x[t] = M[16 * idx[t] + c];
+          for (int r = 0; r < 4; ++r) {
+            double sr = 0, si = 0;
+            for (int t = 0; t < 4; ++t) {
+              const double2 m = val(4 * r + t);
+              sr += m.x * x[t].x - m.y * x[t].y;
+              si += m.x * x[t].y + m.y * x[t].x;
+            }
+            M[16 * idx[r] + c] = make_double2(sr, si);
+          }
+        }
+        break;
+      }
+      case BR_FLIP:
+        for (int i = 0; i < 16; ++i) {
+          if ((i & cmk) != cmk || ((i >> pa) & 1)) continue;
+          const int j = i | (1 << pa);
+          const double2 x = M[16 * i + c];
+          M[16 * i + c] = M[16 * j + c];
+          M[16 * j + c] = x;
+        }
+        break;
+      case BR_SWAP:
+        for (int i = 0; i < 16; ++i) {
+          if ((i & cmk) != cmk || !((i >> pa) & 1) || ((i >> pb) & 1)) continue;
+          const int j = (i ^ (1 << pa)) | (1 << pb);
+          const double2 x = M[16 * i + c];
+          M[16 * i + c] = M[16 * j + c];
+          M[16 * j + c] = x;
+        }
+        break;
+      case BR_SCALE0:
+      case BR_SCALE1:
+      case BR_SCALE2:
+        for (int i = 0; i < 16; ++i) {
+          if ((i & cmk) != cmk) continue;
+          const int k = kind == BR_SCALE0 ? 0
+                        : kind == BR_SCALE1 ? ((i >> pa) & 1)
+                                            : (((i >> pa) & 1) | (((i >> pb) & 1) << 1));
+          const double2 z = val(k), x = M[16 * i + c];
+          M[16 * i + c] = make_double2(z.x * x.x - z.y * x.y, z.x * x.y + z.y * x.x);
+        }
+        break;
+      default: break;
+    }
+  }
+  float2* out = mats + blk * 256;
+  for (int i = 0; i < 16; ++i) {
+    const double2 z = M[16 * i + c];
+    out[16 * i + c] = make_float2((float)z.x, (float)z.y);
+  }
+}
+
+void launch_block_compose(const uint32_t* index, const uint64_t* words, float2* mats, size_t nblk, hipStream_t s) {
+  const unsigned grid = (unsigned)((nblk + kComposeBlocks - 1) / kComposeBlocks);
+  hipLaunchKernelGGL(k_block_compose, dim3(grid), dim3(16 * kComposeBlocks), 0, s, index, words, mats,
+                     (uint64_t)nblk);
+}
+
 // Operand blob of block blockIdx.x from its 16x16 complex64 matrix M: lane l
 // of the blob holds A(l % 32, 16 s + 8 (l / 32) + i) for K-steps s = 0, 1 and
 // i = 0..7 of the 32x32 real form [[re, -im], [im, re]], as 8 fp16 high

@@ -325,7 +325,9 @@ struct BatchPlan {
   size_t oc = 0, op = 0, og = 0, ot = 0, ocnt = 0, total = 0;  // total: host image bytes
   size_t head = 0;                            // host = device offset of the blob area
   size_t nblk = 0, qunits = 0;                // matrix-core blocks, phase-table units
-  size_t dev_q = 0, dev_m = 0, dev_total = 0;  // device offsets: phase tables, matrices; size
+  // device offsets: phase tables, the blocks' word index and row-operation
+  // words, matrices (composed there on the device); size
+  size_t dev_q = 0, dev_bi = 0, dev_bw = 0, dev_m = 0, dev_total = 0;
   uint64_t slot_amps = 0;  // largest queued HBM-resident circuit: each workgroup's state slot
   size_t ncirc = 0, program_ops = 0;
   // split circuits (kBatchSplitQubits and more) follow the nqueue queued ones
