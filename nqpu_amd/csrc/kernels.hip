@@ -447,17 +447,41 @@ __device__ __forceinline__ double nsq(T a) {
 // amplitudes in plain (unswizzled) order whose global index is base | dep(t).
 __device__ double tile_terms(const float2* sm, uint32_t TT, const BatchTerm* terms, int nterm, uint64_t base,
                              int tid) {
+  // |a|^2 of this thread's amplitudes t = tid + 256 r, computed once for all
+  // diagonal terms (a tile holds at most 2^12 amplitudes: r < 16)
+  constexpr int kRows = 16;
+  const uint32_t nr = TT >= 256 ? (TT >> 8) : ((uint32_t)tid < TT ? 1u : 0u);
+  double p[kRows];
+  bool have_p = false;
   double acc = 0;
   for (int k = 0; k < nterm; ++k) {
     const BatchTerm tm = terms[k];
     const double gs = (__popcll(base & tm.zg) & 1) ? -1.0 : 1.0;
     double sr = 0, si = 0;
     if (tm.xl == 0) {
-      for (uint32_t t = tid; t < TT; t += 256) {
-        const float2 a = sm[t];
-        const double p = (double)a.x * a.x + (double)a.y * a.y;
-        sr += (__popc(t & tm.zl) & 1) ? -p : p;
+      if (!have_p) {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+          p[r] = 0.0;
+          if ((uint32_t)r < nr) {
+            const float2 a = sm[tid + 256 * r];
+            p[r] = (double)a.x * a.x + (double)a.y * a.y;
+          }
+        }
+        have_p = true;
       }
+#pragma unroll
+      for (int r = 0; r < kRows; ++r)
+        if ((uint32_t)r < nr) sr += (__popc((uint32_t)(tid + 256 * r) & tm.zl) & 1) ? -p[r] : p[r];
+    } else if (tm.zl == 0) {
+      // no Z factor inside the tile: s_i = s_j = 1, so sr = 2 Re sum and si = 0
+      const int h = 31 - __clz(tm.xl);
+      for (uint32_t k2 = tid; k2 < (TT >> 1); k2 += 256) {
+        const uint32_t i = ins0u(k2, h), j = i ^ tm.xl;
+        const float2 ai = sm[i], aj = sm[j];
+        sr += (double)aj.x * ai.x + (double)aj.y * ai.y;
+      }
+      sr *= 2.0;
     } else {
       const int h = 31 - __clz(tm.xl);
       for (uint32_t k2 = tid; k2 < (TT >> 1); k2 += 256) {
