@@ -501,6 +501,35 @@ __device__ double tile_terms(const float2* sm, uint32_t TT, const BatchTerm* ter
   return acc;
 }
 
+// Observable passes over 12-qubit tiles: the positions, relative to a tile's
+// base index, of this thread's 16 amplitudes t = tid + 256 r (tile bit j on
+// qubit q[j]), computed once per pass; then each tile is 16 independent
+// loads into LDS.
+constexpr int kObsTileQubits = 12;
+static_assert(kObsTileQubits == 8 + kBatchEB, "observable passes use the gate passes' tile width");
+
+__device__ __forceinline__ void obs_offsets(const int32_t* q, int tid, uint64_t (&off)[16]) {
+  uint64_t lo = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) lo |= (uint64_t)((tid >> j) & 1) << q[j];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    uint64_t hi = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) hi |= (uint64_t)((r >> j) & 1) << q[8 + j];
+    off[r] = lo | hi;
+  }
+}
+
+__device__ __forceinline__ void obs_load_tile(const float2* __restrict__ st, uint64_t base, const uint64_t (&off)[16],
+                                              float2* sm, int tid) {
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = st[base | off[r]];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) sm[tid + 256 * r] = v[r];
+}
+
 template <int EB>
 __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, uint64_t slot_amps,
                                                   const BatchCircuit* __restrict__ circs,
@@ -593,21 +622,18 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, u
         __syncthreads();
       }
       for (int rp = 0; rp < C.nred; ++rp) {
-        const BatchPass* P = passes + C.red0 + rp;
-        const int K = P->K;
-        const int32_t* q = P->q;  // read from memory: indexed by a loop bound known only at run time
-        const uint32_t TT = 1u << K;
-        const uint64_t ntiles = 1ull << (C.n - K);
+        const BatchPass* P = passes + C.red0 + rp;  // P->K == kObsTileQubits
+        const int32_t* q = P->q;
+        const uint64_t ntiles = 1ull << (C.n - kObsTileQubits);
+        uint64_t off[16];
+        obs_offsets(q, tid, off);
         for (uint64_t tile = 0; tile < ntiles; ++tile) {
           uint64_t base = tile;
-          for (int j = 0; j < K; ++j) base = ins0(base, q[j]);
-          for (uint32_t t = tid; t < TT; t += 256) {
-            uint64_t d = base;
-            for (int j = 0; j < K; ++j) d |= (uint64_t)((t >> j) & 1) << q[j];
-            sm[t] = st[d];
-          }
+#pragma unroll
+          for (int j = 0; j < kObsTileQubits; ++j) base = ins0(base, q[j]);
+          obs_load_tile(st, base, off, sm, tid);
           __syncthreads();
-          acc += tile_terms(sm, TT, terms + P->term0, P->nterm, base, tid);
+          acc += tile_terms(sm, 1u << kObsTileQubits, terms + P->term0, P->nterm, base, tid);
           __syncthreads();
         }
       }
@@ -713,20 +739,17 @@ __global__ __launch_bounds__(256) void k_batch_split_obs(const float2* __restric
   const BatchCircuit C = circs[it.circ];
   const BatchPass* P = passes + C.red0 + it.pass;
   const float2* st = states + C.off;
-  const int K = P->K;
-  const int32_t* q = P->q;
-  const uint32_t TT = 1u << K;
+  const int32_t* q = P->q;  // P->K == kObsTileQubits
+  uint64_t off[16];
+  obs_offsets(q, tid, off);
   double acc = 0;
   for (uint64_t tile = (uint64_t)it.tile0; tile < (uint64_t)(it.tile0 + it.ntile); ++tile) {
     uint64_t base = tile;
-    for (int j = 0; j < K; ++j) base = ins0(base, q[j]);
-    for (uint32_t t = tid; t < TT; t += 256) {
-      uint64_t d = base;
-      for (int j = 0; j < K; ++j) d |= (uint64_t)((t >> j) & 1) << q[j];
-      sm[t] = st[d];
-    }
+#pragma unroll
+    for (int j = 0; j < kObsTileQubits; ++j) base = ins0(base, q[j]);
+    obs_load_tile(st, base, off, sm, tid);
     __syncthreads();
-    acc += tile_terms(sm, TT, terms + P->term0, P->nterm, base, tid);
+    acc += tile_terms(sm, 1u << kObsTileQubits, terms + P->term0, P->nterm, base, tid);
     __syncthreads();
   }
   const double s = block_sum<256>(acc, red);
