@@ -366,9 +366,10 @@ static std::vector<Pass> plan_tiles_once(const std::vector<HostOp>& ops, int n, 
   // is planned on circuit qubits (its tile S must contain the qubits now at
   // the physical low positions) and emitted in physical positions.  It then
   // hands five qubits of S to the low positions (those shared with the best
-  // next window of K consecutive qubits), puts the rest of S back at their
-  // own positions where it can, and the run ends with passes that only
-  // permute, until pi is the identity again.
+  // next window of K consecutive qubits) and puts the rest of S back at
+  // their own positions where it can.  The plan then ends with passes that
+  // only permute, until pi is the identity again (closing_passes), or with
+  // po.keep_perm leaves the qubits where they are and reports pi.
 
   std::vector<int> pi(n), inv(n);
   for (int q = 0; q < n; ++q) pi[q] = inv[q] = q;
