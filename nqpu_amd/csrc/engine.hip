@@ -1574,7 +1574,8 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
   };
   std::vector<Item> items;
   std::vector<Blk> blks;
-  const bool use_mfma = mfma && eb == 4 && tb == 8;
+  // (register slot 3, and 4 with 13-qubit tiles, are batch bits of a block)
+  const bool use_mfma = mfma && (eb == 4 || eb == 5) && tb == 8;
   std::string choice_key;
   std::shared_ptr<const BlockChoice> choice;
   if (use_mfma) {
@@ -1976,7 +1977,7 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         first = &it;
         break;
       }
-    if (first->op < 0 && xchg && eb == 4) {
+    if (first->op < 0 && xchg && (eb == 4 || eb == 5)) {
       // the block's bits must sit on register slots 0..2 and lane slot 5.
       // Each bit elsewhere comes into a register slot whose bit is needed
       // last (one exchange); lane 5 is refilled from a register only when all
@@ -1993,25 +1994,26 @@ RegProgram<R> build_reg_program(const Pass& p, int eb, int tb, bool mfma = false
         const int from = find(lb);
         if (from == 5 || from >= 8) continue;
         int j = -1;
-        for (int r = 8; r < 12; ++r)
+        for (int r = 8; r < 8 + eb; ++r)
           if (!((B >> nm[r]) & 1) && (j < 0 || dist[nm[r]] > dist[nm[j]])) j = r;
         std::swap(nm[from], nm[j]);
       }
       if ((B >> nm[5]) & 1) {
         // lane 5 already holds a block bit
       } else if (__builtin_popcount(B) == 4) {
-        for (int r = 8; r < 12; ++r)
+        for (int r = 8; r < 8 + eb; ++r)
           if ((B >> nm[r]) & 1) {
             std::swap(nm[5], nm[r]);
             break;
           }
       }
-      if ((B >> nm[11]) & 1)  // register slot 3 is not part of the block
-        for (int r = 8; r < 11; ++r)
-          if (!((B >> nm[r]) & 1)) {
-            std::swap(nm[11], nm[r]);
-            break;
-          }
+      for (int h = 11; h < 8 + eb; ++h)  // register slots 3.. are not part of the block
+        if ((B >> nm[h]) & 1)
+          for (int r = 8; r < 11; ++r)
+            if (!((B >> nm[r]) & 1)) {
+              std::swap(nm[h], nm[r]);
+              break;
+            }
       remap_to(nm, dist);
       continue;
     }
@@ -2327,7 +2329,7 @@ static void emulate_reg_pass(std::vector<cd>& a, int n, const Pass& p, const Reg
         std::vector<cd> nv(v);
         for (int w = 0; w < NTH; w += 64)
           for (int col = 0; col < 32; ++col)
-            for (int e = 0; e < 2; ++e) {
+            for (int e = 0; e < (E >> 3); ++e) {
               cd x[16];
               for (int ai = 0; ai < 16; ++ai) {
                 const int tid = w + col + 32 * ((ai >> 2) & 1);
@@ -2590,9 +2592,9 @@ static NqpuError prepare_reg(const std::vector<Pass>& passes, int n, int eb, boo
   }
   const size_t pb = (std::max<size_t>(prog.size(), 1) * sizeof(RegOp<R>) + 255) & ~size_t(255);
   std::vector<hipFunction_t> fns;
-  // compiled passes: complex64 12-qubit tiles (matrix-core blocks need EB = 4)
-  // and complex128 register tiles of any width
-  if (compile && (std::is_same<R, double>::value || eb == 4)) {
+  // compiled passes: complex64 12- and 13-qubit tiles (matrix-core blocks
+  // need EB = 4 or 5) and complex128 register tiles of any width
+  if (compile && (std::is_same<R, double>::value || eb == 4 || eb == 5)) {
     const NqpuError e = jit_pass_functions<R>(prog, args, eb, &fns, &out->hold);
     if (e != NQPU_OK) return e;
   }
@@ -2793,8 +2795,8 @@ static NqpuError jit_check_circuit_t(const Circuit& c, const NqpuRunOptions& o) 
   PlanOptions po = plan_options(o, prec, c.n);
   po.keep_perm = true;  // the passes a run compiles
   const int K = std::min(po.tile_qubits, c.n);
-  if (!po.registers || (c64 ? K != 12 : (K < 11 || K > 13))) {
-    set_error(c64 ? "compiled passes need complex64 register tiles of 12 qubits"
+  if (!po.registers || (c64 ? (K != 12 && K != 13) : (K < 11 || K > 13))) {
+    set_error(c64 ? "compiled passes need complex64 register tiles of 12 or 13 qubits"
                   : "compiled passes need complex128 register tiles of 11-13 qubits");
     return NQPU_INVALID_PARAMETER;
   }

@@ -433,8 +433,9 @@ __device__ __forceinline__ void reg_qphase(T (&v)[1 << EB], const RegOp<R>& op, 
   }
 }
 
-// R_MFMA: a dense 16x16 complex block on the matrix cores.  Per register
-// half e (register bit 3), the wavefront's 32 columns (lane bits 0..4) are 32
+// R_MFMA: a dense 16x16 complex block on the matrix cores.  Per group e of
+// 8 registers (register bits 3.., one bit for 12-qubit tiles and two for
+// 13-qubit tiles), the wavefront's 32 columns (lane bits 0..4) are 32
 // vectors of 16 amplitudes: K-step s holds the 8 reals of a3 = s, a2 = lane
 // bit 5, (a0, a1) = register bits 0, 1.  Amplitudes (already scaled by
 // kMfmaScale) and the matrix are split into fp16 high and low parts (high =
@@ -447,11 +448,11 @@ __device__ __forceinline__ void reg_qphase(T (&v)[1 << EB], const RegOp<R>& op, 
 // order (a0, a2, a3).
 template <typename T, int EB>
 __device__ __forceinline__ void reg_mfma(T (&v)[1 << EB], const unsigned char* blob, int s0, int lane) {
-  if constexpr (nqpu_is_float2<T>::value && EB == 4) {
+  if constexpr (nqpu_is_float2<T>::value && (EB == 4 || EB == 5)) {
     const nqpu_half8* ap = reinterpret_cast<const nqpu_half8*>(blob + (size_t)s0 * kMfmaBlobBytes + lane * 64);
     const nqpu_half8 ah0 = ap[0], al0 = ap[1], ah1 = ap[2], al1 = ap[3];
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
+    for (int e = 0; e < (1 << (EB - 3)); ++e) {
       nqpu_half8 bh[2], bl[2];
 #pragma unroll
       for (int s = 0; s < 2; ++s)

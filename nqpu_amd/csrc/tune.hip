@@ -126,6 +126,8 @@ std::vector<Candidate> candidates(int n, int prec) {
     v.push_back({12, 5, NQPU_RUN_COMPILE | NQPU_RUN_FIXED_LOW, 1});
     v.push_back({12, 5, NQPU_RUN_COMPILE, 2});
     v.push_back({12, 6, NQPU_RUN_COMPILE, 1});
+    v.push_back({13, 5, NQPU_RUN_COMPILE, 1});
+    v.push_back({13, 5, NQPU_RUN_COMPILE, 2});
     v.push_back({11, 5, 0, 1});
     v.push_back({13, 5, 0, 1});
     if (small) v.push_back({10, 5, 0, 1});

@@ -101,25 +101,33 @@ def test_complex64_tile_kernels(oracle, tile, low, lds):
     assert infidelity(out, exp) < 1e-6
 
 
+@pytest.mark.parametrize("tile", [12, 13])
 @pytest.mark.parametrize("name", ["mixed16", "qft_trotter15", "brick14"])
-def test_compiled_passes_match_oracle(oracle, name):
+def test_compiled_passes_match_oracle(oracle, name, tile):
     """complex64 register tiles run as straight-line kernels compiled at run
     time (compile=True): the same register programs as the interpreting
-    kernel, every op kind, controls and diagonal factors outside the tile."""
+    kernel, every op kind, controls and diagonal factors outside the tile.
+    13-qubit tiles hold 32 amplitudes per thread: matrix-core blocks run over
+    four groups of 8 registers."""
     from tests.test_planner_emulation import CIRCUITS, random_state
     c = CIRCUITS[name]()
     n = c.num_qubits
     a0 = random_state(n, 5)
     exp = a0.copy()
     oracle.apply_circuit(exp, n, c.gates)
+    outs = []
     for compiled in (True, False):
         with StateVector(n, "c64") as sv:
             sv.set_amplitudes(a0)
-            st = sv.run(c, compile=compiled, tile_qubits=12)
+            st = sv.run(c, compile=compiled, tile_qubits=tile)
             out = sv.amplitudes()
         assert st["passes"] > 0
+        if tile == 13 and name != "mixed16":
+            assert st["mfma_blocks"] > 0
         assert np.max(np.abs(out - exp)) < 2e-5, compiled
         assert infidelity(out, exp) < 1e-6, compiled
+        outs.append(out)
+    assert np.array_equal(outs[0], outs[1])  # the same program, compiled or interpreted
 
 
 @pytest.mark.parametrize("n,tile,low,count", [(17, 12, 5, 200), (20, 11, 3, 90), (20, 13, 6, 150)])

@@ -8,11 +8,12 @@ from nqpu_amd.workloads import qft, tfim_trotter
 from tests.test_planner_emulation import CIRCUITS
 
 
+@pytest.mark.parametrize("tile", [12, 13])
 @pytest.mark.parametrize("name", ["mixed16", "qft_trotter15"])
-def test_generated_passes_compile(name):
+def test_generated_passes_compile(name, tile):
     # mixed16: every op kind, controls and diagonal factors outside the tile;
     # qft_trotter15: diagonal runs and matrix-core blocks with selector qubits
-    CIRCUITS[name]().native().check_compile()
+    CIRCUITS[name]().native().check_compile(tile_qubits=tile)
 
 
 def test_reference_semantics_and_vector_only_passes_compile():

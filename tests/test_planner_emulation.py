@@ -38,7 +38,7 @@ CIRCUITS = {
 
 
 @pytest.mark.parametrize("name", sorted(CIRCUITS))
-@pytest.mark.parametrize("tile,low", [(11, 3), (12, 5), (13, 6), (12, 0)])
+@pytest.mark.parametrize("tile,low", [(11, 3), (12, 5), (13, 6), (13, 5), (12, 0)])
 @pytest.mark.parametrize("lds", [False, True])
 def test_emulated_tiles_match_oracle(oracle, name, tile, low, lds):
     c = CIRCUITS[name]()
