@@ -44,14 +44,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(args):
+def measured_traffic(args, tile, compiled):
     """HBM bytes per launch of the dominant kernel, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs recorded in profiles/traffic.json (scaled by
-    the plain-copy calibration); None when no measurement matches this
-    kernel and workload shape."""
-    if args.lds or args.tile not in (0, 12) or args.low not in (0, 5) or args.qubits != 30 or args.depth != 40:
+    the plain-copy calibration) for the plan that ran (tile width, compiled or
+    interpreted passes); None when no measurement matches this kernel and
+    workload shape."""
+    if args.lds or args.low not in (0, 5) or args.qubits != 30 or args.depth != 40:
         return None
-    key = "k_tile_reg" if args.interpret else "nqpu_pass_compiled"  # the tuner's choice on MI355X
+    key = f"nqpu_pass_compiled_t{tile}" if compiled else f"k_tile_reg_t{tile}"
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             return json.load(f)[key]["bytes_per_launch"]
@@ -724,6 +725,8 @@ def main():
         tuned["tuning_s"] = time.perf_counter() - t0
         log(f"autotuned plan: {tuned}")
         run_kw = dict(fusion="tiled")
+    plan_tile = tuned["tile_qubits"] if tuned else (args.tile or 12)  # 12: the default for 30 qubits
+    plan_compiled = bool(tuned["compile"]) if tuned else bool(run_kw["compile"])
     log(f"workload n={n} depth={depth} gates={gates} plan={nc.plan('c64', **run_kw)}")
 
     sv = E.StateVector(n, "c64")
@@ -785,7 +788,7 @@ def main():
         # first execution of the circuit includes generating and compiling its pass kernels
         "first_run_s": warm_s[0] if warm_s else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args, plan_tile, plan_compiled),
                      "kernel": "k_tile" if args.lds else (
                          "nqpu_pass_* (compiled)" if (tuned["compile"] if tuned else run_kw["compile"])
                          else "k_tile_reg"),
