@@ -2829,9 +2829,13 @@ PlanOptions plan_options(const NqpuRunOptions& o, int prec, int n) {
   // too few 12- / 11-qubit tiles to fill 256 CUs, and its passes are latency
   // bound: one qubit less doubles the workgroups (random 20-qubit circuit of
   // depth 40 on MI355X: 2.1 instead of 3.1 ms complex64, 1.7 instead of
-  // 2.0 ms complex128).  Compiled complex64 passes keep 12-qubit tiles.
+  // 2.0 ms complex128).  Compiled complex64 passes keep 12-qubit tiles, and
+  // take 13-qubit tiles from 26 qubits on: fewer, heavier passes (30-qubit
+  // random brickwork of depth 40: 95.6 instead of 101.6 ms; 34-qubit QFT +
+  // 20 Trotter steps: 2.33 instead of 2.53 s).
   const bool c64 = prec == NQPU_PRECISION_C64, small = n > 0 && n <= (c64 ? 22 : 20);
-  const int dflt = c64 ? ((small && !(o.flags & NQPU_RUN_COMPILE)) ? 11 : 12) : (small ? 10 : 11);
+  const bool compiled = (o.flags & NQPU_RUN_COMPILE) != 0;
+  const int dflt = c64 ? ((small && !compiled) ? 11 : (compiled && n >= 26 ? 13 : 12)) : (small ? 10 : 11);
   po.tile_qubits = o.tile_qubits ? std::min<int>((int)o.tile_qubits, kmax) : dflt;
   if (po.tile_qubits < 5) po.tile_qubits = 5;
   po.low_qubits = o.low_qubits ? (int)o.low_qubits : (prec == NQPU_PRECISION_C64 ? 5 : 4);
