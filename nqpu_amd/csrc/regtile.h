@@ -634,6 +634,14 @@ __device__ __forceinline__ void reg_program(T (&v)[1 << EB], T* sm, const RegOp<
   }
 }
 
+// An address the compiler must treat as an unknown per-lane value (an empty
+// asm statement: no instruction), so that sums built from it are not
+// reassociated into per-register scalar constants.
+template <typename T>
+__device__ __forceinline__ void opaque_addr(T*& p) {
+  asm("" : "+v"(p));
+}
+
 // One register tile pass over the state: tile `tile` of 2^(8 + EB) amplitudes
 // goes to workgroup tile % gridDim.x; it is loaded in the slot order pa.ql,
 // scaled for matrix-core blocks if the pass has them, transformed by
@@ -666,13 +674,20 @@ __device__ __forceinline__ void reg_tile_pass(T* __restrict__ st, const PassArgs
         v[r].y = (R)0;
       }
     } else {
+      // register r's amplitude address: the address of r without its lowest
+      // set bit j, plus the stride of the qubit on register slot j (one
+      // 64-bit add each).  Left to itself the compiler keeps all 2^EB offsets
+      // as 64-bit scalars, which overflows the scalar registers into
+      // vector-lane spills; opaque() keeps each address a per-lane value.
+      T* ad[E];
+      ad[0] = st + (base | dlo);
+      opaque_addr(ad[0]);
+      v[0] = load_stream(ad[0]);
 #pragma unroll
-      for (int r = 0; r < E; ++r) {
-        uint64_t d = base | dlo;
-#pragma unroll
-        for (int j = 0; j < EB; ++j)
-          if ((r >> j) & 1) d |= 1ull << pa.ql[8 + j];
-        v[r] = load_stream(st + d);
+      for (int r = 1; r < E; ++r) {
+        ad[r] = ad[r & (r - 1)] + (1ull << pa.ql[8 + __builtin_ctz(r)]);
+        opaque_addr(ad[r]);
+        v[r] = load_stream(ad[r]);
       }
     }
     if (pa.scaled) {
@@ -694,13 +709,17 @@ __device__ __forceinline__ void reg_tile_pass(T* __restrict__ st, const PassArgs
         v[r].y *= (R)(1.0f / kMfmaScale);
       }
     }
+    {
+      T* ad[E];  // as for the loads, with the store positions
+      ad[0] = st + (base | dst);
+      opaque_addr(ad[0]);
+      store_stream(ad[0], v[0]);
 #pragma unroll
-    for (int r = 0; r < E; ++r) {
-      uint64_t d = base | dst;
-#pragma unroll
-      for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) d |= 1ull << pa.qf[8 + j];
-      store_stream(st + d, v[r]);
+      for (int r = 1; r < E; ++r) {
+        ad[r] = ad[r & (r - 1)] + (1ull << pa.qf[8 + __builtin_ctz(r)]);
+        opaque_addr(ad[r]);
+        store_stream(ad[r], v[r]);
+      }
     }
   }
 }
