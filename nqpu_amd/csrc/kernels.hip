@@ -586,17 +586,23 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, u
 #pragma unroll
           for (int j = 0; j < 8 + EB; ++j) base = ins0(base, P.q[j]);
           T v[E];
+          // register r's address: its parent's plus one stride (as in
+          // batch_gate_tiles)
+          T* ad[E];
+          ad[0] = st + (base | dlo);
+          opaque_addr(ad[0]);
+#pragma unroll
+          for (int r = 1; r < E; ++r) {
+            ad[r] = ad[r & (r - 1)] + (1ull << P.q[8 + __builtin_ctz(r)]);
+            opaque_addr(ad[r]);
+          }
 #pragma unroll
           for (int r = 0; r < E; ++r) {
-            uint64_t d = base | dlo;
-#pragma unroll
-            for (int j = 0; j < EB; ++j)
-              if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
-            if (p == 0) {  // the circuit starts from |0...0>
-              v[r].x = d == 0 ? 1.f : 0.f;
+            if (p == 0) {  // the circuit starts from |0...0>: amplitude 1 at index 0
+              v[r].x = (ad[r] == st) ? 1.f : 0.f;
               v[r].y = 0.f;
             } else {
-              v[r] = st[d];
+              v[r] = *ad[r];
             }
             if (P.scaled) {
               v[r].x *= kMfmaScale;
@@ -605,17 +611,20 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, u
           }
           reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true, base,
                                     blob + (size_t)P.blob_unit * kMfmaBlobBytes);
+          ad[0] = st + (base | dlo);
+          opaque_addr(ad[0]);
+#pragma unroll
+          for (int r = 1; r < E; ++r) {
+            ad[r] = ad[r & (r - 1)] + (1ull << P.q[8 + __builtin_ctz(r)]);
+            opaque_addr(ad[r]);
+          }
 #pragma unroll
           for (int r = 0; r < E; ++r) {
-            uint64_t d = base | dlo;
-#pragma unroll
-            for (int j = 0; j < EB; ++j)
-              if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
             if (P.scaled) {
               v[r].x *= 1.0f / kMfmaScale;
               v[r].y *= 1.0f / kMfmaScale;
             }
-            st[d] = v[r];
+            *ad[r] = v[r];
           }
         }
         __threadfence_block();
@@ -674,17 +683,23 @@ __device__ __forceinline__ void batch_gate_tiles(float2* __restrict__ st, const 
 #pragma unroll
     for (int j = 0; j < 8 + EB; ++j) base = ins0(base, P.q[j]);
     T v[E];
+    // register r's address: its parent's (r without its lowest set bit) plus
+    // one stride (per-lane values: no 2^EB scalar offsets)
+    T* ad[E];
+    ad[0] = st + (base | dlo);
+    opaque_addr(ad[0]);
+#pragma unroll
+    for (int r = 1; r < E; ++r) {
+      ad[r] = ad[r & (r - 1)] + (1ull << P.q[8 + __builtin_ctz(r)]);
+      opaque_addr(ad[r]);
+    }
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      uint64_t d = base | dlo;
-#pragma unroll
-      for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
-      if (first) {
-        v[r].x = d == 0 ? 1.f : 0.f;
+      if (first) {  // the circuit starts from |0...0>: amplitude 1 at index 0
+        v[r].x = (ad[r] == st) ? 1.f : 0.f;
         v[r].y = 0.f;
       } else {
-        v[r] = st[d];
+        v[r] = *ad[r];
       }
       if (P.scaled) {
         v[r].x *= kMfmaScale;
@@ -693,17 +708,21 @@ __device__ __forceinline__ void batch_gate_tiles(float2* __restrict__ st, const 
     }
     reg_program<float, T, EB>(v, sm, prog + P.op0, P.nops, tid, true, base,
                               blob + (size_t)P.blob_unit * kMfmaBlobBytes);
+    // the same addresses again for the stores (not kept across the program)
+    ad[0] = st + (base | dlo);
+    opaque_addr(ad[0]);
+#pragma unroll
+    for (int r = 1; r < E; ++r) {
+      ad[r] = ad[r & (r - 1)] + (1ull << P.q[8 + __builtin_ctz(r)]);
+      opaque_addr(ad[r]);
+    }
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      uint64_t d = base | dlo;
-#pragma unroll
-      for (int j = 0; j < EB; ++j)
-        if ((r >> j) & 1) d |= 1ull << P.q[8 + j];
       if (P.scaled) {
         v[r].x *= 1.0f / kMfmaScale;
         v[r].y *= 1.0f / kMfmaScale;
       }
-      st[d] = v[r];
+      *ad[r] = v[r];
     }
   }
 }
