@@ -199,3 +199,28 @@ def test_complex128_from_zero(native, expected, compile):
     x = a.amplitudes()
     assert np.array_equal(x, b.amplitudes())
     assert np.max(np.abs(x - expected)) < 1e-12
+
+
+def test_26_qubit_compiled_run_in_13_qubit_tiles(oracle):
+    """Compiled complex64 runs of 26 qubits and more use 13-qubit tiles by
+    default; the layout such a run leaves is restored by 13-qubit closing
+    passes (amplitudes against the oracle) or mapped (Pauli values equal to
+    the restored state's)."""
+    n = 26
+    c = random_u3_brickwork(n, 8, 31)
+    exp = oracle.run(n, c.gates)
+    nc = c.native()
+    a, b = E.StateVector(n, "c64"), E.StateVector(n, "c64")
+    st = a.run(nc, compile=True, from_zero=True)
+    b.run(nc, compile=True, from_zero=True)
+    assert st["passes"] > 0
+    moved = layout(a)
+    assert moved != list(range(n))
+    zz = [(0, 3 << i) for i in range(n - 1)] + ["X" + "I" * (n - 2) + "Z"]
+    va = a.expectation_pauli_values(zz)
+    assert layout(a) == moved
+    x = b.amplitudes()  # restores b's layout
+    assert layout(b) == list(range(n))
+    assert infidelity(x, exp) < 1e-6
+    vb = b.expectation_pauli_values(zz)
+    assert np.max(np.abs(va - vb)) < 1e-12
