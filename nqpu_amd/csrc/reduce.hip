@@ -58,22 +58,22 @@ __device__ __forceinline__ double wscan(double v, int lane) {
 // ------------------------------------------------------------ Pauli groups
 
 constexpr int kPauliGroup = 32;  // terms evaluated in one pass
+constexpr int kPauliKeys = 16;   // distinct accumulators of one pass
 constexpr int kPauliThreads = 256;
 
 struct PauliGroupArgs {
-  uint64_t x;    // X/Y mask shared by the group's terms (0: diagonal strings)
-  int32_t h;     // highest bit of x
-  int32_t nt;    // terms in this launch
-  uint32_t odd;  // bit t: term t has an odd number of Y factors (its sum is imaginary)
+  uint64_t x;     // X/Y mask shared by the group's terms (0: diagonal strings)
+  int32_t h;      // highest bit of x
+  int32_t nt;     // terms in this launch
+  int32_t nk;     // keys in this launch; key 0 is (no iteration bits, real part)
+  int32_t width;  // doubles per partial row (>= nt)
+  uint32_t kodd;  // bit j: key j accumulates Im(conj(a_j) a_i) (an odd number of Y factors)
   uint32_t pad;
-  uint64_t z[kPauliGroup];  // Z/Y mask of each term
+  uint64_t u[kPauliKeys];      // iteration bits of each key's Z mask
+  uint64_t zt[kPauliGroup];    // thread bits of each term's Z mask
+  uint32_t key[kPauliGroup];   // key of each term
 };
 
-// Per workgroup and term t: A_t = sum over the workgroup's indices of
-//   x == 0: (-1)^popc(k & z_t) |a_k|^2
-//   x != 0: (-1)^popc(i & z_t) w_t(i)  over pairs (i, j = i ^ x), bit h of i clear,
-//           w_t = Re(conj(a_j) a_i) (even Y count) or Im(conj(a_j) a_i) (odd)
-// <P_t> follows on the host: x == 0: A; else 2A with the i^#Y phase.
 // Per workgroup and term t: A_t = sum over the workgroup's indices of
 //   x == 0: (-1)^popc(k & z_t) |a_k|^2
 //   x != 0: (-1)^popc(i & z_t) w_t(i)  over pairs (i, j = i ^ x), bit h of i clear,
@@ -82,15 +82,20 @@ struct PauliGroupArgs {
 // With i = ins0(k, h) and zc_t = z_t with bit h removed, popc(i & z_t) =
 // popc(k & zc_t).  The grid is a power of two, so k = r * stride + base with
 // base = block * 256 + thread < stride: the sign splits into a per-thread
-// part (popc(base & zc_t), applied once at the end) and a part of the
-// iteration r, uniform over the grid (scalar).  Each term then costs one FP64
-// FMA per element; four loads are kept in flight.
-template <typename T, int NT>
+// part popc(base & zt_t), applied once at the end, and a part popc(r & u_t) of
+// the iteration r, uniform over the grid (scalar).  Terms with the same
+// iteration bits u_t and the same real/imaginary choice share one
+// accumulator (a key): the loop costs one FP64 FMA and one scalar parity per
+// key, not per term, and key 0 (no iteration bits: every string on the
+// qubits below the iteration bits) needs no sign.  The accumulator of a term
+// equals that of its key bit for bit, so the sums do not depend on how terms
+// are keyed.  Four loads are kept in flight.
+template <typename T, int NK>
 __global__ __launch_bounds__(kPauliThreads) void k_pauli_group(const T* __restrict__ st, uint64_t work,
                                                               PauliGroupArgs g, double* __restrict__ part) {
-  double acc[NT];
+  double acc[NK];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = 0.0;
+  for (int j = 0; j < NK; ++j) acc[j] = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * kPauliThreads;
   const uint64_t base = (uint64_t)blockIdx.x * kPauliThreads + threadIdx.x;
   const uint64_t hmask = (1ull << g.h) - 1;
@@ -106,10 +111,11 @@ __global__ __launch_bounds__(kPauliThreads) void k_pauli_group(const T* __restri
       w0 = (double)aj.x * ai.x + (double)aj.y * ai.y;  // Re(conj(a_j) a_i)
       w1 = (double)aj.x * ai.y - (double)aj.y * ai.x;  // Im(conj(a_j) a_i)
     }
+    acc[0] += w0;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const double sg = (__popcll(r0 & g.z[t]) & 1) ? -1.0 : 1.0;  // uniform over the grid
-      acc[t] = fma(((g.odd >> t) & 1) ? w1 : w0, sg, acc[t]);
+    for (int j = 1; j < NK; ++j) {
+      const double sg = (__popcll(r0 & g.u[j]) & 1) ? -1.0 : 1.0;  // uniform over the grid
+      acc[j] = fma(((g.kodd >> j) & 1) ? w1 : w0, sg, acc[j]);
     }
   };
   uint64_t r = 0;
@@ -120,20 +126,25 @@ __global__ __launch_bounds__(kPauliThreads) void k_pauli_group(const T* __restri
     one(r + 3 * stride);
   }
   for (; r < work; r += stride) one(r);
-  __shared__ double red[NT][kPauliThreads / 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // Each thread stages its keys in its own LDS column and reads them back
+  // by term (a uniform index), so no barrier is needed before the reads.
+  __shared__ double stage[NK][kPauliThreads];
+  __shared__ double red[kPauliGroup][kPauliThreads / 64];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const double v = (__popcll(base & g.z[t]) & 1) ? -acc[t] : acc[t];  // the per-thread part of the sign
+  for (int j = 0; j < NK; ++j) stage[j][threadIdx.x] = acc[j];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int t = 0; t < g.nt; ++t) {
+    const double a = stage[g.key[t]][threadIdx.x];
+    const double v = (__popcll(base & g.zt[t]) & 1) ? -a : a;  // the per-thread part of the sign
     const double w = wsum(v);
     if (lane == 0) red[t][wave] = w;
   }
   __syncthreads();
-  if ((int)threadIdx.x < NT) {
+  if ((int)threadIdx.x < g.nt) {
     double s = 0;
 #pragma unroll
     for (int w = 0; w < kPauliThreads / 64; ++w) s += red[threadIdx.x][w];
-    part[(uint64_t)blockIdx.x * NT + threadIdx.x] = s;
+    part[(uint64_t)blockIdx.x * g.width + threadIdx.x] = s;
   }
 }
 
@@ -149,6 +160,7 @@ __global__ __launch_bounds__(kPauliThreads) void k_pauli_group(const T* __restri
 // bits and the Z bits outside the tile (one parity per tile).
 
 constexpr int kPauliTileMaxBits = 12;
+constexpr int kPauliTileGroup = 16;  // strings of one tile pass
 
 struct PauliTileArgs {
   int32_t K, nt;
@@ -160,45 +172,132 @@ struct PauliTileArgs {
   uint32_t pad;
 };
 
+// A tile's amplitudes between HBM, registers and LDS: element tid + 256 r
+// of a tile of E amplitudes (E <= 256 R) is (vx[r], vy[r]) in registers.
+// Full tiles take a straight-line path with no per-element guard, so all
+// of a tile's loads are issued back to back.
+template <int R, typename T, typename C>
+__device__ __forceinline__ void tile_regs_load(const T* __restrict__ st, uint64_t b, const uint64_t (&dhi)[R],
+                                               C (&vx)[R], C (&vy)[R], int tid, int E) {
+  if (E == 256 * R) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const T a = st[b | dhi[r]];
+      vx[r] = a.x;
+      vy[r] = a.y;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (r * 256 + tid < E) {
+        const T a = st[b | dhi[r]];
+        vx[r] = a.x;
+        vy[r] = a.y;
+      }
+  }
+}
+
+template <int R, typename T, typename C>
+__device__ __forceinline__ void tile_regs_store(T* sm, const C (&vx)[R], const C (&vy)[R], int tid, int E) {
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (E == 256 * R || r * 256 + tid < E) {
+      T a;
+      a.x = vx[r];
+      a.y = vy[r];
+      sm[r * 256 + tid] = a;
+    }
+}
+
+// Sum over thread tid's pairs p = tid + 256 i of a tile of E amplitudes in
+// LDS of (-1)^popc(off_i & z) w(e0 | off_i, j0 ^ off_i), off_i = 256 i with a
+// zero inserted at bit h (hm = 2^h - 1), w = Re(conj(a_j) a_e), or Im for IM.
+// Straight-line over the pairs of a full tile (no index guard), so the LDS
+// reads of all pairs are in flight together.
+template <bool IM, int R, typename T>
+__device__ __forceinline__ double tile_pair_sum(const T* sm, uint32_t e0, uint32_t j0, uint32_t hm, uint32_t z,
+                                                int tid, int E) {
+  double s = 0.0;
+  auto pair = [&](int i) __attribute__((always_inline)) {
+    const uint32_t pi = 256u * i, off = ((pi & ~hm) << 1) | (pi & hm);
+    // +-1.0 built from its bits: a select of the two constants is turned
+    // into a branch between an add and a subtract
+    const double sg = __longlong_as_double(0x3FF0000000000000ll | ((long long)(__popc(off & z) & 1) << 63));
+    const T ai = sm[e0 | off], aj = sm[j0 ^ off];
+    const double w = IM ? (double)aj.x * ai.y - (double)aj.y * ai.x   // Im(conj(a_j) a_i)
+                        : (double)aj.x * ai.x + (double)aj.y * ai.y;  // Re(conj(a_j) a_i)
+    s = fma(w, sg, s);
+  };
+  if (E == 256 * R) {
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) pair(i);
+  } else {  // tiles of states below the largest tile: a short loop
+#pragma unroll 1
+    for (int i = 0; 256 * i < E / 2; ++i)
+      if (256 * i + tid < E / 2) pair(i);
+  }
+  return s;
+}
+
+// Each thread holds up to R amplitudes of a tile in registers (R * 256 =
+// the largest tile); all of a tile's loads are issued together, and the next
+// tile's loads are in flight while the current tile's pair sums run.
 template <typename T, int NT>
-__global__ __launch_bounds__(256) void k_pauli_tile(const T* __restrict__ st, uint64_t ntiles, PauliTileArgs g,
-                                                    double* __restrict__ part) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_pauli_tile(
+    const T* __restrict__ st, uint64_t ntiles, PauliTileArgs g, double* __restrict__ part) {
+  constexpr int R = (sizeof(T) == 8 ? (1 << kPauliTileMaxBits) : (1 << (kPauliTileMaxBits - 1))) / 256;
   extern __shared__ unsigned char smraw[];
   T* sm = reinterpret_cast<T*>(smraw);
   const int tid = threadIdx.x, E = 1 << g.K;
-  // tile element e = tid + 256 r sits at global index base | dlo | dhi(r)
+  // tile element e = tid + 256 r sits at global index base | dlo | dhi[r]
   uint64_t dlo = 0;
 #pragma unroll
   for (int j = 0; j < 8; ++j)
     if (j < g.K) dlo |= (uint64_t)((tid >> j) & 1) << g.q[j];
+  uint64_t dhi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    uint64_t d = 0;
+    for (int j = 8; j < g.K; ++j) d |= (uint64_t)((r >> (j - 8)) & 1) << g.q[j];
+    dhi[r] = d;
+  }
+  auto tile_base = [&](uint64_t tile) {
+    uint64_t b = tile;
+    for (int j = 0; j < g.K; ++j) b = ins0(b, g.q[j]);
+    return b;
+  };
+  // the amplitudes in flight as separate components (kept in registers)
+  using C = decltype(T{}.x);
+  C vx[R], vy[R];
   double acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = 0.0;
-  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    uint64_t base = tile;
-    for (int j = 0; j < g.K; ++j) base = ins0(base, g.q[j]);
-    for (int e0 = 0; e0 < E; e0 += 256) {
-      if (e0 + tid >= E) break;
-      uint64_t d = base | dlo;
-      for (int j = 8; j < g.K; ++j) d |= (uint64_t)((e0 >> j) & 1) << g.q[j];
-      sm[e0 + tid] = st[d];
-    }
+  uint64_t tile = blockIdx.x;
+  if (tile < ntiles) tile_regs_load<R>(st, tile_base(tile) | dlo, dhi, vx, vy, tid, E);
+  for (; tile < ntiles; tile += gridDim.x) {
+    tile_regs_store<R>(sm, vx, vy, tid, E);
     __syncthreads();
+    const uint64_t base = tile_base(tile);
+    if (tile + gridDim.x < ntiles) tile_regs_load<R>(st, tile_base(tile + gridDim.x) | dlo, dhi, vx, vy, tid, E);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      if (t >= g.nt) break;
-      const uint32_t x = g.xl[t], z = g.zl[t];
-      const int h = 31 - __clz(x);
-      const bool im = (g.odd >> t) & 1;
-      double s = 0.0;
-      for (int p = tid; p < E / 2; p += 256) {  // pairs (e, e ^ x) with bit h of e clear
-        const uint32_t e = (uint32_t)ins0((uint64_t)p, h), j = e ^ x;
-        const T ai = sm[e], aj = sm[j];
-        const double w = im ? (double)aj.x * ai.y - (double)aj.y * ai.x   // Im(conj(a_j) a_i)
-                            : (double)aj.x * ai.x + (double)aj.y * ai.y;  // Re(conj(a_j) a_i)
-        s += (__popc(e & z) & 1) ? -w : w;
+      if (t < g.nt) {
+        // Pair i of thread tid is p = tid + 256 i, its lower element e =
+        // p with a zero inserted at bit h (the highest X bit).  Inserting a
+        // bit maps bit positions one to one, so e = e0 | off_i with e0 =
+        // ins(tid) per thread and off_i = ins(256 i) uniform over the
+        // workgroup, on disjoint bits: the partner is (e0 ^ x) ^ off_i and
+        // the sign parity popc(e & z) = popc(e0 & z) + popc(off_i & z).
+        // The uniform part of the sign multiplies each pair's product; the
+        // per-thread part is applied once.
+        int h = 31 - __clz(g.xl[t]);
+        asm volatile("" : "+s"(h));  // recomputed per tile, not held for every string
+        const uint32_t x = g.xl[t], z = g.zl[t], hm = (1u << h) - 1;
+        const uint32_t e0 = (((uint32_t)tid & ~hm) << 1) | ((uint32_t)tid & hm), j0 = e0 ^ x;
+        const double s = ((g.odd >> t) & 1) ? tile_pair_sum<true, R>(sm, e0, j0, hm, z, tid, E)
+                                            : tile_pair_sum<false, R>(sm, e0, j0, hm, z, tid, E);
+        acc[t] += ((__popc(e0 & z) + __popcll(base & g.zg[t])) & 1) ? -s : s;
       }
-      acc[t] += (__popcll(base & g.zg[t]) & 1) ? -s : s;
     }
     __syncthreads();
   }
@@ -222,8 +321,7 @@ void launch_pauli_tile(const T* st, uint64_t ntiles, const PauliTileArgs& g, int
     case 2: hipLaunchKernelGGL((k_pauli_tile<T, 2>), gr, bl, lds, s, st, ntiles, g, part); break;
     case 4: hipLaunchKernelGGL((k_pauli_tile<T, 4>), gr, bl, lds, s, st, ntiles, g, part); break;
     case 8: hipLaunchKernelGGL((k_pauli_tile<T, 8>), gr, bl, lds, s, st, ntiles, g, part); break;
-    case 16: hipLaunchKernelGGL((k_pauli_tile<T, 16>), gr, bl, lds, s, st, ntiles, g, part); break;
-    default: hipLaunchKernelGGL((k_pauli_tile<T, 32>), gr, bl, lds, s, st, ntiles, g, part); break;
+    default: hipLaunchKernelGGL((k_pauli_tile<T, 16>), gr, bl, lds, s, st, ntiles, g, part); break;
   }
 }
 
@@ -253,14 +351,15 @@ int pauli_grid(uint64_t work) {
 template <typename T>
 void launch_pauli_group(const T* st, uint64_t work, const PauliGroupArgs& g, int grid, double* part,
                         hipStream_t s) {
-  const int nt = g.nt;
+  const int nk = g.nk;
   const dim3 gr(grid), bl(kPauliThreads);
-  if (nt <= 1) hipLaunchKernelGGL((k_pauli_group<T, 1>), gr, bl, 0, s, st, work, g, part);
-  else if (nt <= 2) hipLaunchKernelGGL((k_pauli_group<T, 2>), gr, bl, 0, s, st, work, g, part);
-  else if (nt <= 4) hipLaunchKernelGGL((k_pauli_group<T, 4>), gr, bl, 0, s, st, work, g, part);
-  else if (nt <= 8) hipLaunchKernelGGL((k_pauli_group<T, 8>), gr, bl, 0, s, st, work, g, part);
-  else if (nt <= 16) hipLaunchKernelGGL((k_pauli_group<T, 16>), gr, bl, 0, s, st, work, g, part);
-  else hipLaunchKernelGGL((k_pauli_group<T, 32>), gr, bl, 0, s, st, work, g, part);
+  if (nk <= 1) hipLaunchKernelGGL((k_pauli_group<T, 1>), gr, bl, 0, s, st, work, g, part);
+  else if (nk <= 2) hipLaunchKernelGGL((k_pauli_group<T, 2>), gr, bl, 0, s, st, work, g, part);
+  else if (nk <= 4) hipLaunchKernelGGL((k_pauli_group<T, 4>), gr, bl, 0, s, st, work, g, part);
+  else if (nk <= 6) hipLaunchKernelGGL((k_pauli_group<T, 6>), gr, bl, 0, s, st, work, g, part);
+  else if (nk <= 8) hipLaunchKernelGGL((k_pauli_group<T, 8>), gr, bl, 0, s, st, work, g, part);
+  else if (nk <= 12) hipLaunchKernelGGL((k_pauli_group<T, 12>), gr, bl, 0, s, st, work, g, part);
+  else hipLaunchKernelGGL((k_pauli_group<T, 16>), gr, bl, 0, s, st, work, g, part);
 }
 
 int pow2_at_least(int v) {
@@ -283,7 +382,7 @@ NqpuError pauli_launch(const State* s, const uint64_t* xm, const uint64_t* zm, u
     if (xm[a] != 0) {
       uint64_t qm = low, b = a, last = 0;
       int distinct = 0;
-      while (b < count && b - a < (uint64_t)kPauliGroup) {
+      while (b < count && b - a < (uint64_t)kPauliTileGroup) {
         if (xm[b] != last) {
           if (__builtin_popcountll(qm | xm[b]) > K) break;
           qm |= xm[b];
@@ -332,24 +431,43 @@ NqpuError pauli_launch(const State* s, const uint64_t* xm, const uint64_t* zm, u
         continue;
       }
     }
-    uint64_t b = a;
     const uint64_t x = xm[a];
-    while (b < count && xm[b] == x && b - a < (uint64_t)kPauliGroup) ++b;
     PauliGroupArgs g{};
     g.x = x;
     g.h = x ? 63 - __builtin_clzll(x) : 0;
-    g.nt = (int32_t)(b - a);
-    for (uint64_t t = a; t < b; ++t) {
-      // the kernel indexes pairs by k with i = k with a zero inserted at bit
-      // h, so it takes z with bit h removed
-      const uint64_t z = zm[t];
-      const uint64_t lo = g.h ? (z & ((1ull << g.h) - 1)) : 0;
-      g.z[t - a] = x ? (lo | ((z >> (g.h + 1)) << g.h)) : z;
-      if (__builtin_popcountll(x & z) & 1) g.odd |= 1u << (t - a);
-    }
     const uint64_t work = x ? (s->dim() >> 1) : s->dim();
     const int grid = pauli_grid(work);
+    // bits of the kernel's index k below the grid stride belong to the
+    // thread, the bits above it to the iteration
+    const uint64_t tmask = (uint64_t)grid * kPauliThreads - 1;
+    g.nk = 1;
+    uint64_t b = a;
+    while (b < count && xm[b] == x && b - a < (uint64_t)kPauliGroup) {
+      // the kernel indexes pairs by k with i = k with a zero inserted at bit
+      // h, so it takes z with bit h removed
+      const uint64_t z = zm[b];
+      const uint64_t lo = g.h ? (z & ((1ull << g.h) - 1)) : 0;
+      const uint64_t zc = x ? (lo | ((z >> (g.h + 1)) << g.h)) : z;
+      const uint32_t odd = __builtin_popcountll(x & z) & 1;
+      const uint64_t u = zc & ~tmask;
+      int j = 0;
+      if (u != 0 || odd) {
+        for (j = 1; j < g.nk && !(g.u[j] == u && ((g.kodd >> j) & 1) == odd); ++j) {
+        }
+        if (j == g.nk) {
+          if (g.nk == kPauliKeys) break;  // the next pass takes this term
+          g.u[j] = u;
+          g.kodd |= odd << j;
+          ++g.nk;
+        }
+      }
+      g.zt[b - a] = zc & tmask;
+      g.key[b - a] = (uint32_t)j;
+      ++b;
+    }
+    g.nt = (int32_t)(b - a);
     const int width = pow2_at_least(g.nt);
+    g.width = width;
     if (s->prec == NQPU_PRECISION_C64) launch_pauli_group((const float2*)s->amps, work, g, grid, part, s->stream);
     else launch_pauli_group((const double2*)s->amps, work, g, grid, part, s->stream);
     HIPTRY(hipGetLastError(), "pauli group launch");

@@ -283,3 +283,37 @@ def test_strings_with_different_x_masks_in_one_tile(oracle, n, prec, tol):
     ref_total = float(np.dot(cs, want))
     assert abs(total - ref_total) < 50 * tol
     assert abs(v.value - ref_total) < 50 * tol
+
+
+@pytest.mark.parametrize("prec,tol", [("c128", 1e-12), ("c64", 1e-6)])
+def test_group_with_more_accumulators_than_one_pass(oracle, prec, tol):
+    """32 strings sharing one X mask (X or Y on qubit 0) with Z factors on every
+    subset of qubits 16..19: at 20 qubits those are the pass's iteration bits,
+    so the strings need 32 distinct (iteration bits, real/imaginary)
+    accumulators, more than one pass holds, and the group is split over
+    passes.  Each value must be the oracle's; diagonal strings on the same
+    qubits (16 accumulators) ride along."""
+    n = 20
+    c = random_u3_brickwork(n, 8, 77)
+    exp = oracle.run(n, c.gates)
+    terms = []
+    for first in ("X", "Y"):
+        for sub in range(16):
+            s = ["I"] * n
+            s[0] = first
+            for b in range(4):
+                if (sub >> b) & 1:
+                    s[16 + b] = "Z"
+            terms.append("".join(s))
+    for sub in range(1, 16):
+        s = ["I"] * n
+        s[1] = "Z"
+        for b in range(4):
+            if (sub >> b) & 1:
+                s[16 + b] = "Z"
+        terms.append("".join(s))
+    with StateVector(n, prec) as sv:
+        sv.run(c)
+        got = sv.expectation_pauli_values(terms)
+    want = np.array([oracle.expectation_pauli(exp, n, *E.pauli_masks(p)) for p in terms])
+    assert np.max(np.abs(got - want)) < tol
