@@ -971,15 +971,38 @@ __global__ __launch_bounds__(kBlock) void k_zsum(const T* st, int cb, double* ou
   __shared__ double red[kBlock / 64];
   const uint64_t beg = (uint64_t)blockIdx.x << cb;
   const int nr = 1 << (cb - 8);
-  double tot = 0, hi[16];
+  double tot = 0, hi[16];  // hi[j]: rows of the chunk with bit j set
 #pragma unroll
   for (int j = 0; j < 16; ++j) hi[j] = 0;
-  for (int r = 0; r < nr; ++r) {
-    double p = nsq<R>(st[beg + threadIdx.x + ((uint64_t)r << 8)]);
-    tot += p;
+  if (nr >= 16) {
+    // 16 rows at a time, their loads issued together: bits 0..3 of a row
+    // are fixed by its place in the group, the higher bits are the same for
+    // the whole group (one add of the group's sum)
+    for (int r = 0; r < nr; r += 16) {
+      double p[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j)
-      if (j < cb - 8 && ((r >> j) & 1)) hi[j] += p;
+      for (int k = 0; k < 16; ++k) p[k] = nsq<R>(st[beg + threadIdx.x + ((uint64_t)(r + k) << 8)]);
+      double g = 0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        g += p[k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if ((k >> j) & 1) hi[j] += p[k];
+      }
+      tot += g;
+#pragma unroll
+      for (int j = 4; j < 16; ++j)
+        if (j < cb - 8 && ((r >> j) & 1)) hi[j] += g;
+    }
+  } else {
+    for (int r = 0; r < nr; ++r) {
+      const double p = nsq<R>(st[beg + threadIdx.x + ((uint64_t)r << 8)]);
+      tot += p;
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (j < cb - 8 && ((r >> j) & 1)) hi[j] += p;
+    }
   }
   double* row = out + (uint64_t)blockIdx.x * (cb + 1);
   double s = block_sum<kBlock>(tot, red);
