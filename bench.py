@@ -692,6 +692,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain state copy (k_copy) of known size, for counter calibration")
     args = ap.parse_args()
+    if args.per_circuit_sample < 1:
+        ap.error("--per-circuit-sample must be at least 1 (the batch leg compares against that many circuits)")
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the statevector path runs on a single MI355X)", file=sys.stderr)

@@ -3380,10 +3380,19 @@ CircuitPlan plan_batch_circuit(const Circuit& c, const std::vector<PauliTerm>& t
     cp.progs.push_back(std::move(pr));
   }
   // observable passes: each term on a tile set that holds its X bits
-  std::vector<std::vector<int>> sets(2);
+  // The first set is the lowest KH qubits; the second holds the qubits
+  // above it (at most KH - 4 of them, the highest) together with the lowest
+  // qubits, so a tile's lanes still load runs of at least 16 consecutive
+  // amplitudes (the top KH qubits alone would put lanes 2^(n-KH) apart).
+  std::vector<std::vector<int>> sets(1);
   for (int q = 0; q < KH; ++q) sets[0].push_back(q);
-  for (int q = n - KH; q < n; ++q) sets[1].push_back(q);
-  std::vector<std::vector<PauliTerm>> groups(2);
+  if (n > KH) {
+    const int hi = std::min(n - KH, KH - 4);
+    sets.emplace_back();
+    for (int q = 0; q < KH - hi; ++q) sets[1].push_back(q);
+    for (int q = n - hi; q < n; ++q) sets[1].push_back(q);
+  }
+  std::vector<std::vector<PauliTerm>> groups(sets.size());
   auto setmask = [](const std::vector<int>& s) {
     uint64_t m = 0;
     for (int q : s) m |= bit(q);

@@ -501,12 +501,85 @@ __device__ double tile_terms(const float2* sm, uint32_t TT, const BatchTerm* ter
   return acc;
 }
 
+// +-1.0 from a parity bit, built from its bits (a select of the two
+// constants feeding an FMA is turned into a branch between add and subtract)
+__device__ __forceinline__ double parity_sign(uint32_t odd) {
+  return __longlong_as_double(0x3FF0000000000000ll | ((long long)(odd & 1) << 63));
+}
+
+// tile_terms for a full 12-qubit tile (TT = 4096: rows r = 0..15 of 256).
+// A diagonal term's sign on element tid + 256 r splits into
+// popc(tid & zl) (per thread) and popc(r & zh), zh = zl >> 8 (uniform): the
+// row sum folds the 16 |a|^2 rows one bit of zh at a time, 15 FMAs with
+// uniform signs.  An X/Y term without Z factors in the tile sums the pairs
+// e = e0 | off_i, j = e ^ xl (e0 = ins(tid), off_i = ins(256 i), a zero
+// inserted at the highest X bit; disjoint bits), straight-line.  Other
+// terms take tile_terms' loop.
+__device__ double tile_terms_full(const float2* sm, const BatchTerm* terms, int nterm, uint64_t base, int tid) {
+  constexpr uint32_t TT = 4096;
+  double p[16];
+  bool have_p = false;
+  double acc = 0;
+  for (int k = 0; k < nterm; ++k) {
+    const BatchTerm tm = terms[k];
+    const double gs = (__popcll(base & tm.zg) & 1) ? -1.0 : 1.0;
+    double sr = 0, si = 0;
+    if (tm.xl == 0) {
+      if (!have_p) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float2 a = sm[tid + 256 * r];
+          p[r] = (double)a.x * a.x + (double)a.y * a.y;
+        }
+        have_p = true;
+      }
+      const uint32_t zh = tm.zl >> 8;
+      double q[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) q[r] = fma(p[2 * r + 1], parity_sign(zh), p[2 * r]);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q[r] = fma(q[2 * r + 1], parity_sign(zh >> 1), q[2 * r]);
+#pragma unroll
+      for (int r = 0; r < 2; ++r) q[r] = fma(q[2 * r + 1], parity_sign(zh >> 2), q[2 * r]);
+      q[0] = fma(q[1], parity_sign(zh >> 3), q[0]);
+      sr = (__popc((uint32_t)tid & tm.zl & 0xFFu) & 1) ? -q[0] : q[0];
+    } else if (tm.zl == 0) {
+      const int h = 31 - __clz(tm.xl);
+      const uint32_t hm = (1u << h) - 1, e0 = (((uint32_t)tid & ~hm) << 1) | ((uint32_t)tid & hm),
+                     j0 = e0 ^ tm.xl;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t pi = 256u * i, off = ((pi & ~hm) << 1) | (pi & hm);
+        const float2 ai = sm[e0 | off], aj = sm[j0 ^ off];
+        sr += (double)aj.x * ai.x + (double)aj.y * ai.y;
+      }
+      sr *= 2.0;
+    } else {
+      const int h = 31 - __clz(tm.xl);
+      for (uint32_t k2 = tid; k2 < (TT >> 1); k2 += 256) {
+        const uint32_t i = ins0u(k2, h), j = i ^ tm.xl;
+        const float2 ai = sm[i], aj = sm[j];
+        const double s_i = (__popc(i & tm.zl) & 1) ? -1.0 : 1.0;
+        const double s_j = (__popc(j & tm.zl) & 1) ? -1.0 : 1.0;
+        const double cr = (double)aj.x * ai.x + (double)aj.y * ai.y;
+        const double ci = (double)aj.x * ai.y - (double)aj.y * ai.x;
+        sr += (s_i + s_j) * cr;
+        si += (s_i - s_j) * ci;
+      }
+    }
+    const double val = tm.ny == 0 ? sr : (tm.ny == 1 ? -si : (tm.ny == 2 ? -sr : si));
+    acc += tm.coeff * gs * val;
+  }
+  return acc;
+}
+
 // Observable passes over 12-qubit tiles: the positions, relative to a tile's
 // base index, of this thread's 16 amplitudes t = tid + 256 r (tile bit j on
 // qubit q[j]), computed once per pass; then each tile is 16 independent
 // loads into LDS.
 constexpr int kObsTileQubits = 12;
 static_assert(kObsTileQubits == 8 + kBatchEB, "observable passes use the gate passes' tile width");
+static_assert(kObsTileQubits == 12, "tile_terms_full takes tiles of 4096 amplitudes");
 
 __device__ __forceinline__ void obs_offsets(const int32_t* q, int tid, uint64_t (&off)[16]) {
   uint64_t lo = 0;
@@ -642,7 +715,7 @@ __global__ __launch_bounds__(256, 1) void k_batch(float2* __restrict__ states, u
           for (int j = 0; j < kObsTileQubits; ++j) base = ins0(base, q[j]);
           obs_load_tile(st, base, off, sm, tid);
           __syncthreads();
-          acc += tile_terms(sm, 1u << kObsTileQubits, terms + P->term0, P->nterm, base, tid);
+          acc += tile_terms_full(sm, terms + P->term0, P->nterm, base, tid);
           __syncthreads();
         }
       }
@@ -768,7 +841,7 @@ __global__ __launch_bounds__(256) void k_batch_split_obs(const float2* __restric
     for (int j = 0; j < kObsTileQubits; ++j) base = ins0(base, q[j]);
     obs_load_tile(st, base, off, sm, tid);
     __syncthreads();
-    acc += tile_terms(sm, 1u << kObsTileQubits, terms + P->term0, P->nterm, base, tid);
+    acc += tile_terms_full(sm, terms + P->term0, P->nterm, base, tid);
     __syncthreads();
   }
   const double s = block_sum<256>(acc, red);
