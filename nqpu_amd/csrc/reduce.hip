@@ -3,7 +3,9 @@
 //  * Grouped Pauli sums: terms that share their X/Y mask share one pass over
 //    the state (all diagonal Z strings of a Hamiltonian in one pass; every
 //    term with the same flip pattern in one more), with one FP64 accumulator
-//    per term.  The reference evaluates each string with its own serial loop
+//    per distinct pattern of Z bits above the grid stride; strings with
+//    different X masks that fit one 12-qubit tile share an LDS tile pass.
+//    The reference evaluates each string with its own serial loop
 //    over all 2^n indices (expectation_pauli_string, sdk/rust/src/lib.rs:465-514;
 //    expectation_hamiltonian :518).
 //  * Sampling: sub-block masses in one coalesced pass (one wavefront per
