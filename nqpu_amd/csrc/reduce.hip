@@ -543,35 +543,35 @@ __global__ __launch_bounds__(256) void k_subsum_wave(const T* __restrict__ st, u
   }
 }
 
-// pre[c + 1] = sum of chunk sums 0..c, pre[0] = 0, where chunk c is the
-// 2^(cb - sb) sub-blocks c * 2^(cb - sb) ... (one workgroup of 1024 threads;
-// each thread owns a contiguous run of chunks).
-__global__ __launch_bounds__(1024) void k_chunk_prefix(const double* __restrict__ subs, uint64_t nch, int per_chunk,
-                                                       double* __restrict__ pre) {
-  __shared__ double tot[1024];
-  const int t = threadIdx.x;
+// pre[c + 1] = chunk c's sum of its per_chunk sub-block sums: one
+// wavefront per chunk, its lanes reading the sub-block sums coalesced.
+__global__ __launch_bounds__(256) void k_chunk_sums(const double* __restrict__ subs, uint64_t nch, int per_chunk,
+                                                    double* __restrict__ pre) {
+  const uint64_t c = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // the same for a whole wavefront
+  const int lane = threadIdx.x & 63;
+  if (c >= nch) return;
+  double s = 0;
+  for (int b = lane; b < per_chunk; b += 64) s += subs[c * per_chunk + b];
+  s = wsum(s);
+  if (lane == 0) pre[c + 1] = s;
+}
+
+// pre[c + 1] = sum of chunk sums 0..c (in place over k_chunk_sums' output),
+// pre[0] = 0: one workgroup of 1024 threads, each owning a contiguous run of
+// chunks; the runs' offsets come from wavefront scans.
+__global__ __launch_bounds__(1024) void k_chunk_prefix(uint64_t nch, double* __restrict__ pre) {
+  __shared__ double wtot[16];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint64_t m = (nch + 1023) / 1024;
   const uint64_t c0 = std::min<uint64_t>(nch, (uint64_t)t * m), c1 = std::min<uint64_t>(nch, c0 + m);
   double s = 0;
-  for (uint64_t c = c0; c < c1; ++c) {
-    double cs = 0;
-    for (int b = 0; b < per_chunk; ++b) cs += subs[c * per_chunk + b];
-    s += cs;
-    pre[c + 1] = cs;  // chunk sum for now
-  }
-  tot[t] = s;
+  for (uint64_t c = c0; c < c1; ++c) s += pre[c + 1];
+  const double inc = wscan(s, lane);
+  if (lane == 63) wtot[wave] = inc;
   __syncthreads();
-  if (t == 0) {
-    double run = 0;
-    for (int i = 0; i < 1024; ++i) {
-      const double v = tot[i];
-      tot[i] = run;
-      run += v;
-    }
-    pre[0] = 0;
-  }
-  __syncthreads();
-  double run = tot[t];
+  double run = inc - s;  // exclusive within the wavefront
+  for (int w = 0; w < wave; ++w) run += wtot[w];
+  if (t == 0) pre[0] = 0;
   for (uint64_t c = c0; c < c1; ++c) {
     run += pre[c + 1];
     pre[c + 1] = run;
@@ -870,8 +870,10 @@ NqpuError sample(const State* s0, const std::vector<double>& r, uint64_t* out) {
                        sb, dsub);
   }
   HIPTRY(hipGetLastError(), "sub-block sums");
-  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, s->stream, (const double*)dsub, nch, 1 << (cb - sb),
-                     dpre);
+  hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s->stream, (const double*)dsub, nch,
+                     1 << (cb - sb), dpre);
+  HIPTRY(hipGetLastError(), "chunk sums");
+  hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, s->stream, nch, dpre);
   HIPTRY(hipGetLastError(), "chunk prefix");
   if (s->prec == NQPU_PRECISION_C64) {
     hipLaunchKernelGGL(k_sample_wave<float2>, dim3(gshot), dim3(256), 0, s->stream, (const float2*)s->amps, n, cb,
